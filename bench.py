@@ -1,0 +1,256 @@
+"""Benchmark of the device LOBPCG on one MI355X.
+
+Workload (the 128^3 configuration): 3D 7-point variable-coefficient
+diffusion operator on a 128^3 grid (n = 2,097,152, nnz = 14,581,760),
+per-cell coefficients lognormal(0, 1) (seed 42, counter-based generator of
+blopex_amd.problems), harmonic face averages, Dirichlet boundary; Jacobi
+preconditioner; k = 32 smallest eigenpairs; X0 = 2*drand48()-1 after
+srand48(1), generated on the device; absolute residual tolerance 1e-6
+(relative 1e-50).
+
+A step is one full solve to tolerance in MIXED mode (FP32 blocks, FP64
+Gram accumulation / small dense / finish), starting from X0 already in HBM.
+`value` is its time-to-tolerance (s, lower is better).  The same solve in
+FP64 mode is timed once after the steps, and the JSON line reports the
+ratio.  The roofline object is the CSR SpMM kernel (HIP events around
+back-to-back launches on the library stream, k = 32 columns, FP32 -- the
+precision the mixed solve spends its iterations in); a profiled solve gives
+the per-kernel-class breakdown.  cpu_baseline is the reference's own
+lobpcg_solve_double (oracle/_ref) on this host, timed on a bounded sample.
+
+    python bench.py [--steps K] [--warmup W] [--n 128] [--k 32] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("time-to-residual 1e-6, k=32, 128³ Laplacian: mixed vs FP64 on GPU; SpMM GB/s")
+HBM_PEAK_GBS = 8000.0
+
+
+class HipEvents:
+    """hipEvent timing on an explicit stream (ctypes on libamdhip64)."""
+
+    def __init__(self):
+        self.hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
+        self.hip.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p,
+                                                 ctypes.c_void_p]
+        self.hip.hipEventDestroy.argtypes = [ctypes.c_void_p]
+
+    def event(self):
+        e = ctypes.c_void_p()
+        assert self.hip.hipEventCreate(ctypes.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.hip.hipEventRecord(e, stream) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.hip.hipEventSynchronize(b) == 0
+        ms = ctypes.c_float()
+        assert self.hip.hipEventElapsedTime(ctypes.byref(ms), a, b) == 0
+        return ms.value
+
+
+def spmm_roofline(lib, dA, A, k, prec, reps=30):
+    """Average duration of the CSR SpMM kernel over `reps` back-to-back launches."""
+    from blopex_amd import _lib as L
+    from blopex_amd.lobpcg import DeviceBlock
+
+    X = DeviceBlock(A.n, k, prec)
+    X.set_random(3)
+    Y = DeviceBlock(A.n, k, prec)
+    lib.HipCSRMatMultiVec.argtypes = [ctypes.c_void_p] * 3
+    call = lambda: lib.HipCSRMatMultiVec(ctypes.cast(dA.ptr, ctypes.c_void_p),
+                                         ctypes.cast(X.ptr, ctypes.c_void_p),
+                                         ctypes.cast(Y.ptr, ctypes.c_void_p))
+    for _ in range(3):
+        call()
+    lib.blopex_amd_synchronize()
+    ev = HipEvents()
+    s = lib.blopex_amd_stream()
+    a, b = ev.event(), ev.event()
+    ev.record(a, s)
+    for _ in range(reps):
+        call()
+    ev.record(b, s)
+    ms = ev.elapsed_ms(a, b) / reps
+    es = 4 if prec == L.HIPMV_FP32 else 8
+    algo = A.nnz * (es + 4) + (A.n + 1) * 4 + 2 * A.n * k * es
+    X.close()
+    Y.close()
+    return ms, algo
+
+
+def cpu_baseline(A, n_side, k, seed_x=1):
+    """Reference lobpcg_solve_double (oracle/_ref) on a bounded sample: the
+    64^3 instance of the same operator family (lognormal(0,1), Jacobi, k=32),
+    2 iterations, 1 thread; scaled to the 128^3 size by n (work per iteration
+    is linear in n) and to time-to-tolerance by the GPU FP64 iteration count."""
+    from blopex_amd import problems as pb
+    from oracle import pyoracle as po
+
+    if not po.ref_available():
+        return None, None
+    side = min(64, n_side)
+    As = pb.laplacian_7pt(side, side, side, sigma=1.0, seed=42)
+    X0 = pb.drand48_block(As.n, k, seed_x)
+    os.environ.setdefault("OPENBLAS_NUM_THREADS", "1")
+    t0 = time.perf_counter()
+    po.solve(A=As, X0=X0, tdiag=1.0 / As.diagonal(), maxit=1, atol=1e-6, which="ref")
+    t1 = time.perf_counter()
+    po.solve(A=As, X0=X0, tdiag=1.0 / As.diagonal(), maxit=3, atol=1e-6, which="ref")
+    t3 = time.perf_counter()
+    per_it = ((t3 - t1) - (t1 - t0)) / 2.0  # 2 extra iterations
+    scale = (n_side ** 3) / As.n
+    return per_it * scale, f"reference lobpcg_solve_double, {side}^3 k={k} Jacobi, iterations 2-3 " \
+                           f"({(t3 - t1) + (t1 - t0):.1f} s of CPU work), scaled by n x{scale:.0f}"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=128, help="grid side")
+    ap.add_argument("--k", type=int, default=32)
+    ap.add_argument("--maxit", type=int, default=3000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="one mixed solve and the SpMM launches only (for rocprofv3)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    from blopex_amd import _lib as L
+    from blopex_amd import problems as pb
+    from blopex_amd.lobpcg import DeviceBlock, DeviceCSR, DeviceDiag, solve_device
+
+    lib = L.load()
+    lib.blopex_amd_init()
+    nside, k = args.n, args.k
+    t = time.perf_counter()
+    A = pb.laplacian_7pt(nside, nside, nside, sigma=1.0, seed=42)
+    print(f"# matrix {nside}^3 n={A.n} nnz={A.nnz} built in {time.perf_counter() - t:.1f}s",
+          file=sys.stderr, flush=True)
+    dA = DeviceCSR(A)
+    dT = DeviceDiag(jacobi_of=dA)
+    X0 = DeviceBlock(A.n, k)
+    X0.set_random(1)
+    X = DeviceBlock(A.n, k)
+    nbytes = A.n * k * 8
+    hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
+    hip.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                                   ctypes.c_void_p]
+    xdata = lambda b: L.HipMultiVector.from_address(ctypes.addressof(b.ptr.contents)).data
+
+    def solve(mode, profile=False):
+        hip.hipMemcpyAsync(xdata(X), xdata(X0), nbytes, 3, lib.blopex_amd_stream())
+        return solve_device(dA, X, T=dT, tol=1e-6, maxit=args.maxit, mode=mode, profile=profile,
+                            histories=False)
+
+    def log(msg):
+        print(f"# {msg}", file=sys.stderr, flush=True)
+
+    if args.profile_only:
+        r = solve("mixed")
+        log(f"profile-only mixed solve: flag {r.flag} it {r.iterations} {r.stats['seconds']:.3f}s")
+        ms32, _ = spmm_roofline(lib, dA, A, k, L.HIPMV_FP32, reps=10)
+        ms64, _ = spmm_roofline(lib, dA, A, k, L.HIPMV_FP64, reps=10)
+        log(f"spmm fp32 {ms32 * 1e3:.1f} us  fp64 {ms64 * 1e3:.1f} us")
+        return
+
+    for w in range(args.warmup):
+        r = solve("mixed")
+        log(f"warmup {w}: flag {r.flag} iterations {r.iterations} "
+            f"(fp32 {r.stats['iterations_fp32']}, fp64 {r.stats['iterations_fp64']}) "
+            f"{r.stats['seconds']:.3f}s")
+    lib.blopex_amd_synchronize()
+    times, results = [], []
+    t_all = time.perf_counter()
+    for s in range(args.steps):
+        t0 = time.perf_counter()
+        r = solve("mixed")
+        lib.blopex_amd_synchronize()
+        times.append(time.perf_counter() - t0)
+        results.append(r)
+        log(f"step {s}: flag {r.flag} iterations {r.iterations} "
+            f"(fp32 {r.stats['iterations_fp32']}, fp64 {r.stats['iterations_fp64']}) {times[-1]:.3f}s")
+    lib.blopex_amd_synchronize()
+    total = time.perf_counter() - t_all
+    rm = results[-1]
+    lam_mixed = rm.lam.copy()
+
+    out = {"metric": METRIC, "value": float(np.mean(times)), "unit": "s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": 1e3 * total / args.steps, "higher_is_better": False,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+           "data": "synthetic (seeded lognormal coefficients, drand48 X0)",
+           "config": {"workload": f"LOBPCG k={k}, {nside}^3 7-point variable-coefficient "
+                                  f"diffusion (lognormal sigma=1), Jacobi, tol 1e-6, mixed precision",
+                      "n": A.n, "nnz": A.nnz, "k": k, "tol": 1e-6, "mode": "mixed"},
+           "mixed": {"flag": rm.flag, "iterations": rm.iterations,
+                     "iterations_fp32": rm.stats["iterations_fp32"],
+                     "iterations_fp64": rm.stats["iterations_fp64"],
+                     "max_residual": float(np.max(rm.res))}}
+
+    if not args.no_fp64:
+        t0 = time.perf_counter()
+        r64 = solve("fp64")
+        lib.blopex_amd_synchronize()
+        t64 = time.perf_counter() - t0
+        log(f"fp64: flag {r64.flag} iterations {r64.iterations} {t64:.3f}s")
+        out["fp64"] = {"seconds": t64, "flag": r64.flag, "iterations": r64.iterations,
+                       "max_residual": float(np.max(r64.res))}
+        out["speedup_mixed_vs_fp64"] = t64 / out["value"]
+        out["eig_rel_diff_mixed_vs_fp64"] = float(np.max(np.abs(lam_mixed - r64.lam) / np.abs(r64.lam)))
+
+    # per-class device time of one profiled mixed solve
+    rp = solve("mixed", profile=True)
+    st = rp.stats
+    out["profile_mixed"] = {kk: st[kk] for kk in ("seconds", "seconds_spmm", "seconds_gram",
+                                                  "seconds_update", "seconds_small_dense",
+                                                  "seconds_other", "spmm_calls")}
+    if st["seconds_spmm"] > 0:
+        out["profile_mixed"]["spmm_gbps_in_solve"] = st["spmm_bytes"] / st["seconds_spmm"] / 1e9
+
+    ms32, algo32 = spmm_roofline(lib, dA, A, k, L.HIPMV_FP32)
+    ms64, algo64 = spmm_roofline(lib, dA, A, k, L.HIPMV_FP64)
+    ach = algo32 / (ms32 * 1e-3) / 1e9
+    out["roofline"] = {"kernel": "k_spmm<float,8> (CSR SpMM, k=32, FP32)", "bound": "hbm",
+                       "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                       "algorithmic_bytes": algo32, "avg_launch_us": ms32 * 1e3}
+    out["spmm_fp64"] = {"achieved": algo64 / (ms64 * 1e-3) / 1e9, "avg_launch_us": ms64 * 1e3,
+                        "algorithmic_bytes": algo64}
+
+    if not args.no_cpu:
+        per_it, sample = cpu_baseline(A, nside, k)
+        if per_it is not None:
+            its = out.get("fp64", {}).get("iterations", rm.iterations)
+            out["cpu_baseline"] = {"value": per_it * its, "unit": "s", "cores": 1,
+                                   "kind": "reference", "seconds_per_iteration": per_it,
+                                   "sample": sample + f"; x {its} iterations (GPU FP64 count)"}
+    print(json.dumps(out), flush=True)
+    for o in (X, X0, dT, dA):
+        o.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,58 @@
+// lobpcg_solve_double / lobpcg_solve_complex entry points
+// (krylov/lobpcg.c:332-498): a block wrapped with the HIP interpreter runs
+// the device-resident solver, any other backend the generic algorithm with
+// the FortranMatrix routines of this library and the caller's LAPACK.
+#include <cstdio>
+
+#include "common.h"
+#include "lobpcg.h"
+
+extern "C" int blopex_amd_is_hip_interpreter(const mv_InterfaceInterpreter *i);
+extern "C" mv_InterfaceInterpreter *blopex_amd_mv_interpreter(mv_MultiVectorPtr x);
+extern "C" void blopex_amd_default_interpreters(lobpcg_Interpreter *lp,
+                                                utilities_FortranInterpreter *u);
+
+namespace bx {
+int device_lobpcg(mv_MultiVectorPtr X, void *Ad, void (*opA)(void *, void *, void *), void *Bd,
+                  void (*opB)(void *, void *, void *), void *Td,
+                  void (*opT)(void *, void *, void *), mv_MultiVectorPtr Y, lobpcg_Tolerance tol,
+                  int maxit, int verb, int *iters, double *lam, double *lamh, BlopexInt lamh_gh,
+                  double *res, double *resh, BlopexInt resh_gh);
+}
+
+extern "C" {
+
+int lobpcg_solve_double(mv_MultiVectorPtr X, void *Ad, void (*opA)(void *, void *, void *),
+                        void *Bd, void (*opB)(void *, void *, void *), void *Td,
+                        void (*opT)(void *, void *, void *), mv_MultiVectorPtr Y,
+                        lobpcg_BLASLAPACKFunctions blap, lobpcg_Tolerance tol, int maxit, int verb,
+                        int *iters, double *lam, double *lamh, BlopexInt lamh_gh, double *res,
+                        double *resh, BlopexInt resh_gh) {
+    if (blopex_amd_is_hip_interpreter(blopex_amd_mv_interpreter(X)))
+        return bx::device_lobpcg(X, Ad, opA, Bd, opB, Td, opT, Y, tol, maxit, verb, iters, lam,
+                                 lamh, lamh_gh, res, resh, resh_gh);
+    if (!blap.dpotrf || !blap.dsygv) {
+        bx::set_error("lobpcg_solve_double: dpotrf/dsygv are required for a host backend");
+        if (verb) fprintf(stderr, "Error in LOBPCG:\n%s\n", blopex_amd_last_error());
+        *iters = 0;
+        return REQUESTED_ACCURACY_NOT_ACHIEVED;
+    }
+    lobpcg_Interpreter lp;
+    utilities_FortranInterpreter u;
+    blopex_amd_default_interpreters(&lp, &u);
+    return lobpcg_solve(X, Ad, opA, Bd, opB, Td, opT, Y, blap, tol, maxit, verb, iters, lam, lamh,
+                        lamh_gh, res, resh, resh_gh, &lp, &u);
+}
+
+int lobpcg_solve_complex(mv_MultiVectorPtr, void *, void (*)(void *, void *, void *), void *,
+                         void (*)(void *, void *, void *), void *, void (*)(void *, void *, void *),
+                         mv_MultiVectorPtr, lobpcg_BLASLAPACKFunctions, lobpcg_Tolerance, int,
+                         int verb, int *iters, komplex *, komplex *, BlopexInt, double *, double *,
+                         BlopexInt) {
+    bx::set_error("lobpcg_solve_complex: complex Hermitian problems are not supported");
+    if (verb) fprintf(stderr, "Error in LOBPCG:\ncomplex Hermitian problems are not supported\n");
+    if (iters) *iters = 0;
+    return WRONG_BLOCK_SIZE;
+}
+
+}  // extern "C"

@@ -1,0 +1,617 @@
+// Device-resident LOBPCG: the steps of lobpcg_solve (krylov/lobpcg.c:
+// 502-1388) with every block in HBM and the FP64 small-dense work (Cholesky
+// of the Gram blocks, triangular inverse, Rayleigh-Ritz) on the GPU.  The host
+// keeps only what the reference keeps on the host anyway: the soft-locking
+// mask, the eigenvalue / residual histories and the exit bookkeeping; per
+// iteration it reads back k Ritz values, the active residual norms and a
+// status word.
+//
+// Block layout: X, AX, BX and P, AP, BP are n x k column-major; the residual
+// blocks R, AR, BR hold only the active columns, compacted in ascending
+// order, so every kernel runs over contiguous column ranges.  When the mask
+// shrinks, the surviving columns of R and P/AP/BP are gathered down.  Block
+// results are written to a spare buffer and the pointers swapped instead of
+// copying back (the reference's W/copy pairs).
+//
+// Precision: FP64 runs everything in FP64.  MIXED iterates with FP32 blocks
+// (FP64 Gram accumulation, FP64 small dense) until the FP32 residuals stop
+// improving or fall below switch_factor * atol, then restarts from the
+// current X in FP64 (re-orthonormalisation and Rayleigh-Ritz in FP64) and
+// finishes there, so the final residuals and Ritz values are FP64 ones.
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <utility>
+#include <vector>
+
+#include "dense.h"
+#include "kernels.h"
+#include "lobpcg.h"
+
+extern "C" int blopex_amd_is_hip_interpreter(const mv_InterfaceInterpreter *i);
+extern "C" mv_InterfaceInterpreter *blopex_amd_mv_interpreter(mv_MultiVectorPtr x);
+
+namespace bx {
+
+namespace {
+
+using Op = void (*)(void *, void *, void *);
+
+BlopexInt *identity_indices() {
+    static BlopexInt ids[kMaxCols];
+    static bool init = false;
+    if (!init) {
+        for (int i = 0; i < kMaxCols; i++) ids[i] = i;
+        init = true;
+    }
+    return ids;
+}
+
+hip_Multi_Vector view(void *p, long n, int w, int prec) {
+    hip_Multi_Vector v;
+    memset(&v, 0, sizeof(v));
+    v.data = p;
+    v.size = n;
+    v.num_vectors = w;
+    v.num_active_vectors = w;
+    v.active_indices = identity_indices();
+    v.precision = prec;
+    return v;
+}
+
+// Optional device timing of kernel classes: an event pair around each group
+// of launches, resolved after the stream has been synchronised.
+enum KClass { kSpmm, kGram, kUpdate, kDense, kOther, kNumClasses };
+
+class KTimer {
+  public:
+    explicit KTimer(bool on) : on_(on) {}
+    ~KTimer() {
+        for (hipEvent_t e : pool_) hipEventDestroy(e);
+    }
+    void begin(int cls) {
+        if (!on_) return;
+        cls_ = cls;
+        a_ = ev();
+        BX_HIP(hipEventRecord(a_, stream()));
+    }
+    void end(double bytes = 0.0) {
+        if (!on_) return;
+        hipEvent_t b = ev();
+        BX_HIP(hipEventRecord(b, stream()));
+        recs_.push_back({cls_, a_, b, bytes});
+    }
+    // call after a stream synchronisation
+    void resolve() {
+        if (!on_) return;
+        for (const Rec &r : recs_) {
+            float ms = 0.f;
+            BX_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+            acc_[r.cls] += ms * 1e-3;
+            if (r.cls == kSpmm) {
+                spmm_bytes_ += r.bytes;
+                spmm_calls_++;
+            }
+        }
+        recs_.clear();
+        used_ = 0;
+    }
+    void add_to(hip_LobpcgStats *st) {
+        if (!on_) return;
+        st->seconds_spmm += acc_[kSpmm];
+        st->seconds_gram += acc_[kGram];
+        st->seconds_update += acc_[kUpdate];
+        st->seconds_small_dense += acc_[kDense];
+        st->seconds_other += acc_[kOther];
+        st->spmm_bytes += spmm_bytes_;
+        st->spmm_calls += spmm_calls_;
+    }
+
+  private:
+    struct Rec {
+        int cls;
+        hipEvent_t a, b;
+        double bytes;
+    };
+    hipEvent_t ev() {
+        if (used_ == pool_.size()) {
+            hipEvent_t e;
+            BX_HIP(hipEventCreate(&e));
+            pool_.push_back(e);
+        }
+        return pool_[used_++];
+    }
+    bool on_;
+    int cls_ = 0;
+    hipEvent_t a_ = nullptr;
+    std::vector<hipEvent_t> pool_;
+    size_t used_ = 0;
+    std::vector<Rec> recs_;
+    double acc_[kNumClasses] = {};
+    double spmm_bytes_ = 0.0;
+    int spmm_calls_ = 0;
+};
+
+struct HostOut {
+    double *lam, *lamh, *res, *resh;
+    int lamh_gh, resh_gh;
+};
+
+class Phase {
+  public:
+    Phase(int prec, long n, int k, int ny, Op opA, void *Ad, Op opB, void *Bd, Op opT, void *Td,
+          const hip_Multi_Vector *Yuser, lobpcg_Tolerance tol, int verb, HostOut out, KTimer *tm)
+        : prec_(prec), n_(n), k_(k), ny_(ny), opA_(opA), Ad_(Ad), opB_(opB), Bd_(Bd), opT_(opT),
+          Td_(Td), Yu_(Yuser), tol_(tol), verb_(verb), out_(out), tm_(tm) {
+        noB_ = opB == nullptr;
+        noT_ = opT == nullptr;
+        es_ = elem_size(prec);
+        ld3_ = 3 * k;
+        const size_t blk = (size_t)n * k * es_;
+        int nblk = noB_ ? 7 : 10;
+        BX_HIP(hipMalloc(&pool_, blk * nblk + 256));
+        char *p = (char *)pool_;
+        for (int i = 0; i < nblk; i++) bufs_.push_back(p + i * blk);
+        X_ = bufs_[0];
+        AX_ = bufs_[1];
+        R_ = bufs_[2];
+        AR_ = bufs_[3];
+        P_ = bufs_[4];
+        AP_ = bufs_[5];
+        W_ = bufs_[6];
+        if (!noB_) {
+            BX_ = bufs_[7];
+            BR_ = bufs_[8];
+            BP_ = bufs_[9];
+        }
+        const size_t small = (size_t)ld3_ * ld3_ * 3 + (size_t)ld3_ * k * 2 + 8 * (size_t)k + 64;
+        BX_HIP(hipMalloc(&dsmall_, small * sizeof(double) + sizeof(DevStatus)));
+        GA_ = dsmall_;
+        GB_ = GA_ + (size_t)ld3_ * ld3_;
+        coef_ = GB_ + (size_t)ld3_ * ld3_;
+        lam_ = coef_ + (size_t)ld3_ * k;
+        lam_new_ = lam_ + k;
+        rn2_ = lam_new_ + k;
+        st_ = (DevStatus *)(rn2_ + 2 * k);
+        BX_HIP(hipMemsetAsync(dsmall_, 0, small * sizeof(double) + sizeof(DevStatus), stream()));
+        if (ny_ > 0) {
+            BX_HIP(hipMalloc(&ypool_, (size_t)n * ny_ * es_ * 2 + ((size_t)ny_ * ny_ + 2 * (size_t)ny_ * k) * 8));
+            Y_ = ypool_;
+            BY_ = noB_ ? Y_ : (char *)ypool_ + (size_t)n * ny_ * es_;
+            yby_ = (double *)((char *)ypool_ + (size_t)n * ny_ * es_ * 2);
+            ybx_ = yby_ + (size_t)ny_ * ny_;
+            tybx_ = ybx_ + (size_t)ny_ * k;
+        }
+    }
+    ~Phase() {
+        hipStreamSynchronize(stream());
+        if (pool_) hipFree(pool_);
+        if (dsmall_) hipFree(dsmall_);
+        if (ypool_) hipFree(ypool_);
+    }
+
+    // Returns the exit flag; *it_io is the iteration count so far (in) and
+    // after this phase (out).  stop_mixed: FP32 phase ending criterion active.
+    int run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_LobpcgSettings &set,
+            bool *switched);
+
+  private:
+    hip_Multi_Vector V(void *p, int w) { return view(p, n_, w, prec_); }
+    void apply(Op op, void *d, void *x, void *y, int w) {
+        hip_Multi_Vector vx = V(x, w), vy = V(y, w);
+        const bool isA = op == opA_ && d == Ad_;
+        double bytes = 0.0;
+        if (isA && op == HipCSRMatMultiVec) {
+            const hip_CSRMatrix *A = (const hip_CSRMatrix *)d;
+            bytes = (double)A->nnz * (es_ + 4) + (double)(A->n + 1) * 4 + 2.0 * n_ * w * es_;
+        }
+        tm_->begin(isA ? kSpmm : kOther);
+        op(d, &vx, &vy);
+        tm_->end(bytes);
+    }
+    void gram(const void *x, int wx, const void *y, int wy, double *g) {
+        tm_->begin(kGram);
+        gram_cols(prec_, n_, x, n_, colmap_identity(wx), y, n_, colmap_identity(wy), g, ld3_);
+        tm_->end();
+    }
+    // y = x * c (c rows h, cols w, leading dim ld3) [+ y]
+    void vecmat(const void *x, int h, const double *c, void *y, int w, bool acc = false) {
+        tm_->begin(kUpdate);
+        vecmat_cols(prec_, n_, x, n_, colmap_identity(h), c, ld3_, y, n_, colmap_identity(w), acc);
+        tm_->end();
+    }
+    void dense_chol(double *g, int m, int which) {
+        tm_->begin(kDense);
+        chol_inv(g, ld3_, m, st_, which);
+        tm_->end();
+    }
+    void dense_rr(int a, int p, int initial) {
+        tm_->begin(kDense);
+        rr_solve(GA_, GB_, ld3_, k_, a, p, initial, lam_, lam_new_, coef_, ld3_, st_);
+        tm_->end();
+    }
+    void read_status(DevStatus *h) {
+        download_sync(h, st_, sizeof(DevStatus));
+        tm_->resolve();
+    }
+    void gather(void *buf, const std::vector<int> &src) {
+        // in-place, one column per launch in ascending order (src[t] >= t)
+        for (size_t t = 0; t < src.size(); t++)
+            if (src[t] != (int)t) {
+                ColMap a = colmap_identity(1), b = colmap_identity(1);
+                a.idx[0] = src[t];
+                b.idx[0] = (int)t;
+                copy_cols(prec_, n_, buf, n_, a, buf, n_, b);
+            }
+    }
+    void project_constraints(void *blk, int w);
+    void store_hist(int col) {
+        if (out_.lamh)
+            for (int i = 0; i < k_; i++) out_.lamh[i + (size_t)col * out_.lamh_gh] = out_.lam[i];
+        if (out_.resh)
+            for (int i = 0; i < k_; i++) out_.resh[i + (size_t)col * out_.resh_gh] = out_.res[i];
+    }
+    void dump(int it) {
+        printf("\nEigenvalue lambda       Residual              \n");
+        for (int i = 0; i < k_; i++) printf("%22.14e  %22.14e\n", out_.lam[i], out_.res[i]);
+        printf("\n%d iterations\n", it);
+    }
+    double maxres() {
+        double m = out_.res[0];
+        for (int i = 1; i < k_; i++) m = out_.res[i] > m ? out_.res[i] : m;
+        return m;
+    }
+
+    int prec_;
+    long n_;
+    int k_, ny_;
+    Op opA_;
+    void *Ad_;
+    Op opB_;
+    void *Bd_;
+    Op opT_;
+    void *Td_;
+    const hip_Multi_Vector *Yu_;
+    lobpcg_Tolerance tol_;
+    int verb_;
+    HostOut out_;
+    KTimer *tm_;
+    bool noB_, noT_;
+    size_t es_;
+    int ld3_;
+    void *pool_ = nullptr, *ypool_ = nullptr;
+    std::vector<void *> bufs_;
+    void *X_, *AX_, *BX_ = nullptr, *R_, *AR_, *BR_ = nullptr, *P_, *AP_, *BP_ = nullptr, *W_;
+    void *Y_ = nullptr, *BY_ = nullptr;
+    double *dsmall_ = nullptr, *GA_, *GB_, *coef_, *lam_, *lam_new_, *rn2_;
+    double *yby_ = nullptr, *ybx_ = nullptr, *tybx_ = nullptr;
+    DevStatus *st_;
+};
+
+// blk(:, 0:w) -= Y (Y'BY)^{-1} (BY)' blk   (krylov/lobpcg.c:741-750, 996-1012)
+void Phase::project_constraints(void *blk, int w) {
+    gram_cols(prec_, n_, BY_, n_, colmap_identity(ny_), blk, n_, colmap_identity(w), ybx_, ny_);
+    small_mm(1.0, yby_, ny_, true, ybx_, ny_, false, tybx_, ny_, ny_, w, ny_);
+    small_mm(-1.0, yby_, ny_, false, tybx_, ny_, false, ybx_, ny_, ny_, w, ny_);
+    vecmat_cols(prec_, n_, Y_, n_, colmap_identity(ny_), ybx_, ny_, blk, n_, colmap_identity(w), true);
+}
+
+int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_LobpcgSettings &set,
+               bool *switched) {
+    *switched = false;
+    const int k = k_;
+    DevStatus hs;
+    hipStream_t s = stream();
+    // ---- working copy of X (and Y) ----
+    convert_block(n_, k, Xuser, n_, HIPMV_FP64, X_, n_, prec_);
+    if (ny_ > 0) {
+        convert_block(n_, ny_, Yu_->data, n_, Yu_->precision, Y_, n_, prec_);
+        if (!noB_) apply(opB_, Bd_, Y_, BY_, ny_);
+        gram_cols(prec_, n_, BY_, n_, colmap_identity(ny_), Y_, n_, colmap_identity(ny_), yby_, ny_);
+        chol_inv(yby_, ny_, ny_, st_, kCholX);
+        read_status(&hs);
+        if (hs.abort) {
+            if (verb_) printf("Cannot handle linear dependent constraints\n");
+            return WRONG_CONSTRAINTS;
+        }
+        project_constraints(X_, k);
+    }
+    int flag = 0;
+    // ---- initial Rayleigh-Ritz ----
+    if (!noB_) apply(opB_, Bd_, X_, BX_, k);
+    void *BXp = noB_ ? X_ : BX_;
+    gram(X_, k, BXp, k, GB_);
+    dense_chol(GB_, k, kCholX);
+    read_status(&hs);
+    if (hs.abort) {
+        flag = hs.x_info;
+        if (verb_) {
+            fprintf(stderr, "Error in LOBPCG:\nBad initial vectors: orthonormalization failed\n");
+            printf("DPOTRF INFO = %d\n", flag);
+        }
+    } else {
+        vecmat(X_, k, GB_, W_, k);
+        std::swap(X_, W_);
+        if (!noB_) {
+            vecmat(BX_, k, GB_, W_, k);
+            std::swap(BX_, W_);
+        }
+        BXp = noB_ ? X_ : BX_;
+        apply(opA_, Ad_, X_, AX_, k);
+        gram(X_, k, AX_, k, GA_);
+        gram(X_, k, BXp, k, GB_);
+        dense_rr(0, 0, 1);
+        read_status(&hs);
+        if (hs.abort) {
+            flag = hs.gevp_info;
+            if (verb_) {
+                fprintf(stderr,
+                        "Error in LOBPCG:\nBad problem: Rayleigh-Ritz in the initial subspace failed\n");
+                printf("DSYGV INFO = %d\n", flag);
+            }
+        } else {
+            std::swap(lam_, lam_new_);
+            vecmat(X_, k, coef_, W_, k);
+            std::swap(X_, W_);
+            vecmat(AX_, k, coef_, W_, k);
+            std::swap(AX_, W_);
+            if (!noB_) {
+                vecmat(BX_, k, coef_, W_, k);
+                std::swap(BX_, W_);
+            }
+            BXp = noB_ ? X_ : BX_;
+            ColMap all = colmap_identity(k);
+            tm_->begin(kOther);
+            residual_cols(prec_, n_, BXp, AX_, n_, all, lam_, R_, n_);
+            diagdot_cols(prec_, n_, R_, n_, all, R_, n_, all, rn2_);
+            tm_->end();
+            std::vector<double> rn(k);
+            download_sync(out_.lam, lam_, sizeof(double) * k);
+            download_sync(rn.data(), rn2_, sizeof(double) * k);
+            for (int i = 0; i < k; i++) out_.res[i] = sqrt(fabs(rn[i]));
+            store_hist(*it_io);
+            if (verb_ == 2)
+                dump(*it_io);
+            else if (verb_ == 1)
+                printf("\nInitial Max. Residual %22.14e\n", maxres());
+        }
+    }
+
+    // ---- main loop ----
+    std::vector<int> mask(k, 1), act, prev_act(k);
+    for (int i = 0; i < k; i++) prev_act[i] = i;
+    int local_it = 0;
+    double best = maxres();
+    int since_best = 0;
+    int it = *it_io + 1;
+    for (; flag == 0 && it <= maxit; it++) {
+        act.clear();
+        for (int i = 0; i < k; i++) {
+            mask[i] = out_.res[i] > fabs(out_.lam[i]) * tol_.relative + tol_.absolute + DBL_EPSILON;
+            if (mask[i]) act.push_back(i);
+        }
+        const int nr = (int)act.size();
+        if (fp32_phase) {
+            // leave FP32 on convergence, near the tolerance, or on a stall
+            const double mr = maxres();
+            if (mr < best * 0.9) {
+                best = mr;
+                since_best = 0;
+            } else {
+                since_best++;
+            }
+            if (nr == 0 || mr < set.switch_factor * tol_.absolute ||
+                since_best >= set.stall_iterations) {
+                *switched = true;
+                break;
+            }
+        }
+        if (nr < 1) break;
+        // compact R (laid out by prev_act) and P/AP/BP (full width) to act
+        if (act != prev_act) {
+            std::vector<int> src;
+            size_t q = 0;
+            for (int c : act) {
+                while (prev_act[q] != c) q++;
+                src.push_back((int)q);
+            }
+            gather(R_, src);
+        }
+        if (local_it > 0 && nr < k) {
+            gather(P_, act);
+            gather(AP_, act);
+            if (!noB_) gather(BP_, act);
+        }
+        if (!noT_) {
+            apply(opT_, Td_, R_, W_, nr);
+            std::swap(R_, W_);
+        }
+        if (ny_ > 0) project_constraints(R_, nr);
+        const int r0 = k, p0 = k + nr;
+        if (!noB_) apply(opB_, Bd_, R_, BR_, nr);
+        void *BRp = noB_ ? R_ : BR_;
+        double *ur = GB_ + r0 + (size_t)r0 * ld3_;
+        gram(R_, nr, BRp, nr, ur);
+        dense_chol(ur, nr, kCholR);
+        vecmat(R_, nr, ur, W_, nr);
+        std::swap(R_, W_);
+        if (!noB_) {
+            vecmat(BR_, nr, ur, W_, nr);
+            std::swap(BR_, W_);
+        }
+        BRp = noB_ ? R_ : BR_;
+        apply(opA_, Ad_, R_, AR_, nr);
+        const int preq = local_it > 0 ? nr : 0;
+        if (preq) {
+            void *BPp = noB_ ? P_ : BP_;
+            double *up = GB_ + p0 + (size_t)p0 * ld3_;
+            gram(P_, nr, BPp, nr, up);
+            dense_chol(up, nr, kCholP);
+            vecmat(P_, nr, up, W_, nr);
+            std::swap(P_, W_);
+            if (!noB_) {
+                vecmat(BP_, nr, up, W_, nr);
+                std::swap(BP_, W_);
+            }
+            vecmat(AP_, nr, up, W_, nr);
+            std::swap(AP_, W_);
+        }
+        BXp = noB_ ? X_ : BX_;
+        gram(R_, nr, AX_, k, GA_ + r0);
+        gram(R_, nr, AR_, nr, GA_ + r0 + (size_t)r0 * ld3_);
+        gram(R_, nr, BXp, k, GB_ + r0);
+        if (preq) {
+            void *BPp = noB_ ? P_ : BP_;
+            gram(P_, nr, AX_, k, GA_ + p0);
+            gram(P_, nr, AR_, nr, GA_ + p0 + (size_t)r0 * ld3_);
+            gram(P_, nr, AP_, nr, GA_ + p0 + (size_t)p0 * ld3_);
+            gram(P_, nr, BXp, k, GB_ + p0);
+            gram(P_, nr, BRp, nr, GB_ + p0 + (size_t)r0 * ld3_);
+            (void)BPp;
+        }
+        dense_rr(nr, preq, 0);
+        read_status(&hs);
+        if (hs.abort) {
+            if (hs.r_info) {
+                flag = hs.r_info;
+                if (verb_) {
+                    fprintf(stderr, "Error in LOBPCG:\nOrthonormalization of residuals failed\n");
+                    printf("DPOTRF INFO = %d\n", flag);
+                }
+            } else {
+                flag = hs.gevp_info ? hs.gevp_info : 1;
+                if (verb_) {
+                    fprintf(stderr, "Error in LOBPCG:\nGEVP solver failure\n");
+                    printf("INFO = %d\n", flag);
+                }
+                it--;
+            }
+            break;
+        }
+        const int np = (preq && hs.p_info == 0) ? nr : 0;
+        std::swap(lam_, lam_new_);
+        const double *cX = coef_, *cR = coef_ + r0, *cP = coef_ + p0;
+        // P = P cP + R cR (all k columns), likewise AP, BP
+        auto dir = [&](void *&Pb, void *Rb) {
+            if (np) {
+                vecmat(Pb, np, cP, W_, k);
+                vecmat(Rb, nr, cR, W_, k, true);
+            } else {
+                vecmat(Rb, nr, cR, W_, k);
+            }
+            std::swap(Pb, W_);
+        };
+        dir(P_, R_);
+        dir(AP_, AR_);
+        if (!noB_) dir(BP_, BR_);
+        // X = X cX + P, likewise AX, BX
+        auto upd = [&](void *&Xb, void *Pb) {
+            vecmat(Xb, k, cX, W_, k);
+            tm_->begin(kUpdate);
+            axpy_cols(prec_, n_, 1.0, Pb, n_, colmap_identity(k), W_, n_, colmap_identity(k));
+            tm_->end();
+            std::swap(Xb, W_);
+        };
+        upd(X_, P_);
+        upd(AX_, AP_);
+        if (!noB_) upd(BX_, BP_);
+        BXp = noB_ ? X_ : BX_;
+        ColMap am = colmap_identity(nr);
+        for (int t = 0; t < nr; t++) am.idx[t] = act[t];
+        tm_->begin(kOther);
+        residual_cols(prec_, n_, BXp, AX_, n_, am, lam_, R_, n_);
+        diagdot_cols(prec_, n_, R_, n_, colmap_identity(nr), R_, n_, colmap_identity(nr), rn2_);
+        tm_->end();
+        std::vector<double> rn(nr);
+        download_sync(out_.lam, lam_, sizeof(double) * k);
+        download_sync(rn.data(), rn2_, sizeof(double) * nr);
+        tm_->resolve();
+        for (int t = 0; t < nr; t++) out_.res[act[t]] = sqrt(fabs(rn[t]));
+        store_hist(it);
+        if (verb_ == 2) {
+            printf("Iteration %d \tbsize %d\n", it, nr);
+            dump(it);
+        } else if (verb_ == 1) {
+            printf("Iteration %d \tbsize %d \tmaxres %22.14e\n", it, nr, maxres());
+        }
+        prev_act = act;
+        local_it++;
+    }
+    *it_io = it - 1;
+    // ---- copy the block back ----
+    convert_block(n_, k, X_, n_, prec_, Xuser, n_, HIPMV_FP64);
+    BX_HIP(hipStreamSynchronize(s));
+    if (*switched) return 0;
+    if (flag != 0 || it > maxit) return REQUESTED_ACCURACY_NOT_ACHIEVED;
+    return 0;
+}
+
+}  // namespace
+
+int device_lobpcg(mv_MultiVectorPtr X, void *Ad, Op opA, void *Bd, Op opB, void *Td, Op opT,
+                  mv_MultiVectorPtr Y, lobpcg_Tolerance tol, int maxit, int verb, int *iters,
+                  double *lam, double *lamh, BlopexInt lamh_gh, double *res, double *resh,
+                  BlopexInt resh_gh) {
+    auto t0 = std::chrono::steady_clock::now();
+    hip_Multi_Vector *x = (hip_Multi_Vector *)mv_MultiVectorGetData(X);
+    *iters = 0;
+    const int k = x->num_vectors;
+    if (k < 1) {
+        if (verb) fprintf(stderr, "Error in LOBPCG:\nThe bloc size is wrong.\n");
+        return WRONG_BLOCK_SIZE;
+    }
+    BX_REQUIRE(x->precision == HIPMV_FP64, "lobpcg_solve_double: X must be an FP64 block");
+    BX_REQUIRE(3 * k <= kMaxRR, "device solver: block size %d above %d", k, kMaxRR / 3);
+    hip_Multi_Vector *y = nullptr;
+    int ny = 0;
+    if (Y) {
+        BX_REQUIRE(blopex_amd_is_hip_interpreter(blopex_amd_mv_interpreter(Y)),
+                   "device solver: constraints must be a hip_Multi_Vector");
+        y = (hip_Multi_Vector *)mv_MultiVectorGetData(Y);
+        ny = y->num_vectors;
+        BX_REQUIRE(y->size == x->size, "constraints have %ld rows, X %ld", y->size, x->size);
+    }
+    const hip_LobpcgSettings set = x->settings;
+    memset(&x->stats, 0, sizeof(x->stats));
+    HostOut out{lam, lamh, res, resh, (int)lamh_gh, (int)resh_gh};
+    if (verb) {
+        printf("\nSolving %s eigenvalue problem with%s preconditioning\n\n",
+               opB ? "generalized" : "standard", opT ? "" : "out");
+        printf("block size %d\n\n", k);
+        if (ny == 0)
+            printf("No constraints\n\n");
+        else
+            printf("%d constraint%s\n\n", ny, ny > 1 ? "s" : "");
+    }
+    int it = 0, flag = 0;
+    bool switched = false;
+    KTimer tm(set.profile != 0);
+    if (set.mode == HIPLOBPCG_MIXED) {
+        Phase ph(HIPMV_FP32, x->size, k, ny, opA, Ad, opB, Bd, opT, Td, y, tol, verb, out, &tm);
+        flag = ph.run(x->data, &it, maxit, true, set, &switched);
+        x->stats.iterations_fp32 = it;
+    }
+    if (set.mode != HIPLOBPCG_MIXED || (switched && flag == 0)) {
+        const int it0 = it;
+        Phase ph(HIPMV_FP64, x->size, k, ny, opA, Ad, opB, Bd, opT, Td, y, tol, verb, out, &tm);
+        bool dummy;
+        flag = ph.run(x->data, &it, maxit, false, set, &dummy);
+        x->stats.iterations_fp64 = it - it0;
+    }
+    *iters = it;
+    if (verb == 1) {
+        printf("\nEigenvalue lambda       Residual              \n");
+        for (int i = 0; i < k; i++) printf("%22.14e  %22.14e\n", lam[i], res[i]);
+        printf("\n%d iterations\n", it);
+    }
+    BX_HIP(hipStreamSynchronize(stream()));
+    tm.resolve();
+    tm.add_to(&x->stats);
+    x->stats.exit_flag = flag;
+    x->stats.seconds =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return flag;
+}
+
+}  // namespace bx

@@ -1,0 +1,182 @@
+"""Python host side of the device LOBPCG, mirroring the BLOPEX harness contract.
+
+lobpcg(X, A, B, T, Y, tol, maxit, verbosity) -> X, lambda, failureFlag,
+lambdaHistory, residualNormsHistory, as blopex_matlab.m:279-287 returns them
+(blopex_matlab/driver/blopex_matlab.m, gateway blopex_matlab_gateway.c:
+64-67, 207-210), with the relative tolerance fixed at 1e-50 as the gateway
+does (:66).  Everything runs through the C ABI of libblopex_amd.so:
+lobpcg_solve_double on a hip_Multi_Vector wrapped by HipSetupInterpreter,
+with HipCSRMatMultiVec / HipDiagMatMultiVec as the operator callbacks.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+from .problems import CSR
+
+
+@dataclass
+class Result:
+    X: np.ndarray | None
+    lam: np.ndarray
+    flag: int
+    iterations: int
+    lam_hist: np.ndarray
+    res_hist: np.ndarray
+    res: np.ndarray
+    stats: dict = field(default_factory=dict)
+
+
+class DeviceCSR:
+    def __init__(self, A: CSR):
+        self.lib = L.load()
+        self.n = A.n
+        self.ptr = self.lib.hip_CSRMatrixCreate(A.n, A.nnz, A.rowptr.ctypes.data,
+                                                A.colind.ctypes.data, A.val.ctypes.data)
+        self.op = L.symbol_address("HipCSRMatMultiVec")
+
+    def close(self):
+        if self.ptr:
+            self.lib.hip_CSRMatrixDestroy(self.ptr)
+            self.ptr = None
+
+
+class DeviceDiag:
+    def __init__(self, d: np.ndarray | None = None, jacobi_of: DeviceCSR | None = None):
+        self.lib = L.load()
+        if jacobi_of is not None:
+            self.ptr = self.lib.hip_DiagMatrixCreateJacobi(jacobi_of.ptr)
+        else:
+            d = np.ascontiguousarray(d, dtype=np.float64)
+            self.ptr = self.lib.hip_DiagMatrixCreate(len(d), d.ctypes.data)
+        self.op = L.symbol_address("HipDiagMatMultiVec")
+
+    def close(self):
+        if self.ptr:
+            self.lib.hip_DiagMatrixDestroy(self.ptr)
+            self.ptr = None
+
+
+class DeviceBlock:
+    """hip_Multi_Vector (n x k, FP64 unless precision says otherwise)."""
+
+    def __init__(self, n: int, k: int, precision: int = L.HIPMV_FP64):
+        self.lib = L.load()
+        self.n, self.k = n, k
+        self.ptr = self.lib.hip_Multi_VectorCreate(n, k, precision)
+
+    @classmethod
+    def from_numpy(cls, X: np.ndarray, precision: int = L.HIPMV_FP64) -> "DeviceBlock":
+        b = cls(X.shape[0], X.shape[1], precision)
+        Xf = np.asfortranarray(X, dtype=np.float64)
+        b.lib.hip_Multi_VectorUpload(b.ptr, Xf.ctypes.data)
+        return b
+
+    def to_numpy(self) -> np.ndarray:
+        out = np.zeros((self.n, self.k), order="F")
+        self.lib.hip_Multi_VectorDownload(self.ptr, out.ctypes.data)
+        return out
+
+    def set_random(self, seed: int) -> None:
+        self.lib.hip_Multi_VectorSetRandomValues(self.ptr, seed)
+
+    def close(self):
+        if self.ptr:
+            self.lib.hip_Multi_VectorDestroy(self.ptr)
+            self.ptr = None
+
+
+def interpreter() -> L.Interpreter:
+    ii = L.Interpreter()
+    L.load().HipSetupInterpreter(ctypes.byref(ii))
+    return ii
+
+
+def solve_device(A: DeviceCSR, X: DeviceBlock, B=None, T=None, Y: DeviceBlock | None = None,
+                 tol: float = 1e-6, rtol: float = 1e-50, maxit: int = 200, verbosity: int = 0,
+                 mode: str = "fp64", switch_factor: float = 4.0, stall_iterations: int = 8,
+                 refine_iterations: int = 0, histories: bool = True, profile: bool = False) -> Result:
+    """lobpcg_solve_double on device-resident operands; X is overwritten."""
+    lib = L.load()
+    k = X.k
+    s = L.Settings(L.HIPLOBPCG_MIXED if mode == "mixed" else L.HIPLOBPCG_FP64, switch_factor,
+                   stall_iterations, refine_iterations, int(profile))
+    lib.hip_Multi_VectorSetSettings(X.ptr, ctypes.byref(s))
+    ii = interpreter()
+    xx = lib.mv_MultiVectorWrap(ctypes.byref(ii), ctypes.cast(X.ptr, ctypes.c_void_p), 0)
+    yy = None
+    if Y is not None:
+        yy = lib.mv_MultiVectorWrap(ctypes.byref(ii), ctypes.cast(Y.ptr, ctypes.c_void_p), 0)
+    lam = np.zeros(k)
+    res = np.zeros(k)
+    lh = np.zeros((k, maxit + 1), order="F") if histories else None
+    rh = np.zeros((k, maxit + 1), order="F") if histories else None
+    it = ctypes.c_int(0)
+    flag = lib.lobpcg_solve_double(
+        xx, ctypes.cast(A.ptr, ctypes.c_void_p), A.op,
+        None if B is None else ctypes.cast(B.ptr, ctypes.c_void_p), None if B is None else B.op,
+        None if T is None else ctypes.cast(T.ptr, ctypes.c_void_p), None if T is None else T.op,
+        yy, L.BlasLapack(), L.Tolerance(tol, rtol), maxit, verbosity, ctypes.byref(it),
+        lam.ctypes.data, None if lh is None else lh.ctypes.data, k, res.ctypes.data,
+        None if rh is None else rh.ctypes.data, k)
+    st = L.Stats()
+    lib.hip_Multi_VectorGetStats(X.ptr, ctypes.byref(st))
+    lib.mv_MultiVectorDestroy(xx)
+    if yy:
+        lib.mv_MultiVectorDestroy(yy)
+    n_it = it.value
+    stats = {f: getattr(st, f) for f, _ in L.Stats._fields_}
+    return Result(None, lam, flag, n_it,
+                  np.zeros((k, 0)) if lh is None else lh[:, : n_it + 1],
+                  np.zeros((k, 0)) if rh is None else rh[:, : n_it + 1], res, stats)
+
+
+def lobpcg(X0: np.ndarray | None, A: CSR, B: np.ndarray | CSR | None = None, T=None,
+           Y: np.ndarray | None = None, tol: float = 1e-6, maxit: int = 200, verbosity: int = 0,
+           *, k: int | None = None, seed: int = 1, rtol: float = 1e-50, mode: str = "fp64",
+           **settings) -> Result:
+    """Smallest eigenpairs of A x = lambda B x.
+
+    X0: n x k initial block (None: k columns of 2*drand48()-1 after srand48(seed),
+    generated on the device).  B: None, a diagonal (1-D array) or a CSR matrix.
+    T: None, "jacobi" (diag(A)^-1) or a 1-D array d (T = diag(d)).  Y: n x ny
+    constraints.  mode: "fp64" or "mixed".
+    """
+    res = []
+    dA = DeviceCSR(A)
+    res.append(dA)
+    try:
+        dB = None
+        if B is not None:
+            dB = DeviceCSR(B) if isinstance(B, CSR) else DeviceDiag(B)
+            res.append(dB)
+        dT = None
+        if isinstance(T, str):
+            if T != "jacobi":
+                raise ValueError(f"unknown preconditioner {T!r}")
+            dT = DeviceDiag(jacobi_of=dA)
+            res.append(dT)
+        elif T is not None:
+            dT = DeviceDiag(np.asarray(T))
+            res.append(dT)
+        if X0 is None:
+            X = DeviceBlock(A.n, k)
+            X.set_random(seed)
+        else:
+            X = DeviceBlock.from_numpy(X0)
+        res.append(X)
+        dY = None
+        if Y is not None:
+            dY = DeviceBlock.from_numpy(Y)
+            res.append(dY)
+        r = solve_device(dA, X, dB, dT, dY, tol=tol, rtol=rtol, maxit=maxit, verbosity=verbosity,
+                         mode=mode, **settings)
+        r.X = X.to_numpy()
+        return r
+    finally:
+        for o in reversed(res):
+            o.close()

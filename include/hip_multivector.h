@@ -1,0 +1,143 @@
+/*
+ * Device multivector backend for the BLOPEX C API on MI355X (gfx950).
+ *
+ * The reference's serial backend (blopex_serial_double/multivector/
+ * multi_vector.h:22-32, pcg_multi/pcg_multi.h:17-51) stores a block as one
+ * column-major n x k host array and loops over its columns one at a time.
+ * hip_Multi_Vector keeps the same layout and mask semantics in device memory
+ * (HBM) and every block operation is a HIP kernel over the whole block.
+ *
+ *   hip_Multi_Vector      <- serial_Multi_Vector       (multi_vector.h:22-32)
+ *   HipSetupInterpreter   <- SerialSetupInterpreter    (pcg_multi.c:203-242)
+ *   HipCSRMatMultiVec     <- MatMultiVec               (pcg_multi.c:167-193)
+ *   HipDiagMatMultiVec    <- operatorB / operatorT slots (krylov/lobpcg.c:505-509)
+ *
+ * lobpcg_solve_double (lobpcg.h) on a block wrapped with this interpreter
+ * runs the device-resident solver; its precision and statistics hang off
+ * the block (hip_Multi_VectorSetSettings / hip_Multi_VectorGetStats).
+ *
+ * Operator callbacks receive hip_Multi_Vector pointers.  They act on the
+ * active columns of X and Y, enqueue their work on blopex_amd_stream() and
+ * may return before it completes: everything the solver does afterwards is
+ * ordered behind it on that stream.
+ *
+ * All work of the library runs on one HIP stream per process (default
+ * priority), created on the device current at the first call.
+ */
+#ifndef BLOPEX_AMD_HIP_MULTIVECTOR_H
+#define BLOPEX_AMD_HIP_MULTIVECTOR_H
+
+#include "interpreter.h"
+#include "multivector.h"
+
+#define HIPMV_FP64 0
+#define HIPMV_FP32 1
+
+#define HIPLOBPCG_FP64 0  /* every block operation in FP64 */
+#define HIPLOBPCG_MIXED 1 /* FP32 blocks, FP64 Gram accumulation and small dense, FP64 finish */
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hip_LobpcgSettings {
+    int mode;                 /* HIPLOBPCG_FP64 (default) or HIPLOBPCG_MIXED */
+    double switch_factor;     /* MIXED: leave FP32 once every active residual is below
+                                 switch_factor * atol, or FP32 has stalled (default 4) */
+    int stall_iterations;     /* MIXED: iterations without a 10% drop of the largest active
+                                 residual that count as a stall (default 8) */
+    int refine_iterations;    /* FP64 iterations with all columns active after convergence
+                                 (default 0) */
+    int profile;              /* 1: time kernel classes with HIP events (adds event records
+                                 to the stream; default 0) */
+} hip_LobpcgSettings;
+
+typedef struct hip_LobpcgStats {
+    double seconds;             /* wall time of the last device solve */
+    int iterations_fp32;        /* iterations done in FP32 (MIXED) */
+    int iterations_fp64;        /* iterations done in FP64 */
+    int refine_iterations;      /* of which refinement */
+    int exit_flag;
+    /* device time per kernel class, filled when settings.profile = 1 */
+    double seconds_small_dense; /* FP64 Cholesky / inverse / Rayleigh-Ritz */
+    double seconds_spmm;        /* operator A applications */
+    double seconds_gram;        /* Gram products */
+    double seconds_update;      /* block updates (block x small matrix, axpy) */
+    double seconds_other;       /* residuals, norms, preconditioner, B, gathers, copies */
+    double spmm_bytes;          /* algorithmic bytes of the CSR products timed above */
+    int spmm_calls;
+    int pad_;
+} hip_LobpcgStats;
+
+typedef struct hip_Multi_Vector {
+    void *data;                    /* device memory, column-major, leading dimension = size */
+    long size;                     /* rows */
+    BlopexInt num_vectors;         /* columns */
+    BlopexInt num_active_vectors;
+    BlopexInt *active_indices;     /* host, 0-based, ascending */
+    BlopexInt owns_data;
+    int precision;                 /* HIPMV_FP64 or HIPMV_FP32 */
+    hip_LobpcgSettings settings;
+    hip_LobpcgStats stats;
+} hip_Multi_Vector;
+
+/* ---- library ---------------------------------------------------------- */
+int blopex_amd_init(void);              /* optional: 0 on success */
+void *blopex_amd_stream(void);          /* the hipStream_t all work runs on */
+int blopex_amd_synchronize(void);       /* waits for the stream; 0 on success */
+const char *blopex_amd_last_error(void);/* last error message ("" if none) */
+
+/* ---- blocks ------------------------------------------------------------ */
+hip_Multi_Vector *hip_Multi_VectorCreate(long size, BlopexInt num_vectors, int precision);
+/* wraps existing device memory (not freed by Destroy) */
+hip_Multi_Vector *hip_Multi_VectorWrapDevice(void *data, long size, BlopexInt num_vectors,
+                                             int precision);
+void hip_Multi_VectorDestroy(hip_Multi_Vector *x);
+/* host <-> device, all columns, host array column-major n x k FP64 */
+int hip_Multi_VectorUpload(hip_Multi_Vector *x, const double *host);
+int hip_Multi_VectorDownload(const hip_Multi_Vector *x, double *host);
+/* 2*drand48()-1 over the active columns, column by column, after srand48(seed):
+   the stream of serial_Multi_VectorSetRandomValues (multi_vector.c:166-184),
+   generated on the device by jumping the LCG ahead per thread */
+int hip_Multi_VectorSetRandomValues(hip_Multi_Vector *x, BlopexInt seed);
+int hip_Multi_VectorSetSettings(hip_Multi_Vector *x, const hip_LobpcgSettings *s);
+int hip_Multi_VectorGetStats(const hip_Multi_Vector *x, hip_LobpcgStats *s);
+
+/* fills every multivector slot of the table the solver uses */
+BlopexInt HipSetupInterpreter(mv_InterfaceInterpreter *ii);
+
+/* ---- operators ----------------------------------------------------------- */
+typedef struct hip_CSRMatrix {
+    long n;
+    long nnz;
+    int *rowptr; /* device */
+    int *colind; /* device */
+    double *val; /* device FP64 */
+    float *val32;/* device FP32 copy (for FP32 blocks) */
+} hip_CSRMatrix;
+
+typedef struct hip_DiagMatrix {
+    long n;
+    double *d;   /* device FP64 */
+    float *d32;  /* device FP32 copy */
+} hip_DiagMatrix;
+
+/* square CSR from host arrays (int32 row pointers / columns, FP64 values) */
+hip_CSRMatrix *hip_CSRMatrixCreate(long n, long nnz, const int *rowptr, const int *colind,
+                                   const double *val);
+void hip_CSRMatrixDestroy(hip_CSRMatrix *A);
+/* operatorA / operatorB callback: Y(active) = A * X(active) */
+void HipCSRMatMultiVec(void *A, void *X, void *Y);
+
+hip_DiagMatrix *hip_DiagMatrixCreate(long n, const double *d);
+/* Jacobi preconditioner T = diag(A)^{-1} */
+hip_DiagMatrix *hip_DiagMatrixCreateJacobi(const hip_CSRMatrix *A);
+void hip_DiagMatrixDestroy(hip_DiagMatrix *D);
+/* operatorB / operatorT callback: Y(active) = diag(d) * X(active) */
+void HipDiagMatMultiVec(void *D, void *X, void *Y);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,84 @@
+"""CPU checks of the C ABI: the built library loads and exports every function
+include/*.h declares, and the struct layouts the bindings use match the
+headers.  No GPU calls."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from blopex_amd import _lib as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_functions():
+    names = set()
+    for h in os.listdir(os.path.join(ROOT, "include")):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"//[^\n]*", "", text)
+        # function declarations: identifier followed by '(' at top level ending in ');'
+        for m in re.finditer(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\([^;{]*\)\s*;", text):
+            name = m.group(1)
+            if name in ("if", "while", "for", "return", "sizeof", "void", "double", "long", "int",
+                        "char", "float", "BlopexInt", "komplex"):
+                continue  # function-pointer members: `void (*Slot)(...)`
+            names.add(name)
+    # typedef'd function-pointer members are not exports
+    return {n for n in names if not n[0].isupper() or n.startswith("Hip")}
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(L.LIB_PATH):
+        pytest.skip("library not built")
+    return L.load()
+
+
+def test_exports_every_declared_function(lib):
+    declared = _declared_functions()
+    assert "lobpcg_solve_double" in declared and "HipSetupInterpreter" in declared
+    missing = [n for n in sorted(declared) if not hasattr(lib, n)]
+    assert not missing, f"declared but not exported: {missing}"
+    for n in L.HEADER_FUNCTIONS:
+        assert hasattr(lib, n), n
+
+
+def test_struct_layouts():
+    assert ctypes.sizeof(L.Interpreter) == 26 * 8
+    assert ctypes.sizeof(L.FortranInterp) == 28 * 8
+    assert ctypes.sizeof(L.FortranMatrix) == 40
+    assert ctypes.sizeof(L.Tolerance) == 16
+    assert ctypes.sizeof(L.BlasLapack) == 32
+    # hip_Multi_Vector: data, size, 2 ints, ptr, 2 ints, settings(32), stats(80)
+    assert L.HipMultiVector.settings.offset == 40
+    assert ctypes.sizeof(L.Settings) == 32 and ctypes.sizeof(L.Stats) == 80
+    assert ctypes.sizeof(L.HipMultiVector) == 40 + 32 + 80
+
+
+def test_fortran_matrix_host_routines(lib):
+    """utilities_FortranMatrixUpperInv / Symmetrize against numpy (host code only)."""
+    import numpy as np
+
+    rng = np.random.default_rng(3)
+    n, gh = 7, 9
+    U = np.triu(rng.standard_normal((n, n))) + 4 * np.eye(n)
+    buf = np.zeros((gh, n), order="F")
+    buf[:n, :] = U
+    m = L.FortranMatrix(gh, n, n, buf.ctypes.data, 0)
+    lib.utilities_FortranMatrixUpperInv(ctypes.byref(m))
+    np.testing.assert_allclose(buf[:n, :] @ U, np.eye(n), atol=1e-12)
+    S = rng.standard_normal((n, n))
+    buf[:n, :] = S
+    lib.utilities_FortranMatrixSymmetrize(ctypes.byref(m))
+    np.testing.assert_array_equal(buf[:n, :], (S + S.T) * 0.5)
+
+
+def test_complex_solver_reports_unsupported(lib):
+    it = ctypes.c_int(7)
+    r = lib.lobpcg_solve_complex(None, None, None, None, None, None, None, None, L.BlasLapack(),
+                                 L.Tolerance(1e-6, 0), 10, 0, ctypes.byref(it), None, None, 0,
+                                 None, None, 0)
+    assert r == 2 and it.value == 0
+    assert "complex" in L.last_error()

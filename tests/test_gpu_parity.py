@@ -1,0 +1,241 @@
+"""GPU parity: the HIP path through the C ABI against the FP64 oracle.
+
+Bars (written per test):
+* integer / stream work (drand48 X0) is bit-exact;
+* single kernels in FP64 agree with the oracle / numpy FP64 to 1e-13 relative
+  (summation order differs), FP32 kernels to 1e-5 relative;
+* FP64 solves reproduce the reference's eigenvalues to 1e-8 relative at the
+  same tolerance, with the same exit flag and an
+  iteration count within 2 of the oracle's.
+"""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+from blopex_amd import _lib as L
+from blopex_amd import lobpcg as bl
+from blopex_amd import problems as pb
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def relerr(a, b):
+    return np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300))
+
+
+# ---------------------------------------------------------------- kernels
+def test_drand48_on_device_is_bit_exact():
+    say("drand48 fill 1000x7 seed 1 and 5000x3 seed 7")
+    for n, k, seed in [(1000, 7, 1), (5000, 3, 7)]:
+        X = bl.DeviceBlock(n, k)
+        X.set_random(seed)
+        got = X.to_numpy()
+        X.close()
+        np.testing.assert_array_equal(got, po.drand48_block(n, k, seed))
+
+
+@pytest.mark.parametrize("prec", [L.HIPMV_FP64, L.HIPMV_FP32])
+def test_spmm_against_oracle(prec):
+    say(f"spmm 9x8x7 sigma=1 k=5 prec={prec}")
+    A = pb.laplacian_7pt(9, 8, 7, sigma=1.0, seed=4)
+    X = np.random.default_rng(1).standard_normal((A.n, 5))
+    dA = bl.DeviceCSR(A)
+    dX = bl.DeviceBlock.from_numpy(X, prec)
+    dY = bl.DeviceBlock(A.n, 5, prec)
+    lib = L.load()
+    lib.HipCSRMatMultiVec.argtypes = [ctypes.c_void_p] * 3
+    lib.HipCSRMatMultiVec(ctypes.cast(dA.ptr, ctypes.c_void_p), ctypes.cast(dX.ptr, ctypes.c_void_p),
+                          ctypes.cast(dY.ptr, ctypes.c_void_p))
+    got = dY.to_numpy()
+    ref = po.spmm(A, X)
+    tol = 1e-13 if prec == L.HIPMV_FP64 else 1e-5
+    scale = np.abs(A.to_scipy()) @ np.abs(X)
+    assert np.max(np.abs(got - ref) / scale) < tol
+    for o in (dA, dX, dY):
+        o.close()
+
+
+def _wrap(ii, blk):
+    return L.load().mv_MultiVectorWrap(ctypes.byref(ii), ctypes.cast(blk.ptr, ctypes.c_void_p), 0)
+
+
+@pytest.mark.parametrize("prec", [L.HIPMV_FP64, L.HIPMV_FP32])
+def test_interpreter_slots_with_masks(prec):
+    """Every multivector slot lobpcg_solve uses, on masked columns, against numpy."""
+    say(f"interpreter slots prec={prec}")
+    lib = L.load()
+    rng = np.random.default_rng(7)
+    n, k = 3001, 6
+    Xh = rng.standard_normal((n, k))
+    Yh = rng.standard_normal((n, k))
+    tol = 1e-12 if prec == L.HIPMV_FP64 else 2e-5
+    if prec == L.HIPMV_FP32:
+        Xh = Xh.astype(np.float32).astype(np.float64)
+        Yh = Yh.astype(np.float32).astype(np.float64)
+    X = bl.DeviceBlock.from_numpy(Xh, prec)
+    Y = bl.DeviceBlock.from_numpy(Yh, prec)
+    ii = bl.interpreter()
+    xx, yy = _wrap(ii, X), _wrap(ii, Y)
+    mx = np.array([1, 0, 1, 1, 0, 1], dtype=np.int32)
+    my = np.array([0, 1, 1, 0, 1, 1], dtype=np.int32)
+    ax, ay = np.nonzero(mx)[0], np.nonzero(my)[0]
+    lib.mv_MultiVectorSetMask(xx, mx.ctypes.data)
+    lib.mv_MultiVectorSetMask(yy, my.ctypes.data)
+    # Gram into a sub-block with leading dimension gh
+    gh = 7
+    G = np.full((gh, 4), np.nan, order="F")
+    lib.mv_MultiVectorByMultiVector(xx, yy, gh, 4, 4, G.ctypes.data)
+    ref = Xh[:, ax].T @ Yh[:, ay]
+    np.testing.assert_allclose(G[:4, :], ref, rtol=tol, atol=tol * np.abs(ref).max())
+    assert np.all(np.isnan(G[4:, :]))
+    # diag of the Gram into the masked entries
+    d = np.full(6, -7.0)
+    lib.mv_MultiVectorByMultiVectorDiag(xx, yy, mx.ctypes.data, 6, d.ctypes.data)
+    np.testing.assert_allclose(d[ax], np.sum(Xh[:, ax] * Yh[:, ay], axis=0), rtol=tol,
+                               atol=tol * n)
+    assert np.all(d[mx == 0] == -7.0)
+    # y = x * C (overwrite), C 4 x 4 with leading dimension 5
+    C = np.asfortranarray(rng.standard_normal((5, 4)))
+    lib.mv_MultiVectorByMatrix(xx, 5, 4, 4, C.ctypes.data, yy)
+    Yn = Yh.copy()
+    Yn[:, ay] = Xh[:, ax] @ C[:4, :]
+    got = Y.to_numpy()
+    np.testing.assert_allclose(got, Yn, rtol=tol, atol=tol * 10)
+    # y += x * C (Xapy)
+    lib.mv_MultiVectorXapy(xx, 5, 4, 4, C.ctypes.data, yy)
+    Yn[:, ay] += Xh[:, ax] @ C[:4, :]
+    np.testing.assert_allclose(Y.to_numpy(), Yn, rtol=tol, atol=tol * 20)
+    # y = x * diag(d(mask))
+    dd = np.arange(1.0, 7.0)
+    lib.mv_MultiVectorByDiagonal(xx, mx.ctypes.data, 6, dd.ctypes.data, yy)
+    Yn[:, ay] = Xh[:, ax] * dd[ax]
+    np.testing.assert_allclose(Y.to_numpy(), Yn, rtol=tol, atol=tol)
+    # axpy and copy on masked columns
+    lib.mv_MultiVectorAxpy(-2.5, xx, yy)
+    Yn[:, ay] += -2.5 * Xh[:, ax]
+    np.testing.assert_allclose(Y.to_numpy(), Yn, rtol=tol, atol=tol * 4)
+    lib.mv_MultiVectorCopy(xx, yy)
+    Yn[:, ay] = Xh[:, ax]
+    np.testing.assert_array_equal(Y.to_numpy(), Yn)
+    # clear, copy-create, width/height
+    lib.mv_MultiVectorClear(yy)
+    Yn[:, ay] = 0.0
+    np.testing.assert_array_equal(Y.to_numpy(), Yn)
+    assert lib.mv_MultiVectorWidth(xx) == 6 and lib.mv_MultiVectorHeight(xx) == n
+    zz = lib.mv_MultiVectorCreateCopy(xx, 1)
+    Z = L.HipMultiVector.from_address(lib.mv_MultiVectorGetData(zz))
+    assert Z.num_vectors == 6 and Z.num_active_vectors == 6
+    lib.mv_MultiVectorDestroy(zz)
+    for h in (xx, yy):
+        lib.mv_MultiVectorDestroy(h)
+    X.close()
+    Y.close()
+
+
+# ---------------------------------------------------------------- solves
+def _cases():
+    from tests.golden.make_golden import cases
+
+    out = {}
+    for name, kw in cases():
+        kw["X0"] = po.drand48_block(kw["X0"].shape[0], kw["X0"].shape[1], 1)
+        out[name] = kw
+    return out
+
+
+CASES = _cases()
+
+
+def _device_solve(kw, mode="fp64", **extra):
+    T = kw.get("tdiag")
+    return bl.lobpcg(kw["X0"], kw["A"], B=kw.get("bdiag"), T=T, Y=kw.get("Y"),
+                     tol=kw.get("atol", 1e-6), maxit=kw["maxit"], mode=mode, **extra)
+
+
+@pytest.mark.parametrize("name", ["cfg1", "diag40", "testdiag", "gen12"])
+def test_device_fp64_matches_reference(golden, name):
+    say(f"device FP64 solve {name}")
+    kw = CASES[name]
+    r = _device_solve(kw)
+    ref_lam = golden[name + "_lam"]
+    assert r.flag == int(golden[name + "_flag"])
+    assert abs(r.iterations - int(golden[name + "_iters"])) <= 2, (r.iterations,
+                                                                     int(golden[name + "_iters"]))
+    assert relerr(r.lam, ref_lam) < 1e-8
+    # histories start from the same initial Rayleigh-Ritz
+    np.testing.assert_allclose(r.lam_hist[:, 0], golden[name + "_lam_hist"][:, 0], rtol=1e-12)
+    assert np.all(r.res[: len(ref_lam)] <= kw.get("atol", 1e-6) + 1e-15)
+
+
+def test_generic_path_through_hip_interpreter(golden):
+    """lobpcg_solve with the library's small-dense interpreters and scipy's
+    LAPACK, every block op through the HIP interpreter slots."""
+    say("generic lobpcg_solve via HIP interpreter, cfg1")
+    lib = L.load()
+    kw = CASES["cfg1"]
+    A = bl.DeviceCSR(kw["A"])
+    X = bl.DeviceBlock.from_numpy(kw["X0"])
+    ii = bl.interpreter()
+    xx = _wrap(ii, X)
+    lp, fu = L.LobpcgInterp(), L.FortranInterp()
+    lib.blopex_amd_default_interpreters(ctypes.byref(lp), ctypes.byref(fu))
+    potrf, sygv = po.lapack_pointers()
+    k, maxit = 10, 200
+    lam, res = np.zeros(k), np.zeros(k)
+    lh = np.zeros((k, maxit + 1), order="F")
+    it = ctypes.c_int(0)
+    flag = lib.lobpcg_solve(xx, ctypes.cast(A.ptr, ctypes.c_void_p), A.op, None, None, None, None,
+                            None, L.BlasLapack(potrf, sygv), L.Tolerance(1e-6, 1e-50), maxit, 0,
+                            ctypes.byref(it), lam.ctypes.data, lh.ctypes.data, k, res.ctypes.data,
+                            None, k, ctypes.byref(lp), ctypes.byref(fu))
+    lib.mv_MultiVectorDestroy(xx)
+    Xo = X.to_numpy()
+    X.close()
+    A.close()
+    assert flag == 0
+    assert abs(it.value - int(golden["cfg1_iters"])) <= 2
+    assert relerr(lam, golden["cfg1_lam"]) < 1e-8
+    np.testing.assert_allclose(lh[:, : 20], golden["cfg1_lam_hist"][:, :20], rtol=1e-7)
+    np.testing.assert_allclose(Xo.T @ Xo, np.eye(k), atol=1e-7)
+
+
+def test_device_exit_codes():
+    say("device exit codes")
+    A = pb.laplacian_7pt(6, 6, 6)
+    X = po.drand48_block(A.n, 3, 1)
+    X[:, 1] = X[:, 0]
+    r = bl.lobpcg(X, A, maxit=10)
+    assert (r.flag, r.iterations) == (-1, 0)
+    r = bl.lobpcg(po.drand48_block(A.n, 3, 1), A, Y=np.ones((A.n, 2)), maxit=10)
+    assert r.flag == 3
+    r = bl.lobpcg(po.drand48_block(A.n, 3, 1), A, maxit=2, tol=1e-14)
+    assert (r.flag, r.iterations) == (-1, 2)
+
+
+def test_device_x0_from_seed_matches_host_x0():
+    say("device X0 (seeded) solve equals host-X0 solve")
+    A = pb.laplacian_7pt(10, 9, 8)
+    r1 = bl.lobpcg(None, A, k=4, seed=1, maxit=60)
+    r2 = bl.lobpcg(po.drand48_block(A.n, 4, 1), A, maxit=60)
+    assert r1.iterations == r2.iterations
+    np.testing.assert_array_equal(r1.lam_hist, r2.lam_hist)
+
+
+def test_mixed_precision_small():
+    say("mixed precision cfg1-like (16^3, k=8, jacobi)")
+    A = pb.laplacian_7pt(16, 16, 16, sigma=1.0, seed=5)
+    X0 = po.drand48_block(A.n, 8, 1)
+    o = po.solve(A=A, X0=X0, tdiag=1.0 / A.diagonal(), maxit=400)
+    r = bl.lobpcg(X0, A, T="jacobi", maxit=400, mode="mixed")
+    assert r.flag == 0 and o.flag == 0
+    assert r.stats["iterations_fp32"] > 0
+    assert np.all(r.res <= 1e-6 + 1e-15)
+    assert relerr(r.lam, o.lam) < 1e-8
