@@ -40,7 +40,7 @@ class Stats(ctypes.Structure):
 
 
 class HipMultiVector(ctypes.Structure):
-    _fields_ = [("data", c_void_p), ("size", c_long), ("num_vectors", c_int),
+    _fields_ = [("data", c_void_p), ("size", c_long), ("ld", c_long), ("num_vectors", c_int),
                 ("num_active_vectors", c_int), ("active_indices", ctypes.POINTER(c_int)),
                 ("owns_data", c_int), ("precision", c_int), ("settings", Settings),
                 ("stats", Stats)]
@@ -122,6 +122,9 @@ _SIGS = {
                                      P(c_int), c_void_p, c_void_p, c_int, c_void_p, c_void_p,
                                      c_int]),
     "blopex_amd_default_interpreters": (None, [P(LobpcgInterp), P(FortranInterp)]),
+    "blopex_amd_kernel_update": (c_int, [c_int, c_long, c_int, c_int, c_int, c_void_p, c_void_p,
+                                         c_void_p, c_void_p, c_void_p, c_void_p]),
+    "blopex_amd_kernel_gram": (c_int, [c_int, c_long, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "utilities_FortranMatrixUpperInv": (None, [P(FortranMatrix)]),
     "utilities_FortranMatrixSymmetrize": (None, [P(FortranMatrix)]),
 }

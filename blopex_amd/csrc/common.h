@@ -52,6 +52,16 @@ void download_sync(void *dst, const void *src, size_t bytes);
 
 inline size_t elem_size(int precision) { return precision == HIPMV_FP32 ? 4 : 8; }
 
+// Device-side status of one solve; kernels after a fatal failure are no-ops.
+struct DevStatus {
+    int abort;      // set on a failure that ends the solve
+    int x_info;     // Cholesky info of X'BX (initial orthonormalisation)
+    int r_info;     // Cholesky info of R'BR
+    int p_info;     // Cholesky info of P'BP (nonzero: P dropped this iteration)
+    int gevp_info;  // Rayleigh-Ritz failure (dsygv-style info)
+    int pad[3];
+};
+
 // Column maps passed to kernels by value (kernel arguments).
 constexpr int kMaxCols = 128;
 struct ColMap {

@@ -39,14 +39,18 @@ __device__ double block_sum(double v, double *red) {
 }
 
 // In-place upper Cholesky of the m x m matrix a (leading dim ld); returns the
-// LAPACK info (0, or the 1-based column whose pivot is not positive).
-__device__ int chol_upper(double *a, int ld, int m, int *shared_info) {
+// LAPACK-style info (0, or the 1-based column whose pivot fails).  A pivot
+// fails when it is not positive or has lost all but 4 eps of the column's own
+// diagonal entry (the column is numerically dependent on the ones before it);
+// orig (>= m doubles of shared memory) keeps those entries.
+__device__ int chol_upper(double *a, int ld, int m, int *shared_info, double *orig) {
     if (tid() == 0) *shared_info = 0;
+    for (int j = tid(); j < m; j += kDT) orig[j] = a[j + j * ld];
     __syncthreads();
     for (int j = 0; j < m; j++) {
         // a(j,j) already holds a_jj - sum_{i<j} u_ij^2 (right-looking updates)
         const double d = a[j + j * ld];
-        if (!(d > 0.0) || !isfinite(d)) {
+        if (!(d > 4.0 * 2.220446049250313e-16 * orig[j]) || !isfinite(d)) {
             if (tid() == 0) *shared_info = j + 1;
             __syncthreads();
             return *shared_info;
@@ -106,7 +110,7 @@ __global__ void __launch_bounds__(kDT) k_chol_inv(double *g, int ld, int m, DevS
     __shared__ int info;
     __shared__ double diag[kMaxRR];
     if (st->abort) return;
-    int r = chol_upper(g, ld, m, &info);
+    int r = chol_upper(g, ld, m, &info, diag);
     if (r) {
         // a failed factor becomes the identity, so the block products that
         // follow leave their operand unchanged (P then drops out of the basis)
@@ -285,7 +289,7 @@ __global__ void __launch_bounds__(kDT) k_rr(RRArgs r) {
     }
     __syncthreads();
     // ---- B = U'U ----
-    int bi = chol_upper(B, nA, nA, &info);
+    int bi = chol_upper(B, nA, nA, &info, diag);
     if (bi) {
         if (tid() == 0) {
             r.st->gevp_info = nA + bi;  // LAPACK dsygv reports N + i for a B failure
@@ -376,6 +380,238 @@ __global__ void __launch_bounds__(kDT) k_rr(RRArgs r) {
     if (tid() == 0) r.st->gevp_info = 0;
 }
 
+
+// ---- Rayleigh-Ritz with the R/P orthonormalisation folded in ----------
+//
+// The Gram buffers hold raw products of [X R P] with [AX AR AP] and
+// [BX BR BP] (R, P not yet B-orthonormal).  With U_R = chol(R'BR),
+// U_P = chol(P'BP) and S = diag(I, U_R^{-1}, U_P^{-1}), the pencil of the
+// B-orthonormalised basis is S' G S: exactly what lobpcg_solve assembles from
+// the explicitly normalised blocks (lobpcg.c:1027-1075 then 1092-1165).  The
+// Ritz coordinates returned are S Y, so the caller combines the raw blocks.
+struct RR2Args {
+    const double *ga, *gb;  // raw Gram buffers, leading dim lda
+    int lda;
+    int k, a, p;            // block widths (p = 0: no P this iteration)
+    const double *lam_in;   // Ritz values of X (XAX = diag)
+    double *lam_out;
+    double *coef;           // (k+a+p) x k, leading dim ldc
+    int ldc;
+    double *work;           // global scratch
+    DevStatus *st;
+};
+
+__global__ void __launch_bounds__(kDT) k_rr_implicit(RR2Args r) {
+    extern __shared__ double lds[];
+    __shared__ double cs[kMaxRR / 2 + 1], sn[kMaxRR / 2 + 1], red[kDT / 64], diag[kMaxRR];
+    __shared__ int rot, info, rank_of[kMaxRR];
+    if (r.st->abort) return;
+    const int k = r.k, a = r.a, pq = r.p;
+    const int nmax = k + a + pq, nEmax = (nmax + 1) / 2 * 2;
+    const bool in_lds = (size_t)2 * nEmax * nEmax * sizeof(double) <= kRRLdsBytes;
+    double *G1 = r.work, *G2 = G1 + (size_t)nEmax * nEmax;
+    double *WR = G2 + (size_t)nEmax * nEmax, *WP = WR + (size_t)a * a;
+    double *Y = WP + (size_t)(pq > 0 ? pq : 1) * (pq > 0 ? pq : 1);
+    double *M1 = in_lds ? lds : Y + (size_t)nmax * k;
+    double *M2 = in_lds ? lds + (size_t)nEmax * nEmax : M1 + (size_t)nEmax * nEmax;
+
+    // ---- U_R^{-1}, U_P^{-1} (small Cholesky factors in the M2 area) ----
+    for (int e = tid(); e < a * a; e += kDT) M2[e] = r.gb[(k + e % a) + (size_t)(k + e / a) * r.lda];
+    __syncthreads();
+    int ci = chol_upper(M2, a, a, &info, diag);
+    if (ci) {
+        if (tid() == 0) {
+            r.st->r_info = ci;
+            r.st->abort = 1;
+        }
+        return;
+    }
+    upper_inverse(M2, a, a, diag);
+    for (int e = tid(); e < a * a; e += kDT) WR[e] = M2[e];
+    __syncthreads();
+    int p = 0;
+    if (pq > 0) {
+        for (int e = tid(); e < pq * pq; e += kDT)
+            M2[e] = r.gb[(k + a + e % pq) + (size_t)(k + a + e / pq) * r.lda];
+        __syncthreads();
+        ci = chol_upper(M2, pq, pq, &info, diag);
+        if (ci == 0) {
+            upper_inverse(M2, pq, pq, diag);
+            for (int e = tid(); e < pq * pq; e += kDT) WP[e] = M2[e];
+            p = pq;
+        }
+        if (tid() == 0) r.st->p_info = ci;
+        __syncthreads();
+    }
+    const int nA = k + a + p, nE = (nA + 1) / 2 * 2;
+    // block of a basis index: 0 = X, 1 = R, 2 = P
+    auto blk = [&](int i) { return i < k ? 0 : (i < k + a ? 1 : 2); };
+
+    // ---- raw symmetric pencil, then M <- S' M S ----
+    for (int pass = 0; pass < 2; pass++) {
+        double *M = pass == 0 ? M1 : M2;
+        const double *g = pass == 0 ? r.ga : r.gb;
+        for (int e = tid(); e < nA * nA; e += kDT) {
+            const int i = e % nA, j = e / nA, bi = blk(i), bj = blk(j);
+            double v;
+            if (bi == 0 && bj == 0)
+                v = (i == j) ? (pass == 0 ? r.lam_in[i] : 1.0) : 0.0;
+            else if (bi > bj)
+                v = g[i + (size_t)j * r.lda];
+            else if (bi < bj)
+                v = g[j + (size_t)i * r.lda];
+            else
+                v = g[i + (size_t)j * r.lda];
+            M[i + j * nE] = v;
+        }
+        __syncthreads();
+        // G1 = M S
+        for (int e = tid(); e < nA * nA; e += kDT) {
+            const int i = e % nA, j = e / nA, bj = blk(j);
+            double s;
+            if (bj == 0) {
+                s = M[i + j * nE];
+            } else {
+                const int o = bj == 1 ? k : k + a, m = bj == 1 ? a : p, jj = j - o;
+                const double *W = bj == 1 ? WR : WP;
+                s = 0.0;
+                for (int l = 0; l <= jj; l++) s += M[i + (o + l) * nE] * W[l + jj * m];
+            }
+            G1[i + (size_t)j * nE] = s;
+        }
+        __syncthreads();
+        // M = S' G1
+        for (int e = tid(); e < nA * nA; e += kDT) {
+            const int i = e % nA, j = e / nA, bi = blk(i);
+            double s;
+            if (bi == 0) {
+                s = G1[i + (size_t)j * nE];
+            } else {
+                const int o = bi == 1 ? k : k + a, m = bi == 1 ? a : p, ii = i - o;
+                const double *W = bi == 1 ? WR : WP;
+                s = 0.0;
+                for (int l = 0; l <= ii; l++) s += W[l + ii * m] * G1[(o + l) + (size_t)j * nE];
+            }
+            M[i + j * nE] = s;
+        }
+        __syncthreads();
+        // lower triangle wins off the diagonal blocks; RAR/PAP symmetrised;
+        // RBR = PBP = I (lobpcg.c:1113-1162)
+        for (int e = tid(); e < nA * nA; e += kDT) {
+            const int i = e % nA, j = e / nA;
+            if (i <= j) continue;
+            const int bi = blk(i), bj = blk(j);
+            double v;
+            if (bi == bj && bi > 0)
+                v = pass == 0 ? 0.5 * (M[i + j * nE] + M[j + i * nE]) : 0.0;
+            else
+                v = M[i + j * nE];
+            M[i + j * nE] = v;
+            M[j + i * nE] = v;
+        }
+        __syncthreads();
+        if (pass == 1)
+            for (int i = tid(); i < nA; i += kDT) M[i + i * nE] = 1.0;
+        __syncthreads();
+    }
+
+    // ---- generalized eigenproblem M1 u = lambda M2 u ----
+    int bi = chol_upper(M2, nE, nA, &info, diag);
+    if (bi) {
+        if (tid() == 0) {
+            r.st->gevp_info = nA + bi;
+            r.st->abort = 1;
+        }
+        return;
+    }
+    for (int e = tid(); e < nA * nA; e += kDT) {
+        const int i = e % nA, j = e / nA;
+        if (i > j) M2[i + j * nE] = 0.0;
+    }
+    __syncthreads();
+    upper_inverse(M2, nE, nA, diag);  // M2 := U^{-1}
+    // G1 = M1 U^{-1}
+    for (int e = tid(); e < nA * nA; e += kDT) {
+        const int i = e % nA, j = e / nA;
+        double s = 0.0;
+        for (int l = 0; l <= j; l++) s += M1[i + l * nE] * M2[l + j * nE];
+        G1[i + (size_t)j * nE] = s;
+    }
+    // G2 = U^{-1} (kept for the eigenvectors)
+    for (int e = tid(); e < nA * nA; e += kDT) G2[e % nA + (size_t)(e / nA) * nE] = M2[e % nA + (e / nA) * nE];
+    __syncthreads();
+    // M1 = U^{-T} G1 (padded to nE), M2 = I
+    for (int e = tid(); e < nE * nE; e += kDT) {
+        const int i = e % nE, j = e / nE;
+        double s = 0.0;
+        if (i < nA && j < nA)
+            for (int l = 0; l <= i; l++) s += G2[l + (size_t)i * nE] * G1[l + (size_t)j * nE];
+        M1[i + j * nE] = s;
+        M2[i + j * nE] = (i == j) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    for (int e = tid(); e < nA * nA; e += kDT) {
+        const int i = e % nA, j = e / nA;
+        if (i < j) {
+            const double s = 0.5 * (M1[i + j * nE] + M1[j + i * nE]);
+            M1[i + j * nE] = s;
+            M1[j + i * nE] = s;
+        }
+    }
+    __syncthreads();
+    if (nE > nA && tid() == 0) {
+        double mx = 0.0;
+        for (int i = 0; i < nA; i++) {
+            double s = 0.0;
+            for (int j = 0; j < nA; j++) s += fabs(M1[i + j * nE]);
+            mx = s > mx ? s : mx;
+        }
+        M1[nA + nA * nE] = 2.0 * mx + 1.0;
+    }
+    __syncthreads();
+    jacobi(M1, M2, nE, cs, sn, &rot, red);
+    for (int i = tid(); i < nE; i += kDT) {
+        const double li = M1[i + i * nE];
+        int rk = 0;
+        for (int j = 0; j < nE; j++) {
+            const double lj = M1[j + j * nE];
+            rk += (lj < li) || (lj == li && j < i);
+        }
+        rank_of[i] = rk;
+    }
+    __syncthreads();
+    for (int i = tid(); i < nE; i += kDT) {
+        const int rk = rank_of[i];
+        if (rk < k) r.lam_out[rk] = M1[i + i * nE];
+        diag[rk] = (double)i;
+    }
+    __syncthreads();
+    // Y = U^{-1} Q(:, first k), then coef = S Y
+    for (int e = tid(); e < nA * k; e += kDT) {
+        const int i = e % nA, c = e / nA, col = (int)diag[c];
+        double s = 0.0;
+        for (int l = i; l < nA; l++) s += G2[i + (size_t)l * nE] * M2[l + col * nE];
+        Y[i + (size_t)c * nA] = s;
+    }
+    __syncthreads();
+    for (int e = tid(); e < (k + a + pq) * k; e += kDT) {
+        const int i = e % (k + a + pq), c = e / (k + a + pq), bi2 = blk(i);
+        double s;
+        if (i >= nA) {
+            s = 0.0;  // dropped P block
+        } else if (bi2 == 0) {
+            s = Y[i + (size_t)c * nA];
+        } else {
+            const int o = bi2 == 1 ? k : k + a, m = bi2 == 1 ? a : p, ii = i - o;
+            const double *W = bi2 == 1 ? WR : WP;
+            s = 0.0;
+            for (int l = ii; l < m; l++) s += W[ii + l * m] * Y[(o + l) + (size_t)c * nA];
+        }
+        r.coef[i + (size_t)c * r.ldc] = s;
+    }
+    if (tid() == 0) r.st->gevp_info = 0;
+}
+
 }  // namespace
 
 void chol_inv(double *g, int ld, int m, DevStatus *st, int which) {
@@ -400,6 +636,27 @@ void rr_solve(const double *ga, const double *gb, int lda, int k, int a, int p, 
     }
     const bool in_lds = (size_t)2 * nE * nE * sizeof(double) <= kRRLdsBytes;
     hipLaunchKernelGGL(k_rr, dim3(1), dim3(kDT), in_lds ? kRRLdsBytes : 0, stream(), r);
+    BX_HIP(hipGetLastError());
+}
+
+void rr_solve_implicit(const double *ga, const double *gb, int lda, int k, int a, int p,
+                       const double *lam_in, double *lam_out, double *coef, int ldc,
+                       DevStatus *st) {
+    const int nmax = k + a + p;
+    BX_REQUIRE(nmax <= kMaxRR, "rr_solve_implicit: pencil of order %d (max %d)", nmax, kMaxRR);
+    const int nE = (nmax + 1) / 2 * 2;
+    const size_t words = (size_t)4 * nE * nE + (size_t)a * a + (size_t)(p > 0 ? p * p : 1) +
+                         (size_t)nmax * k + 64;
+    double *work = (double *)scratch(7, words * sizeof(double));
+    RR2Args r{ga, gb, lda, k, a, p, lam_in, lam_out, coef, ldc, work, st};
+    static bool attr_set = false;
+    if (!attr_set) {
+        BX_HIP(hipFuncSetAttribute((const void *)k_rr_implicit,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, kRRLdsBytes));
+        attr_set = true;
+    }
+    const bool in_lds = (size_t)2 * nE * nE * sizeof(double) <= kRRLdsBytes;
+    hipLaunchKernelGGL(k_rr_implicit, dim3(1), dim3(kDT), in_lds ? kRRLdsBytes : 0, stream(), r);
     BX_HIP(hipGetLastError());
 }
 

@@ -31,6 +31,7 @@ hip_Multi_Vector *alloc_header(long size, BlopexInt k, int precision) {
                "hip_Multi_VectorCreate: bad precision %d", precision);
     hip_Multi_Vector *x = (hip_Multi_Vector *)calloc(1, sizeof(hip_Multi_Vector));
     x->size = size;
+    x->ld = size;
     x->num_vectors = k;
     x->precision = precision;
     x->active_indices = (BlopexInt *)malloc(sizeof(BlopexInt) * (k > 0 ? k : 1));
@@ -90,13 +91,13 @@ void hip_Multi_VectorDestroy(hip_Multi_Vector *x) {
 int hip_Multi_VectorUpload(hip_Multi_Vector *x, const double *host) {
     size_t n = (size_t)x->size * x->num_vectors;
     if (!n) return 0;
-    if (x->precision == HIPMV_FP64) {
+    if (x->precision == HIPMV_FP64 && x->ld == x->size) {
         BX_HIP(hipMemcpyAsync(x->data, host, n * 8, hipMemcpyHostToDevice, stream()));
     } else {
         void *tmp = scratch(4, n * 8);
         BX_HIP(hipMemcpyAsync(tmp, host, n * 8, hipMemcpyHostToDevice, stream()));
-        convert_block(x->size, x->num_vectors, tmp, x->size, HIPMV_FP64, x->data, x->size,
-                      HIPMV_FP32);
+        convert_block(x->size, x->num_vectors, tmp, x->size, HIPMV_FP64, x->data, x->ld,
+                      x->precision);
     }
     BX_HIP(hipStreamSynchronize(stream()));
     return 0;
@@ -105,11 +106,11 @@ int hip_Multi_VectorUpload(hip_Multi_Vector *x, const double *host) {
 int hip_Multi_VectorDownload(const hip_Multi_Vector *x, double *host) {
     size_t n = (size_t)x->size * x->num_vectors;
     if (!n) return 0;
-    if (x->precision == HIPMV_FP64) {
+    if (x->precision == HIPMV_FP64 && x->ld == x->size) {
         BX_HIP(hipMemcpyAsync(host, x->data, n * 8, hipMemcpyDeviceToHost, stream()));
     } else {
         void *tmp = scratch(4, n * 8);
-        convert_block(x->size, x->num_vectors, x->data, x->size, HIPMV_FP32, tmp, x->size,
+        convert_block(x->size, x->num_vectors, x->data, x->ld, x->precision, tmp, x->size,
                       HIPMV_FP64);
         BX_HIP(hipMemcpyAsync(host, tmp, n * 8, hipMemcpyDeviceToHost, stream()));
     }
@@ -118,7 +119,7 @@ int hip_Multi_VectorDownload(const hip_Multi_Vector *x, double *host) {
 }
 
 int hip_Multi_VectorSetRandomValues(hip_Multi_Vector *x, BlopexInt seed) {
-    drand48_cols(x->precision, x->size, (int)seed, x->data, x->size, colmap_of(x));
+    drand48_cols(x->precision, x->size, (int)seed, x->data, x->ld, colmap_of(x));
     return 0;
 }
 
@@ -145,9 +146,8 @@ static void *hipmv_CopyCreate(void *src_, BlopexInt copyValues) {
     hip_Multi_Vector *dst = hip_Multi_VectorCreate(src->size, src->num_vectors, src->precision);
     dst->settings = src->settings;
     if (copyValues && src->size && src->num_vectors)
-        BX_HIP(hipMemcpyAsync(dst->data, src->data,
-                              (size_t)src->size * src->num_vectors * elem_size(src->precision),
-                              hipMemcpyDeviceToDevice, stream()));
+        convert_block(src->size, src->num_vectors, src->data, src->ld, src->precision, dst->data,
+                      dst->ld, dst->precision);
     return dst;
 }
 
@@ -167,12 +167,12 @@ static void hipmv_SetMask(void *x_, BlopexInt *mask) {
 static void hipmv_Copy(void *x_, void *y_) {
     hip_Multi_Vector *x = H(x_), *y = H(y_);
     same_shape(x, y, "CopyMultiVector");
-    copy_cols(x->precision, x->size, x->data, x->size, colmap_of(x), y->data, y->size, colmap_of(y));
+    copy_cols(x->precision, x->size, x->data, x->ld, colmap_of(x), y->data, y->ld, colmap_of(y));
 }
 
 static void hipmv_Clear(void *x_) {
     hip_Multi_Vector *x = H(x_);
-    clear_cols(x->precision, x->size, x->data, x->size, colmap_of(x));
+    clear_cols(x->precision, x->size, x->data, x->ld, colmap_of(x));
 }
 
 static void hipmv_SetRandom(void *x, BlopexInt seed) { hip_Multi_VectorSetRandomValues(H(x), seed); }
@@ -185,7 +185,7 @@ static void hipmv_InnerProd(void *x_, void *y_, BlopexInt gh, BlopexInt h, Blope
                (int)x->num_active_vectors, (int)y->num_active_vectors);
     if (h == 0 || w == 0) return;
     double *g = (double *)scratch(3, (size_t)h * w * sizeof(double));
-    gram_cols(x->precision, x->size, x->data, x->size, colmap_of(x), y->data, y->size,
+    gram_cols(x->precision, x->size, x->data, x->ld, colmap_of(x), y->data, y->ld,
               colmap_of(y), g, h);
     std::vector<double> tmp((size_t)h * w);
     download_sync(tmp.data(), g, tmp.size() * sizeof(double));
@@ -203,7 +203,7 @@ static void hipmv_InnerProdDiag(void *x_, void *y_, BlopexInt *mask, BlopexInt n
                (int)x->num_active_vectors);
     if (pos.empty()) return;
     double *d = (double *)scratch(3, pos.size() * sizeof(double));
-    diagdot_cols(x->precision, x->size, x->data, x->size, colmap_of(x), y->data, y->size,
+    diagdot_cols(x->precision, x->size, x->data, x->ld, colmap_of(x), y->data, y->ld,
                  colmap_of(y), d);
     std::vector<double> tmp(pos.size());
     download_sync(tmp.data(), d, tmp.size() * sizeof(double));
@@ -224,7 +224,7 @@ static void vecmat_common(void *x_, BlopexInt gh, BlopexInt h, BlopexInt w, void
         memcpy(c.data() + (size_t)j * h, (const double *)v + (size_t)j * gh, sizeof(double) * h);
     double *dc = (double *)scratch(2, c.size() * sizeof(double));
     upload_async(dc, c.data(), c.size() * sizeof(double));
-    vecmat_cols(x->precision, x->size, x->data, x->size, colmap_of(x), dc, h, y->data, y->size,
+    vecmat_cols(x->precision, x->size, x->data, x->ld, colmap_of(x), dc, h, y->data, y->ld,
                 colmap_of(y), acc);
 }
 
@@ -245,14 +245,14 @@ static void hipmv_VecMatDiag(void *x_, BlopexInt *mask, BlopexInt n, void *diag,
                (int)x->num_active_vectors);
     std::vector<double> coef(pos.size());
     for (size_t i = 0; i < pos.size(); i++) coef[i] = ((const double *)diag)[pos[i]];
-    scale_cols(x->precision, x->size, coef.data(), x->data, x->size, colmap_of(x), y->data,
-               y->size, colmap_of(y));
+    scale_cols(x->precision, x->size, coef.data(), x->data, x->ld, colmap_of(x), y->data,
+               y->ld, colmap_of(y));
 }
 
 static void hipmv_Axpy(double alpha, void *x_, void *y_) {
     hip_Multi_Vector *x = H(x_), *y = H(y_);
     same_shape(x, y, "MultiAxpy");
-    axpy_cols(x->precision, x->size, alpha, x->data, x->size, colmap_of(x), y->data, y->size,
+    axpy_cols(x->precision, x->size, alpha, x->data, x->ld, colmap_of(x), y->data, y->ld,
               colmap_of(y));
 }
 
@@ -283,9 +283,9 @@ static void hipmv_Eval(void (*f)(void *, void *, void *), void *par, void *x_, v
     size_t es = elem_size(x->precision);
     for (int j = 0; j < x->num_active_vectors; j++) {
         hip_Multi_Vector *xc = hip_Multi_VectorWrapDevice(
-            (char *)x->data + (size_t)x->active_indices[j] * x->size * es, x->size, 1, x->precision);
+            (char *)x->data + (size_t)x->active_indices[j] * x->ld * es, x->size, 1, x->precision);
         hip_Multi_Vector *yc = hip_Multi_VectorWrapDevice(
-            (char *)y->data + (size_t)y->active_indices[j] * y->size * es, y->size, 1, y->precision);
+            (char *)y->data + (size_t)y->active_indices[j] * y->ld * es, y->size, 1, y->precision);
         f(par, xc, yc);
         hip_Multi_VectorDestroy(xc);
         hip_Multi_VectorDestroy(yc);
@@ -360,7 +360,7 @@ void HipCSRMatMultiVec(void *A_, void *X_, void *Y_) {
     BX_REQUIRE(A && X && Y, "HipCSRMatMultiVec: NULL argument");
     same_shape(X, Y, "HipCSRMatMultiVec");
     BX_REQUIRE(X->size == A->n, "HipCSRMatMultiVec: block has %ld rows, matrix %ld", X->size, A->n);
-    spmm_csr(X->precision, A, X->data, X->size, colmap_of(X), Y->data, Y->size, colmap_of(Y));
+    spmm_csr(X->precision, A, X->data, X->ld, colmap_of(X), Y->data, Y->ld, colmap_of(Y));
 }
 
 hip_DiagMatrix *hip_DiagMatrixCreate(long n, const double *d) {
@@ -408,7 +408,7 @@ void HipDiagMatMultiVec(void *D_, void *X_, void *Y_) {
     BX_REQUIRE(D && X && Y, "HipDiagMatMultiVec: NULL argument");
     same_shape(X, Y, "HipDiagMatMultiVec");
     BX_REQUIRE(X->size == D->n, "HipDiagMatMultiVec: block has %ld rows, diagonal %ld", X->size, D->n);
-    diag_apply(X->precision, D, X->data, X->size, colmap_of(X), Y->data, Y->size, colmap_of(Y));
+    diag_apply(X->precision, D, X->data, X->ld, colmap_of(X), Y->data, Y->ld, colmap_of(Y));
 }
 
 }  // extern "C"

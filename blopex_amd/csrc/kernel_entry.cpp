@@ -1,0 +1,81 @@
+// Host-array wrappers of the MFMA building blocks (include/blopex_amd_kernels.h).
+#include <vector>
+
+#include "blopex_amd_kernels.h"
+#include "kernels.h"
+
+using namespace bx;
+
+namespace {
+
+struct DevBlock {
+    void *p = nullptr;
+    long ld = 0;
+    int w = 0;
+    DevBlock(long n, int w_, int prec, const double *host) : w(w_) {
+        ld = (n + 63) / 64 * 64;
+        const size_t bytes = (size_t)ld * (w > 0 ? w : 1) * elem_size(prec);
+        BX_HIP(hipMalloc(&p, bytes));
+        BX_HIP(hipMemsetAsync(p, 0, bytes, stream()));
+        if (host && w > 0) {
+            void *tmp = scratch(5, (size_t)n * w * sizeof(double));
+            BX_HIP(hipMemcpyAsync(tmp, host, (size_t)n * w * 8, hipMemcpyHostToDevice, stream()));
+            convert_block(n, w, tmp, n, HIPMV_FP64, p, ld, prec);
+        }
+    }
+    void download(long n, int prec, double *host) {
+        void *tmp = scratch(5, (size_t)n * w * sizeof(double));
+        convert_block(n, w, p, ld, prec, tmp, n, HIPMV_FP64);
+        download_sync(host, tmp, (size_t)n * w * sizeof(double));
+    }
+    ~DevBlock() {
+        hipStreamSynchronize(stream());
+        hipFree(p);
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int blopex_amd_kernel_update(int prec, long n, int k, int a, int p, const double *X,
+                             const double *R, const double *P, const double *C, double *full,
+                             double *tail) {
+    DevBlock dX(n, k, prec, X), dR(n, a, prec, R), dP(n, p, prec, P), dT(n, k, prec, nullptr);
+    const int crows = k + a + p;
+    double *dC = (double *)scratch(8, (size_t)crows * k * sizeof(double));
+    upload_async(dC, C, (size_t)crows * k * sizeof(double));
+    UpdGroup g{};
+    g.in[0] = dX.p;
+    g.h[0] = k;
+    g.crow[0] = 0;
+    if (a) {
+        g.in[1] = dR.p;
+        g.h[1] = a;
+        g.crow[1] = k;
+    }
+    if (p) {
+        g.in[2] = dP.p;
+        g.h[2] = p;
+        g.crow[2] = k + a;
+    }
+    g.out_tail = dT.p;
+    g.out_full = dX.p;  // in place, as the solver uses it
+    block_update(prec, dX.ld, &g, 1, dC, crows, crows, k, nullptr);
+    dX.download(n, prec, full);
+    if (tail) dT.download(n, prec, tail);
+    return 0;
+}
+
+int blopex_amd_kernel_gram(int prec, long n, int lw, int qw, const double *L, const double *Q,
+                           double *G) {
+    DevBlock dL(n, lw, prec, L), dQ(n, qw, prec, Q);
+    double *dG = (double *)scratch(8, (size_t)lw * qw * sizeof(double));
+    GramBatch gb(prec, dL.ld, lw);
+    gb.add(dL.p, lw, dQ.p, qw, dG);
+    gb.flush();
+    download_sync(G, dG, (size_t)lw * qw * sizeof(double));
+    return 0;
+}
+
+}  // extern "C"

@@ -148,9 +148,11 @@ class Phase {
         noT_ = opT == nullptr;
         es_ = elem_size(prec);
         ld3_ = 3 * k;
-        const size_t blk = (size_t)n * k * es_;
-        int nblk = noB_ ? 7 : 10;
+        ld_ = (n + 63) / 64 * 64;  // padded leading dimension; padding rows stay zero
+        const size_t blk = (size_t)ld_ * k * es_;
+        const int nblk = noB_ ? 7 : 10;
         BX_HIP(hipMalloc(&pool_, blk * nblk + 256));
+        BX_HIP(hipMemsetAsync(pool_, 0, blk * nblk + 256, stream()));
         char *p = (char *)pool_;
         for (int i = 0; i < nblk; i++) bufs_.push_back(p + i * blk);
         X_ = bufs_[0];
@@ -165,21 +167,25 @@ class Phase {
             BR_ = bufs_[8];
             BP_ = bufs_[9];
         }
-        const size_t small = (size_t)ld3_ * ld3_ * 3 + (size_t)ld3_ * k * 2 + 8 * (size_t)k + 64;
+        const size_t small = (size_t)ld3_ * ld3_ * 2 + (size_t)ld3_ * k + 6 * (size_t)k + 64;
         BX_HIP(hipMalloc(&dsmall_, small * sizeof(double) + sizeof(DevStatus)));
         GA_ = dsmall_;
         GB_ = GA_ + (size_t)ld3_ * ld3_;
         coef_ = GB_ + (size_t)ld3_ * ld3_;
-        lam_ = coef_ + (size_t)ld3_ * k;
+        lam_ = coef_ + (size_t)ld3_ * k;  // lam_, lam_new_, rn2_ (2k), status: one readback
         lam_new_ = lam_ + k;
         rn2_ = lam_new_ + k;
         st_ = (DevStatus *)(rn2_ + 2 * k);
+        tail_ = lam_;
+        tail_bytes_ = 4 * (size_t)k * sizeof(double) + sizeof(DevStatus);
         BX_HIP(hipMemsetAsync(dsmall_, 0, small * sizeof(double) + sizeof(DevStatus), stream()));
         if (ny_ > 0) {
-            BX_HIP(hipMalloc(&ypool_, (size_t)n * ny_ * es_ * 2 + ((size_t)ny_ * ny_ + 2 * (size_t)ny_ * k) * 8));
+            const size_t yb = (size_t)ld_ * ny_ * es_;
+            BX_HIP(hipMalloc(&ypool_, yb * 2 + ((size_t)ny_ * ny_ + 2 * (size_t)ny_ * k) * 8));
+            BX_HIP(hipMemsetAsync(ypool_, 0, yb * 2, stream()));
             Y_ = ypool_;
-            BY_ = noB_ ? Y_ : (char *)ypool_ + (size_t)n * ny_ * es_;
-            yby_ = (double *)((char *)ypool_ + (size_t)n * ny_ * es_ * 2);
+            BY_ = noB_ ? Y_ : (char *)ypool_ + yb;
+            yby_ = (double *)((char *)ypool_ + yb * 2);
             ybx_ = yby_ + (size_t)ny_ * ny_;
             tybx_ = ybx_ + (size_t)ny_ * k;
         }
@@ -192,12 +198,16 @@ class Phase {
     }
 
     // Returns the exit flag; *it_io is the iteration count so far (in) and
-    // after this phase (out).  stop_mixed: FP32 phase ending criterion active.
+    // after this phase (out).  fp32_phase: stop when FP32 has done its part.
     int run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_LobpcgSettings &set,
             bool *switched);
 
   private:
-    hip_Multi_Vector V(void *p, int w) { return view(p, n_, w, prec_); }
+    hip_Multi_Vector V(void *p, int w) {
+        hip_Multi_Vector v = view(p, n_, w, prec_);
+        v.ld = ld_;
+        return v;
+    }
     void apply(Op op, void *d, void *x, void *y, int w) {
         hip_Multi_Vector vx = V(x, w), vy = V(y, w);
         const bool isA = op == opA_ && d == Ad_;
@@ -210,26 +220,30 @@ class Phase {
         op(d, &vx, &vy);
         tm_->end(bytes);
     }
-    void gram(const void *x, int wx, const void *y, int wy, double *g) {
+    GramBatch gram_batch() { return GramBatch(prec_, ld_, ld3_); }
+    void run_gram(GramBatch &gb) {
         tm_->begin(kGram);
-        gram_cols(prec_, n_, x, n_, colmap_identity(wx), y, n_, colmap_identity(wy), g, ld3_);
+        gb.flush();
         tm_->end();
     }
-    // y = x * c (c rows h, cols w, leading dim ld3) [+ y]
-    void vecmat(const void *x, int h, const double *c, void *y, int w, bool acc = false) {
+    void update(const UpdGroup *g, int ng, const double *C, int crows, int w) {
         tm_->begin(kUpdate);
-        vecmat_cols(prec_, n_, x, n_, colmap_identity(h), c, ld3_, y, n_, colmap_identity(w), acc);
+        block_update(prec_, ld_, g, ng, C, ld3_, crows, w, st_);
         tm_->end();
     }
-    void dense_chol(double *g, int m, int which) {
-        tm_->begin(kDense);
-        chol_inv(g, ld3_, m, st_, which);
-        tm_->end();
+    // x <- x * c (in place), c a w x w block of the Gram buffer
+    void rotate(void *x, const double *c, int w) {
+        UpdGroup g{};
+        g.in[0] = x;
+        g.h[0] = w;
+        g.out_full = x;
+        update(&g, 1, c, w, w);
     }
-    void dense_rr(int a, int p, int initial) {
-        tm_->begin(kDense);
-        rr_solve(GA_, GB_, ld3_, k_, a, p, initial, lam_, lam_new_, coef_, ld3_, st_);
-        tm_->end();
+    void read_tail(std::vector<double> &buf, DevStatus *hs) {
+        buf.resize(tail_bytes_ / sizeof(double) + 1);
+        download_sync(buf.data(), tail_, tail_bytes_);
+        memcpy(hs, (const char *)buf.data() + 4 * (size_t)k_ * sizeof(double), sizeof(DevStatus));
+        tm_->resolve();
     }
     void read_status(DevStatus *h) {
         download_sync(h, st_, sizeof(DevStatus));
@@ -242,7 +256,7 @@ class Phase {
                 ColMap a = colmap_identity(1), b = colmap_identity(1);
                 a.idx[0] = src[t];
                 b.idx[0] = (int)t;
-                copy_cols(prec_, n_, buf, n_, a, buf, n_, b);
+                copy_cols(prec_, n_, buf, ld_, a, buf, ld_, b);
             }
     }
     void project_constraints(void *blk, int w);
@@ -280,21 +294,27 @@ class Phase {
     bool noB_, noT_;
     size_t es_;
     int ld3_;
+    long ld_;
     void *pool_ = nullptr, *ypool_ = nullptr;
     std::vector<void *> bufs_;
     void *X_, *AX_, *BX_ = nullptr, *R_, *AR_, *BR_ = nullptr, *P_, *AP_, *BP_ = nullptr, *W_;
     void *Y_ = nullptr, *BY_ = nullptr;
     double *dsmall_ = nullptr, *GA_, *GB_, *coef_, *lam_, *lam_new_, *rn2_;
+    double *tail_;
+    size_t tail_bytes_;
     double *yby_ = nullptr, *ybx_ = nullptr, *tybx_ = nullptr;
     DevStatus *st_;
 };
 
 // blk(:, 0:w) -= Y (Y'BY)^{-1} (BY)' blk   (krylov/lobpcg.c:741-750, 996-1012)
 void Phase::project_constraints(void *blk, int w) {
-    gram_cols(prec_, n_, BY_, n_, colmap_identity(ny_), blk, n_, colmap_identity(w), ybx_, ny_);
+    tm_->begin(kOther);
+    gram_cols(prec_, n_, BY_, ld_, colmap_identity(ny_), blk, ld_, colmap_identity(w), ybx_, ny_);
     small_mm(1.0, yby_, ny_, true, ybx_, ny_, false, tybx_, ny_, ny_, w, ny_);
     small_mm(-1.0, yby_, ny_, false, tybx_, ny_, false, ybx_, ny_, ny_, w, ny_);
-    vecmat_cols(prec_, n_, Y_, n_, colmap_identity(ny_), ybx_, ny_, blk, n_, colmap_identity(w), true);
+    vecmat_cols(prec_, n_, Y_, ld_, colmap_identity(ny_), ybx_, ny_, blk, ld_, colmap_identity(w),
+                true);
+    tm_->end();
 }
 
 int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_LobpcgSettings &set,
@@ -302,13 +322,14 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
     *switched = false;
     const int k = k_;
     DevStatus hs;
-    hipStream_t s = stream();
-    // ---- working copy of X (and Y) ----
-    convert_block(n_, k, Xuser, n_, HIPMV_FP64, X_, n_, prec_);
+    std::vector<double> tail;
+    // ---- working copy of X (and Y) in the padded layout ----
+    convert_block(n_, k, Xuser, n_, HIPMV_FP64, X_, ld_, prec_);
     if (ny_ > 0) {
-        convert_block(n_, ny_, Yu_->data, n_, Yu_->precision, Y_, n_, prec_);
+        convert_block(n_, ny_, Yu_->data, Yu_->ld, Yu_->precision, Y_, ld_, prec_);
         if (!noB_) apply(opB_, Bd_, Y_, BY_, ny_);
-        gram_cols(prec_, n_, BY_, n_, colmap_identity(ny_), Y_, n_, colmap_identity(ny_), yby_, ny_);
+        gram_cols(prec_, n_, BY_, ld_, colmap_identity(ny_), Y_, ld_, colmap_identity(ny_), yby_,
+                  ny_);
         chol_inv(yby_, ny_, ny_, st_, kCholX);
         read_status(&hs);
         if (hs.abort) {
@@ -318,11 +339,17 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
         project_constraints(X_, k);
     }
     int flag = 0;
-    // ---- initial Rayleigh-Ritz ----
+    // ---- initial Rayleigh-Ritz in span(X) (lobpcg.c:840-960) ----
     if (!noB_) apply(opB_, Bd_, X_, BX_, k);
     void *BXp = noB_ ? X_ : BX_;
-    gram(X_, k, BXp, k, GB_);
-    dense_chol(GB_, k, kCholX);
+    {
+        GramBatch gb = gram_batch();
+        gb.add(X_, k, BXp, k, GB_);
+        run_gram(gb);
+    }
+    tm_->begin(kDense);
+    chol_inv(GB_, ld3_, k, st_, kCholX);
+    tm_->end();
     read_status(&hs);
     if (hs.abort) {
         flag = hs.x_info;
@@ -331,18 +358,34 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
             printf("DPOTRF INFO = %d\n", flag);
         }
     } else {
-        vecmat(X_, k, GB_, W_, k);
-        std::swap(X_, W_);
-        if (!noB_) {
-            vecmat(BX_, k, GB_, W_, k);
-            std::swap(BX_, W_);
-        }
-        BXp = noB_ ? X_ : BX_;
+        rotate(X_, GB_, k);
+        if (!noB_) rotate(BX_, GB_, k);
         apply(opA_, Ad_, X_, AX_, k);
-        gram(X_, k, AX_, k, GA_);
-        gram(X_, k, BXp, k, GB_);
-        dense_rr(0, 0, 1);
-        read_status(&hs);
+        {
+            GramBatch gb = gram_batch();
+            gb.add(X_, k, AX_, k, GA_);
+            gb.add(X_, k, BXp, k, GB_);
+            run_gram(gb);
+        }
+        tm_->begin(kDense);
+        rr_solve(GA_, GB_, ld3_, k, 0, 0, 1, lam_, lam_new_, coef_, ld3_, st_);
+        tm_->end();
+        UpdGroup g[3] = {};
+        int ng = 0;
+        for (void *b : {X_, AX_, noB_ ? nullptr : BX_}) {
+            if (!b) continue;
+            g[ng].in[0] = b;
+            g[ng].h[0] = k;
+            g[ng].out_full = b;
+            ng++;
+        }
+        update(g, ng, coef_, k, k);
+        ColMap all = colmap_identity(k);
+        tm_->begin(kOther);
+        residual_cols(prec_, n_, BXp, AX_, ld_, all, lam_new_, R_, ld_);
+        diagdot_cols(prec_, n_, R_, ld_, all, R_, ld_, all, rn2_);
+        tm_->end();
+        read_tail(tail, &hs);
         if (hs.abort) {
             flag = hs.gevp_info;
             if (verb_) {
@@ -352,24 +395,12 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
             }
         } else {
             std::swap(lam_, lam_new_);
-            vecmat(X_, k, coef_, W_, k);
-            std::swap(X_, W_);
-            vecmat(AX_, k, coef_, W_, k);
-            std::swap(AX_, W_);
-            if (!noB_) {
-                vecmat(BX_, k, coef_, W_, k);
-                std::swap(BX_, W_);
+            const double *lam_h = tail.data() + (lam_ - tail_);
+            const double *rn_h = tail.data() + 2 * k;
+            for (int i = 0; i < k; i++) {
+                out_.lam[i] = lam_h[i];
+                out_.res[i] = sqrt(fabs(rn_h[i]));
             }
-            BXp = noB_ ? X_ : BX_;
-            ColMap all = colmap_identity(k);
-            tm_->begin(kOther);
-            residual_cols(prec_, n_, BXp, AX_, n_, all, lam_, R_, n_);
-            diagdot_cols(prec_, n_, R_, n_, all, R_, n_, all, rn2_);
-            tm_->end();
-            std::vector<double> rn(k);
-            download_sync(out_.lam, lam_, sizeof(double) * k);
-            download_sync(rn.data(), rn2_, sizeof(double) * k);
-            for (int i = 0; i < k; i++) out_.res[i] = sqrt(fabs(rn[i]));
             store_hist(*it_io);
             if (verb_ == 2)
                 dump(*it_io);
@@ -378,7 +409,7 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
         }
     }
 
-    // ---- main loop ----
+    // ---- main loop (lobpcg.c:965-1318) ----
     std::vector<int> mask(k, 1), act, prev_act(k);
     for (int i = 0; i < k; i++) prev_act[i] = i;
     int local_it = 0;
@@ -416,12 +447,14 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
                 while (prev_act[q] != c) q++;
                 src.push_back((int)q);
             }
+            tm_->begin(kOther);
             gather(R_, src);
-        }
-        if (local_it > 0 && nr < k) {
-            gather(P_, act);
-            gather(AP_, act);
-            if (!noB_) gather(BP_, act);
+            if (local_it > 0) {
+                gather(P_, act);
+                gather(AP_, act);
+                if (!noB_) gather(BP_, act);
+            }
+            tm_->end();
         }
         if (!noT_) {
             apply(opT_, Td_, R_, W_, nr);
@@ -430,48 +463,59 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
         if (ny_ > 0) project_constraints(R_, nr);
         const int r0 = k, p0 = k + nr;
         if (!noB_) apply(opB_, Bd_, R_, BR_, nr);
-        void *BRp = noB_ ? R_ : BR_;
-        double *ur = GB_ + r0 + (size_t)r0 * ld3_;
-        gram(R_, nr, BRp, nr, ur);
-        dense_chol(ur, nr, kCholR);
-        vecmat(R_, nr, ur, W_, nr);
-        std::swap(R_, W_);
-        if (!noB_) {
-            vecmat(BR_, nr, ur, W_, nr);
-            std::swap(BR_, W_);
-        }
-        BRp = noB_ ? R_ : BR_;
         apply(opA_, Ad_, R_, AR_, nr);
         const int preq = local_it > 0 ? nr : 0;
-        if (preq) {
-            void *BPp = noB_ ? P_ : BP_;
-            double *up = GB_ + p0 + (size_t)p0 * ld3_;
-            gram(P_, nr, BPp, nr, up);
-            dense_chol(up, nr, kCholP);
-            vecmat(P_, nr, up, W_, nr);
-            std::swap(P_, W_);
-            if (!noB_) {
-                vecmat(BP_, nr, up, W_, nr);
-                std::swap(BP_, W_);
-            }
-            vecmat(AP_, nr, up, W_, nr);
-            std::swap(AP_, W_);
-        }
+        void *BRp = noB_ ? R_ : BR_, *BPp = noB_ ? P_ : BP_;
         BXp = noB_ ? X_ : BX_;
-        gram(R_, nr, AX_, k, GA_ + r0);
-        gram(R_, nr, AR_, nr, GA_ + r0 + (size_t)r0 * ld3_);
-        gram(R_, nr, BXp, k, GB_ + r0);
-        if (preq) {
-            void *BPp = noB_ ? P_ : BP_;
-            gram(P_, nr, AX_, k, GA_ + p0);
-            gram(P_, nr, AR_, nr, GA_ + p0 + (size_t)r0 * ld3_);
-            gram(P_, nr, AP_, nr, GA_ + p0 + (size_t)p0 * ld3_);
-            gram(P_, nr, BXp, k, GB_ + p0);
-            gram(P_, nr, BRp, nr, GB_ + p0 + (size_t)r0 * ld3_);
-            (void)BPp;
+        {
+            // every Gram block of the iteration in one pass over the rows
+            GramBatch gb = gram_batch();
+            gb.add(R_, nr, AX_, k, GA_ + r0);
+            gb.add(R_, nr, AR_, nr, GA_ + r0 + (size_t)r0 * ld3_);
+            gb.add(R_, nr, BXp, k, GB_ + r0);
+            gb.add(R_, nr, BRp, nr, GB_ + r0 + (size_t)r0 * ld3_);
+            if (preq) {
+                gb.add(P_, nr, AX_, k, GA_ + p0);
+                gb.add(P_, nr, AR_, nr, GA_ + p0 + (size_t)r0 * ld3_);
+                gb.add(P_, nr, AP_, nr, GA_ + p0 + (size_t)p0 * ld3_);
+                gb.add(P_, nr, BXp, k, GB_ + p0);
+                gb.add(P_, nr, BRp, nr, GB_ + p0 + (size_t)r0 * ld3_);
+                gb.add(P_, nr, BPp, nr, GB_ + p0 + (size_t)p0 * ld3_);
+            }
+            run_gram(gb);
         }
-        dense_rr(nr, preq, 0);
-        read_status(&hs);
+        tm_->begin(kDense);
+        rr_solve_implicit(GA_, GB_, ld3_, k, nr, preq, lam_, lam_new_, coef_, ld3_, st_);
+        tm_->end();
+        // P' = R C_R + P C_P and X' = X C_X + P' (likewise A-, B-blocks), in place
+        UpdGroup g[3] = {};
+        int ng = 0;
+        void *trip[3][3] = {{X_, R_, P_}, {AX_, AR_, AP_}, {BX_, BR_, BP_}};
+        for (int q = 0; q < (noB_ ? 2 : 3); q++) {
+            g[ng].in[0] = trip[q][0];
+            g[ng].h[0] = k;
+            g[ng].crow[0] = 0;
+            g[ng].in[1] = trip[q][1];
+            g[ng].h[1] = nr;
+            g[ng].crow[1] = r0;
+            if (preq) {
+                g[ng].in[2] = trip[q][2];
+                g[ng].h[2] = nr;
+                g[ng].crow[2] = p0;
+            }
+            g[ng].out_tail = trip[q][2];
+            g[ng].out_full = trip[q][0];
+            ng++;
+        }
+        update(g, ng, coef_, k + nr + preq, k);
+        BXp = noB_ ? X_ : BX_;
+        ColMap am = colmap_identity(nr);
+        for (int t = 0; t < nr; t++) am.idx[t] = act[t];
+        tm_->begin(kOther);
+        residual_cols(prec_, n_, BXp, AX_, ld_, am, lam_new_, R_, ld_);
+        diagdot_cols(prec_, n_, R_, ld_, colmap_identity(nr), R_, ld_, colmap_identity(nr), rn2_);
+        tm_->end();
+        read_tail(tail, &hs);
         if (hs.abort) {
             if (hs.r_info) {
                 flag = hs.r_info;
@@ -489,45 +533,11 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
             }
             break;
         }
-        const int np = (preq && hs.p_info == 0) ? nr : 0;
         std::swap(lam_, lam_new_);
-        const double *cX = coef_, *cR = coef_ + r0, *cP = coef_ + p0;
-        // P = P cP + R cR (all k columns), likewise AP, BP
-        auto dir = [&](void *&Pb, void *Rb) {
-            if (np) {
-                vecmat(Pb, np, cP, W_, k);
-                vecmat(Rb, nr, cR, W_, k, true);
-            } else {
-                vecmat(Rb, nr, cR, W_, k);
-            }
-            std::swap(Pb, W_);
-        };
-        dir(P_, R_);
-        dir(AP_, AR_);
-        if (!noB_) dir(BP_, BR_);
-        // X = X cX + P, likewise AX, BX
-        auto upd = [&](void *&Xb, void *Pb) {
-            vecmat(Xb, k, cX, W_, k);
-            tm_->begin(kUpdate);
-            axpy_cols(prec_, n_, 1.0, Pb, n_, colmap_identity(k), W_, n_, colmap_identity(k));
-            tm_->end();
-            std::swap(Xb, W_);
-        };
-        upd(X_, P_);
-        upd(AX_, AP_);
-        if (!noB_) upd(BX_, BP_);
-        BXp = noB_ ? X_ : BX_;
-        ColMap am = colmap_identity(nr);
-        for (int t = 0; t < nr; t++) am.idx[t] = act[t];
-        tm_->begin(kOther);
-        residual_cols(prec_, n_, BXp, AX_, n_, am, lam_, R_, n_);
-        diagdot_cols(prec_, n_, R_, n_, colmap_identity(nr), R_, n_, colmap_identity(nr), rn2_);
-        tm_->end();
-        std::vector<double> rn(nr);
-        download_sync(out_.lam, lam_, sizeof(double) * k);
-        download_sync(rn.data(), rn2_, sizeof(double) * nr);
-        tm_->resolve();
-        for (int t = 0; t < nr; t++) out_.res[act[t]] = sqrt(fabs(rn[t]));
+        const double *lam_h = tail.data() + (lam_ - tail_);
+        const double *rn_h = tail.data() + 2 * k;
+        for (int i = 0; i < k; i++) out_.lam[i] = lam_h[i];
+        for (int t = 0; t < nr; t++) out_.res[act[t]] = sqrt(fabs(rn_h[t]));
         store_hist(it);
         if (verb_ == 2) {
             printf("Iteration %d \tbsize %d\n", it, nr);
@@ -540,8 +550,8 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
     }
     *it_io = it - 1;
     // ---- copy the block back ----
-    convert_block(n_, k, X_, n_, prec_, Xuser, n_, HIPMV_FP64);
-    BX_HIP(hipStreamSynchronize(s));
+    convert_block(n_, k, X_, ld_, prec_, Xuser, n_, HIPMV_FP64);
+    BX_HIP(hipStreamSynchronize(stream()));
     if (*switched) return 0;
     if (flag != 0 || it > maxit) return REQUESTED_ACCURACY_NOT_ACHIEVED;
     return 0;

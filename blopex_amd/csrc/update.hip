@@ -1,0 +1,153 @@
+// Block update on MFMA: for each group g,
+//     tail = in1 * C[crow1 : crow1+h1, :] + in2 * C[crow2 : crow2+h2, :]
+//     full = tail + in0 * C[crow0 : crow0+h0, :]
+// written to out_tail / out_full (either may be absent).  With in0 = X,
+// in1 = R, in2 = P this is LOBPCG's P' = R C_R + P C_P and X' = X C_X + P'
+// in one pass over [X R P] (krylov/lobpcg.c:1200-1272, there 9-14
+// MultiVecMat/Axpy/Copy calls); with one input it is the B-orthonormalising
+// X <- X U^{-1} and the Ritz rotation X <- X C.
+//
+// Layout: each wave owns 64 consecutive rows r0..r0+63 and computes them for
+// every output column, as four interleaved 16-row MFMA tiles (tile t holds
+// rows r0 + 4j + t), so a lane reads and writes 4 consecutive rows of one
+// column with one vector access.  The transposed product is formed,
+//     D'(16 cols x 16 rows) += C'(16 x 4) * In'(4 x 16 rows),
+// so C comes in as the A operand (from LDS) and the block rows as B.
+// Rows a wave owns are read completely before they are written, so every
+// output may alias the input of the same group it replaces.
+#include "kernels.h"
+#include "mfma.h"
+
+namespace bx {
+
+namespace {
+
+constexpr int kUT = 256;  // threads per workgroup (4 waves)
+
+struct UpdKArgs {
+    UpdGroup g[3];
+    const double *C;
+    int ldc;
+    int crows;
+    int w;
+    long ld;
+    const DevStatus *st;
+};
+
+template <typename T, int NOT>
+__device__ __forceinline__ void upd_block(const T *in, int h, int crow, const T *Cl, long ld,
+                                          long r0, int li, int lk,
+                                          typename Mfma<T>::acc (&acc)[NOT][4]) {
+    constexpr int WT = NOT * 16;
+    for (int s = 0; s < h; s += 4) {
+        const int kc = s + lk;
+        T v[4] = {T(0), T(0), T(0), T(0)};
+        if (kc < h) load4(in + (size_t)kc * ld + r0, v);
+#pragma unroll
+        for (int o = 0; o < NOT; o++) {
+            const T cf = (kc < h) ? Cl[(crow + kc) * WT + o * 16 + li] : T(0);
+#pragma unroll
+            for (int t = 0; t < 4; t++) acc[o][t] = Mfma<T>::mma(cf, v[t], acc[o][t]);
+        }
+    }
+}
+
+template <typename T, int NOT>
+__device__ __forceinline__ void upd_store(T *out, int w, long ld, long r0, int lk,
+                                          const typename Mfma<T>::acc (&acc)[NOT][4]) {
+#pragma unroll
+    for (int o = 0; o < NOT; o++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int c = o * 16 + 4 * lk + r;
+            if (c < w) store4(out + (size_t)c * ld + r0, acc[o][0][r], acc[o][1][r], acc[o][2][r],
+                              acc[o][3][r]);
+        }
+}
+
+template <typename T, int NOT>
+__global__ void __launch_bounds__(kUT) k_update(UpdKArgs a) {
+    extern __shared__ unsigned char smem_raw[];
+    T *Cl = reinterpret_cast<T *>(smem_raw);
+    constexpr int WT = NOT * 16;
+    if (a.st && a.st->abort) return;
+    const UpdGroup &g = a.g[blockIdx.y];
+    for (int e = threadIdx.x; e < a.crows * WT; e += kUT) {
+        const int r = e / WT, c = e % WT;
+        Cl[e] = (c < a.w) ? (T)a.C[r + (size_t)c * a.ldc] : T(0);
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const long nchunks = a.ld / 64;
+    const T *in0 = (const T *)g.in[0], *in1 = (const T *)g.in[1], *in2 = (const T *)g.in[2];
+    for (long ch = (long)blockIdx.x * 4 + wave; ch < nchunks; ch += (long)gridDim.x * 4) {
+        const long r0 = ch * 64 + 4 * li;
+        typename Mfma<T>::acc acc[NOT][4];
+#pragma unroll
+        for (int o = 0; o < NOT; o++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) acc[o][t] = typename Mfma<T>::acc{};
+        if (in1) upd_block<T, NOT>(in1, g.h[1], g.crow[1], Cl, a.ld, r0, li, lk, acc);
+        if (in2) upd_block<T, NOT>(in2, g.h[2], g.crow[2], Cl, a.ld, r0, li, lk, acc);
+        if (g.out_tail) upd_store<T, NOT>((T *)g.out_tail, a.w, a.ld, r0, lk, acc);
+        if (in0) upd_block<T, NOT>(in0, g.h[0], g.crow[0], Cl, a.ld, r0, li, lk, acc);
+        if (g.out_full) upd_store<T, NOT>((T *)g.out_full, a.w, a.ld, r0, lk, acc);
+    }
+}
+
+template <typename T, int NOT>
+void launch_update(const UpdKArgs &a, int ngroups) {
+    const size_t lds = (size_t)a.crows * NOT * 16 * sizeof(T);
+    static bool attr = false;
+    if (!attr) {
+        BX_HIP(hipFuncSetAttribute((const void *)k_update<T, NOT>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024));
+        attr = true;
+    }
+    const long nchunks = a.ld / 64;
+    long blocks = (nchunks + 3) / 4;
+    const long cap = (long)num_cus() * 4;
+    if (blocks > cap) blocks = cap;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL((k_update<T, NOT>), dim3((unsigned)blocks, ngroups), dim3(kUT), lds, stream(),
+                       a);
+    BX_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void block_update(int prec, long ld, const UpdGroup *groups, int ngroups, const double *C, int ldc,
+                  int crows, int w, const DevStatus *st) {
+    BX_REQUIRE(ld % 64 == 0, "block_update: leading dimension %ld not a multiple of 64", ld);
+    BX_REQUIRE(ngroups >= 1 && ngroups <= 3, "block_update: %d groups", ngroups);
+    BX_REQUIRE(w >= 1 && w <= 64, "block_update: %d output columns (max 64)", w);
+    UpdKArgs a;
+    for (int i = 0; i < ngroups; i++) {
+        a.g[i] = groups[i];
+        for (int b = 0; b < 3; b++) {
+            if (!a.g[i].in[b]) continue;
+            BX_REQUIRE(a.g[i].crow[b] >= 0 && a.g[i].crow[b] + a.g[i].h[b] <= crows,
+                       "block_update: coefficient rows %d..%d outside %d", a.g[i].crow[b],
+                       a.g[i].crow[b] + a.g[i].h[b], crows);
+        }
+    }
+    a.C = C;
+    a.ldc = ldc;
+    a.crows = crows;
+    a.w = w;
+    a.ld = ld;
+    a.st = st;
+    const int NOT = (w + 15) / 16;
+    if (prec == HIPMV_FP32) {
+        if (NOT == 1) launch_update<float, 1>(a, ngroups);
+        else if (NOT == 2) launch_update<float, 2>(a, ngroups);
+        else launch_update<float, 4>(a, ngroups);
+    } else {
+        if (NOT == 1) launch_update<double, 1>(a, ngroups);
+        else if (NOT == 2) launch_update<double, 2>(a, ngroups);
+        else launch_update<double, 4>(a, ngroups);
+    }
+}
+
+}  // namespace bx

@@ -1,0 +1,31 @@
+/*
+ * Kernel-level entry points of libblopex_amd.so: the MFMA building blocks of
+ * the device-resident solver, callable on host arrays for tests and tuning.
+ * They have no counterpart in the reference (whose block operations are the
+ * per-column loops of blopex_serial_double/multivector/multi_vector.c:
+ * 354-524); each allocates, runs the kernel on the current device and
+ * copies the result back.  All arrays are column-major FP64 on the host;
+ * precision selects the device arithmetic (HIPMV_FP64 / HIPMV_FP32).
+ */
+#ifndef BLOPEX_AMD_KERNELS_H
+#define BLOPEX_AMD_KERNELS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* tail = R*C[k:k+a,:] + P*C[k+a:k+a+p,:], full = tail + X*C[0:k,:]
+   (X n x k, R n x a, P n x p, C (k+a+p) x k); a or p may be 0. */
+int blopex_amd_kernel_update(int precision, long n, int k, int a, int p, const double *X,
+                             const double *R, const double *P, const double *C, double *full,
+                             double *tail);
+
+/* G (lw x qw) = L' * Q for L n x lw, Q n x qw. */
+int blopex_amd_kernel_gram(int precision, long n, int lw, int qw, const double *L, const double *Q,
+                           double *G);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -70,8 +70,9 @@ typedef struct hip_LobpcgStats {
 } hip_LobpcgStats;
 
 typedef struct hip_Multi_Vector {
-    void *data;                    /* device memory, column-major, leading dimension = size */
+    void *data;                    /* device memory, column-major */
     long size;                     /* rows */
+    long ld;                       /* leading dimension (>= size); callbacks touch rows < size */
     BlopexInt num_vectors;         /* columns */
     BlopexInt num_active_vectors;
     BlopexInt *active_indices;     /* host, 0-based, ascending */

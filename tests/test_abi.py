@@ -51,10 +51,10 @@ def test_struct_layouts():
     assert ctypes.sizeof(L.FortranMatrix) == 40
     assert ctypes.sizeof(L.Tolerance) == 16
     assert ctypes.sizeof(L.BlasLapack) == 32
-    # hip_Multi_Vector: data, size, 2 ints, ptr, 2 ints, settings(32), stats(80)
-    assert L.HipMultiVector.settings.offset == 40
+    # hip_Multi_Vector: data, size, ld, 2 ints, ptr, 2 ints, settings(32), stats(80)
+    assert L.HipMultiVector.settings.offset == 48
     assert ctypes.sizeof(L.Settings) == 32 and ctypes.sizeof(L.Stats) == 80
-    assert ctypes.sizeof(L.HipMultiVector) == 40 + 32 + 80
+    assert ctypes.sizeof(L.HipMultiVector) == 48 + 32 + 80
 
 
 def test_fortran_matrix_host_routines(lib):

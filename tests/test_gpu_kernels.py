@@ -1,0 +1,64 @@
+"""GPU checks of the MFMA building blocks against numpy FP64 (host arrays
+through include/blopex_amd_kernels.h).
+
+Bars: FP64 kernels within 1e-13 of numpy relative to the sum of absolute
+terms; FP32 kernels (FP32 inputs, FP32 MFMA, FP64 accumulation across 64-row
+chunks for the Gram) within 64-96 FP32 roundings (4e-6 / 8e-6) of the same scale."""
+import sys
+
+import numpy as np
+import pytest
+
+from blopex_amd import _lib as L
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def f(a):
+    return np.asfortranarray(a, dtype=np.float64)
+
+
+@pytest.mark.parametrize("prec", [L.HIPMV_FP64, L.HIPMV_FP32])
+@pytest.mark.parametrize("n,k,a,p", [(1000, 20, 5, 13), (4096, 32, 32, 32), (777, 8, 8, 0),
+                                     (130, 64, 64, 64), (64, 3, 2, 1)])
+def test_block_update(prec, n, k, a, p):
+    say(f"block update prec={prec} n={n} k={k} a={a} p={p}")
+    rng = np.random.default_rng(n + k)
+    X, R, P = (f(rng.standard_normal((n, w))) for w in (k, a, p))
+    C = f(rng.standard_normal((k + a + p, k)))
+    if prec == L.HIPMV_FP32:
+        X, R, P = (f(v.astype(np.float32)) for v in (X, R, P))
+    full = np.zeros((n, k), order="F")
+    tail = np.zeros((n, k), order="F")
+    L.load().blopex_amd_kernel_update(prec, n, k, a, p, X.ctypes.data, R.ctypes.data,
+                                      P.ctypes.data, C.ctypes.data, full.ctypes.data,
+                                      tail.ctypes.data)
+    ref_tail = R @ C[k:k + a] + P @ C[k + a:]
+    ref_full = ref_tail + X @ C[:k]
+    scale = np.abs(X) @ np.abs(C[:k]) + np.abs(R) @ np.abs(C[k:k + a]) + np.abs(P) @ np.abs(C[k + a:])
+    tol = 1e-13 if prec == L.HIPMV_FP64 else 8e-6
+    if a + p:
+        assert np.max(np.abs(tail - ref_tail) / (scale + 1e-300)) < tol
+    assert np.max(np.abs(full - ref_full) / (scale + 1e-300)) < tol
+
+
+@pytest.mark.parametrize("prec", [L.HIPMV_FP64, L.HIPMV_FP32])
+@pytest.mark.parametrize("n,lw,qw", [(1000, 20, 13), (70000, 32, 32), (257, 64, 96), (64, 1, 1),
+                                     (5000, 96, 64)])
+def test_gram(prec, n, lw, qw):
+    say(f"gram prec={prec} n={n} lw={lw} qw={qw}")
+    rng = np.random.default_rng(n + lw)
+    Lm, Qm = f(rng.standard_normal((n, lw))), f(rng.standard_normal((n, qw)))
+    if prec == L.HIPMV_FP32:
+        Lm, Qm = f(Lm.astype(np.float32)), f(Qm.astype(np.float32))
+    G = np.zeros((lw, qw), order="F")
+    L.load().blopex_amd_kernel_gram(prec, n, lw, qw, Lm.ctypes.data, Qm.ctypes.data, G.ctypes.data)
+    ref = Lm.T @ Qm
+    scale = np.abs(Lm).T @ np.abs(Qm)
+    tol = 1e-14 if prec == L.HIPMV_FP64 else 4e-6
+    assert np.max(np.abs(G - ref) / scale) < tol
