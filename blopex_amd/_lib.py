@@ -125,6 +125,9 @@ _SIGS = {
     "blopex_amd_kernel_update": (c_int, [c_int, c_long, c_int, c_int, c_int, c_void_p, c_void_p,
                                          c_void_p, c_void_p, c_void_p, c_void_p]),
     "blopex_amd_kernel_gram": (c_int, [c_int, c_long, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    "blopex_amd_kernel_mfma_probe": (c_int, [c_int, c_void_p, c_void_p]),
+    "blopex_amd_kernel_rr": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p,
+                                     c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "utilities_FortranMatrixUpperInv": (None, [P(FortranMatrix)]),
     "utilities_FortranMatrixSymmetrize": (None, [P(FortranMatrix)]),
 }

@@ -17,10 +17,11 @@ void rr_solve(const double *ga, const double *gb, int lda, int k, int a, int p, 
 
 // Rayleigh-Ritz on raw Gram blocks of [X R P], folding in the Cholesky
 // normalisation of R and P (see dense.hip); coef gets (k+a+p) x k coordinates
-// for the raw blocks.  Sets st->r_info / abort, st->p_info (P dropped) and
-// st->gevp_info.
+// for the raw blocks.  explicit_x: take X'AX and X'BX from the Gram buffers
+// instead of diag(lambda) and I.  Sets st->r_info / abort, st->p_info (P
+// dropped) and st->gevp_info.
 void rr_solve_implicit(const double *ga, const double *gb, int lda, int k, int a, int p,
                        const double *lam_in, double *lam_out, double *coef, int ldc,
-                       DevStatus *st);
+                       DevStatus *st, int explicit_x);
 
 }  // namespace bx

@@ -105,7 +105,7 @@ __global__ void __launch_bounds__(kGTh) k_gram_jobs(GramKArgs a) {
             if (q < nq) {
                 double *dst = a.part + ((size_t)blockIdx.x * p.ntiles + p.qout[j][q]) * 256;
 #pragma unroll
-                for (int r = 0; r < 4; r++) dst[(4 * lk + r) * 16 + li] = acc[q][r];
+                for (int r = 0; r < 4; r++) dst[Mfma<T>::drow(lk, r) * 16 + li] = acc[q][r];
             }
         }
     }

@@ -2,7 +2,13 @@
 #include <vector>
 
 #include "blopex_amd_kernels.h"
+#include "dense.h"
 #include "kernels.h"
+
+namespace bx {
+void mfma_layout_probe(int prec, int *rows, int *cols);
+extern bool g_force_jacobi;
+}
 
 using namespace bx;
 
@@ -75,6 +81,38 @@ int blopex_amd_kernel_gram(int prec, long n, int lw, int qw, const double *L, co
     gb.add(dL.p, lw, dQ.p, qw, dG);
     gb.flush();
     download_sync(G, dG, (size_t)lw * qw * sizeof(double));
+    return 0;
+}
+
+int blopex_amd_kernel_rr(int k, int a, int p, const double *ga, const double *gb, int lda,
+                         const double *lam_in, int explicit_x, int jacobi, double *lam_out,
+                         double *coef, int *flags) {
+    const int n = k + a + p;
+    const size_t gbytes = (size_t)lda * n * sizeof(double);
+    char *d = (char *)scratch(9, 2 * gbytes + (size_t)(2 * k + n * k) * 8 + sizeof(DevStatus) + 64);
+    double *dga = (double *)d, *dgb = (double *)(d + gbytes);
+    double *dlam = (double *)(d + 2 * gbytes), *dlo = dlam + k, *dc = dlo + k;
+    DevStatus *st = (DevStatus *)(dc + (size_t)n * k);
+    upload_async(dga, ga, gbytes);
+    upload_async(dgb, gb, gbytes);
+    upload_async(dlam, lam_in, (size_t)k * 8);
+    BX_HIP(hipMemsetAsync(st, 0, sizeof(DevStatus), stream()));
+    g_force_jacobi = jacobi != 0;
+    rr_solve_implicit(dga, dgb, lda, k, a, p, dlam, dlo, dc, n, st, explicit_x);
+    g_force_jacobi = false;
+    DevStatus h;
+    download_sync(&h, st, sizeof(h));
+    download_sync(lam_out, dlo, (size_t)k * 8);
+    download_sync(coef, dc, (size_t)n * k * 8);
+    flags[0] = h.abort;
+    flags[1] = h.r_info;
+    flags[2] = h.p_info;
+    flags[3] = h.gevp_info;
+    return 0;
+}
+
+int blopex_amd_kernel_mfma_probe(int prec, int *rows, int *cols) {
+    mfma_layout_probe(prec, rows, cols);
     return 0;
 }
 

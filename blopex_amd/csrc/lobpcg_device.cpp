@@ -416,6 +416,7 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
     double best = maxres();
     int since_best = 0;
     int it = *it_io + 1;
+    const int explicit_x = prec_ == HIPMV_FP32 ? 1 : 0;
     for (; flag == 0 && it <= maxit; it++) {
         act.clear();
         for (int i = 0; i < k; i++) {
@@ -439,17 +440,20 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
             }
         }
         if (nr < 1) break;
-        // compact R (laid out by prev_act) and P/AP/BP (full width) to act
-        if (act != prev_act) {
-            std::vector<int> src;
-            size_t q = 0;
-            for (int c : act) {
-                while (prev_act[q] != c) q++;
-                src.push_back((int)q);
-            }
+        // compact R (laid out by prev_act) and P/AP/BP (rebuilt at full width k
+        // by every update) to the active columns
+        if (act != prev_act || (local_it > 0 && nr < k)) {
             tm_->begin(kOther);
-            gather(R_, src);
-            if (local_it > 0) {
+            if (act != prev_act) {
+                std::vector<int> src;
+                size_t q = 0;
+                for (int c : act) {
+                    while (prev_act[q] != c) q++;
+                    src.push_back((int)q);
+                }
+                gather(R_, src);
+            }
+            if (local_it > 0 && nr < k) {
                 gather(P_, act);
                 gather(AP_, act);
                 if (!noB_) gather(BP_, act);
@@ -470,6 +474,12 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
         {
             // every Gram block of the iteration in one pass over the rows
             GramBatch gb = gram_batch();
+            if (explicit_x) {
+                // FP32 iterates drift from X'BX = I, X'AX = diag(lambda): use the
+                // computed X blocks (the explicit-Gram variant of lobpcg.m:692-696)
+                gb.add(X_, k, AX_, k, GA_);
+                gb.add(X_, k, BXp, k, GB_);
+            }
             gb.add(R_, nr, AX_, k, GA_ + r0);
             gb.add(R_, nr, AR_, nr, GA_ + r0 + (size_t)r0 * ld3_);
             gb.add(R_, nr, BXp, k, GB_ + r0);
@@ -485,7 +495,8 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
             run_gram(gb);
         }
         tm_->begin(kDense);
-        rr_solve_implicit(GA_, GB_, ld3_, k, nr, preq, lam_, lam_new_, coef_, ld3_, st_);
+        rr_solve_implicit(GA_, GB_, ld3_, k, nr, preq, lam_, lam_new_, coef_, ld3_, st_,
+                          explicit_x);
         tm_->end();
         // P' = R C_R + P C_P and X' = X C_X + P' (likewise A-, B-blocks), in place
         UpdGroup g[3] = {};

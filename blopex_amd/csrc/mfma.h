@@ -1,8 +1,9 @@
 // CDNA MFMA wrappers (16x16x4 FP32 and FP64) and 4-row vector access.
 //
 // v_mfma_*_16x16x4: D(16x16) += A(16x4) B(4x16).  Lane l (0..63) supplies
-// A[l%16][l/16] and B[l/16][l%16] and holds D[4*(l/16)+r][l%16] in
-// accumulator register r (0..3).
+// A[l%16][l/16] and B[l/16][l%16]; accumulator register r (0..3) of lane l
+// holds D[Mfma<T>::drow(l/16, r)][l%16]: row 4*(l/16)+r for FP32, row
+// l/16+4*r for FP64 (checked on the device by the layout probe test).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -18,6 +19,7 @@ struct Mfma;
 template <>
 struct Mfma<float> {
     typedef f32x4 acc;
+    static __device__ __forceinline__ int drow(int lk, int r) { return 4 * lk + r; }
     static __device__ __forceinline__ acc mma(float a, float b, acc c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
@@ -26,6 +28,7 @@ struct Mfma<float> {
 template <>
 struct Mfma<double> {
     typedef f64x4 acc;
+    static __device__ __forceinline__ int drow(int lk, int r) { return lk + 4 * r; }
     static __device__ __forceinline__ acc mma(double a, double b, acc c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }

@@ -59,7 +59,7 @@ __device__ __forceinline__ void upd_store(T *out, int w, long ld, long r0, int l
     for (int o = 0; o < NOT; o++)
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const int c = o * 16 + 4 * lk + r;
+            const int c = o * 16 + Mfma<T>::drow(lk, r);
             if (c < w) store4(out + (size_t)c * ld + r0, acc[o][0][r], acc[o][1][r], acc[o][2][r],
                               acc[o][3][r]);
         }
@@ -115,7 +115,34 @@ void launch_update(const UpdKArgs &a, int ngroups) {
     BX_HIP(hipGetLastError());
 }
 
+// D = A B with A[i][0] = i (other k-slots 0), B[0][j] = 1: D[i][j] = i;
+// then A[i][0] = 1, B[0][j] = j: D[i][j] = j.
+template <typename T>
+__global__ void k_mfma_probe(int *rows, int *cols) {
+    const int l = threadIdx.x, li = l & 15, lk = l >> 4;
+    typename Mfma<T>::acc d1{}, d2{};
+    d1 = Mfma<T>::mma(lk == 0 ? T(li) : T(0), lk == 0 ? T(1) : T(0), d1);
+    d2 = Mfma<T>::mma(lk == 0 ? T(1) : T(0), lk == 0 ? T(li) : T(0), d2);
+    for (int r = 0; r < 4; r++) {
+        rows[4 * l + r] = (int)d1[r];
+        cols[4 * l + r] = (int)d2[r];
+    }
+}
+
 }  // namespace
+
+void mfma_layout_probe(int prec, int *rows, int *cols) {
+    int *d;
+    BX_HIP(hipMalloc(&d, 2 * 256 * sizeof(int)));
+    if (prec == HIPMV_FP32)
+        hipLaunchKernelGGL(k_mfma_probe<float>, dim3(1), dim3(64), 0, stream(), d, d + 256);
+    else
+        hipLaunchKernelGGL(k_mfma_probe<double>, dim3(1), dim3(64), 0, stream(), d, d + 256);
+    BX_HIP(hipGetLastError());
+    download_sync(rows, d, 256 * sizeof(int));
+    download_sync(cols, d + 256, 256 * sizeof(int));
+    BX_HIP(hipFree(d));
+}
 
 void block_update(int prec, long ld, const UpdGroup *groups, int ngroups, const double *C, int ldc,
                   int crows, int w, const DevStatus *st) {

@@ -24,6 +24,22 @@ int blopex_amd_kernel_update(int precision, long n, int k, int a, int p, const d
 int blopex_amd_kernel_gram(int precision, long n, int lw, int qw, const double *L, const double *Q,
                            double *G);
 
+/* Rayleigh-Ritz step of the device solver on host Gram buffers ga, gb
+   (lda x (k+a+p), raw blocks of [X R P] as the solver stores them, only the
+   lower block triangle and the R-R / P-P blocks read): coefficients
+   coef ((k+a+p) x k) for the raw blocks and the k smallest Ritz values.
+   explicit_x: X'AX, X'BX from the buffers instead of diag(lam_in), I.
+   jacobi != 0 forces the cyclic-Jacobi eigensolver instead of the
+   tridiagonal one.  flags = {abort, r_info, p_info, gevp_info}. */
+int blopex_amd_kernel_rr(int k, int a, int p, const double *ga, const double *gb, int lda,
+                         const double *lam_in, int explicit_x, int jacobi, double *lam_out,
+                         double *coef, int *flags);
+
+/* For each lane l (0..63) and accumulator register r (0..3) of the 16x16x4
+   MFMA used by the kernels (FP64 or FP32), the row and column of D it holds:
+   rows[4*l+r], cols[4*l+r].  Checks the layout the kernels assume. */
+int blopex_amd_kernel_mfma_probe(int precision, int *rows, int *cols);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,6 +24,21 @@ def f(a):
 
 
 @pytest.mark.parametrize("prec", [L.HIPMV_FP64, L.HIPMV_FP32])
+def test_mfma_accumulator_layout(prec):
+    """The (row, col) each lane/register holds is what the kernels assume:
+    col = lane % 16; row = 4*(lane/16)+r (FP32) or lane/16+4*r (FP64)."""
+    say(f"mfma layout probe prec={prec}")
+    rows = np.zeros(256, dtype=np.int32)
+    cols = np.zeros(256, dtype=np.int32)
+    L.load().blopex_amd_kernel_mfma_probe(prec, rows.ctypes.data, cols.ctypes.data)
+    lane = np.repeat(np.arange(64), 4)
+    r = np.tile(np.arange(4), 64)
+    np.testing.assert_array_equal(cols, lane % 16)
+    want = 4 * (lane // 16) + r if prec == L.HIPMV_FP32 else lane // 16 + 4 * r
+    np.testing.assert_array_equal(rows, want)
+
+
+@pytest.mark.parametrize("prec", [L.HIPMV_FP64, L.HIPMV_FP32])
 @pytest.mark.parametrize("n,k,a,p", [(1000, 20, 5, 13), (4096, 32, 32, 32), (777, 8, 8, 0),
                                      (130, 64, 64, 64), (64, 3, 2, 1)])
 def test_block_update(prec, n, k, a, p):
