@@ -36,7 +36,7 @@ class Stats(ctypes.Structure):
                 ("refine_iterations", c_int), ("exit_flag", c_int), ("seconds_small_dense", c_double),
                 ("seconds_spmm", c_double), ("seconds_gram", c_double), ("seconds_update", c_double),
                 ("seconds_other", c_double), ("spmm_bytes", c_double), ("spmm_calls", c_int),
-                ("pad_", c_int)]
+                ("fp32_exit", c_int)]
 
 
 class HipMultiVector(ctypes.Structure):
@@ -127,7 +127,7 @@ _SIGS = {
     "blopex_amd_kernel_gram": (c_int, [c_int, c_long, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "blopex_amd_kernel_mfma_probe": (c_int, [c_int, c_void_p, c_void_p]),
     "blopex_amd_kernel_rr": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p,
-                                     c_int, c_int, c_void_p, c_void_p, c_void_p]),
+                                     c_int, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "utilities_FortranMatrixUpperInv": (None, [P(FortranMatrix)]),
     "utilities_FortranMatrixSymmetrize": (None, [P(FortranMatrix)]),
 }

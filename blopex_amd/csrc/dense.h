@@ -15,13 +15,34 @@ void chol_inv(double *g, int ld, int m, DevStatus *st, int which);
 void rr_solve(const double *ga, const double *gb, int lda, int k, int a, int p, int initial,
               const double *lam_in, double *lam_out, double *coef, int ldc, DevStatus *st);
 
+struct RR2Args {
+    const double *ga, *gb;  // raw Gram buffers, leading dim lda
+    int lda;
+    int k, a, p;            // block widths (p = 0: no P this iteration)
+    const double *lam_in;   // Ritz values of X (XAX = diag)
+    double *lam_out;
+    double *coef;           // (k+a+p) x k, leading dim ldc
+    int ldc;
+    double *work;           // global scratch
+    DevStatus *st;
+    int explicit_x;         // 1: X'AX, X'BX from the Gram buffers (FP32 iterates)
+    int retry_without_p;    // 1: a pencil that is not positive definite is solved again
+                            // without P (p_info = -1); the tridiagonal solver only
+};
+
+// pencils the one-workgroup tridiagonal solver takes (rr_fast.hip)
+constexpr int kFastRRN = 96;
+constexpr int kFastRRK = 32;
+bool rr_fast_fits(int k, int a, int p);
+void rr_fast(const RR2Args &r);
+
 // Rayleigh-Ritz on raw Gram blocks of [X R P], folding in the Cholesky
 // normalisation of R and P (see dense.hip); coef gets (k+a+p) x k coordinates
 // for the raw blocks.  explicit_x: take X'AX and X'BX from the Gram buffers
-// instead of diag(lambda) and I.  Sets st->r_info / abort, st->p_info (P
-// dropped) and st->gevp_info.
+// instead of diag(lambda) and I.  retry_without_p: see RR2Args.  Sets
+// st->r_info / abort, st->p_info (P dropped) and st->gevp_info.
 void rr_solve_implicit(const double *ga, const double *gb, int lda, int k, int a, int p,
                        const double *lam_in, double *lam_out, double *coef, int ldc,
-                       DevStatus *st, int explicit_x);
+                       DevStatus *st, int explicit_x, int retry_without_p = 0);
 
 }  // namespace bx

@@ -398,18 +398,6 @@ __global__ void __launch_bounds__(kDT) k_rr(RRArgs r) {
 // B-orthonormalised basis is S' G S: exactly what lobpcg_solve assembles from
 // the explicitly normalised blocks (lobpcg.c:1027-1075 then 1092-1165).  The
 // Ritz coordinates returned are S Y, so the caller combines the raw blocks.
-struct RR2Args {
-    const double *ga, *gb;  // raw Gram buffers, leading dim lda
-    int lda;
-    int k, a, p;            // block widths (p = 0: no P this iteration)
-    const double *lam_in;   // Ritz values of X (XAX = diag)
-    double *lam_out;
-    double *coef;           // (k+a+p) x k, leading dim ldc
-    int ldc;
-    double *work;           // global scratch
-    DevStatus *st;
-    int explicit_x;         // 1: X'AX, X'BX from the Gram buffers (FP32 iterates)
-};
 
 __global__ void __launch_bounds__(kDT) k_rr_implicit(RR2Args r) {
     extern __shared__ double lds[];
@@ -624,406 +612,6 @@ __global__ void __launch_bounds__(kDT) k_rr_implicit(RR2Args r) {
     if (tid() == 0) r.st->gevp_info = 0;
 }
 
-// ---- fast Rayleigh-Ritz: Householder tridiagonalisation + bisection +
-// inverse iteration (pencils up to kFastN, k <= kFastK, all in LDS) -------
-//
-// Same inputs and outputs as k_rr_implicit.  After the pencil is assembled
-// and reduced to C = U^{-T} A U^{-1} (B = U'U), C = Q T Q' (Householder, n-2
-// reflections stored below the diagonal), the k smallest eigenvalues of T
-// come from Sturm-count multisection (16 points per eigenvalue per round, 14
-// rounds: 17^14 > 2^57 subintervals), their vectors from inverse iteration
-// with T - lambda I = L D L' (one thread per eigenvalue, 3 steps), classical
-// Gram-Schmidt applied twice in eigenvalue order keeps the block orthonormal
-// inside clusters, and Q, U^{-1}, S map them back.
-constexpr int kFastN = 96;
-constexpr int kFastK = 32;
-
-__device__ __forceinline__ double wave_sum(double v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-// row[jj*stride] <- sum_{l <= jj} row[l*stride] W[l + jj*m] (W upper), in place
-__device__ void right_mul_upper(double *row, int stride, const double *W, int m) {
-    for (int jj = m - 1; jj >= 0; jj--) {
-        double s = 0.0;
-        for (int l = 0; l <= jj; l++) s += row[l * stride] * W[l + jj * m];
-        row[jj * stride] = s;
-    }
-}
-
-// number of eigenvalues of the symmetric tridiagonal (d, e) below x
-__device__ int sturm_count(const double *d, const double *e, int n, double x, double pivmin) {
-    int c = 0;
-    double q = d[0] - x;
-    for (int i = 0;; i++) {
-        if (fabs(q) < pivmin) q = -pivmin;
-        c += q < 0.0;
-        if (i + 1 >= n) break;
-        q = d[i + 1] - x - e[i] * e[i] / q;
-    }
-    return c;
-}
-
-__global__ void __launch_bounds__(kDT) k_rr_fast(RR2Args r) {
-    extern __shared__ double lds[];
-    __shared__ double dT[kFastN], eT[kFastN], tauT[kFastN], vb[kFastN], wb[kFastN];
-    __shared__ double lamk[kFastK], proj[kFastK], red[kDT / 64], sc[4];
-    __shared__ double diag[kMaxRR];
-    __shared__ int info;
-    if (r.st->abort) return;
-    const int k = r.k, a = r.a, pq = r.p;
-    const int nmax = k + a + pq, ld = nmax;
-    // LDS: M1 (nmax^2), then M2 (the B pencil, nmax^2) whose space later
-    // holds Z and D (n*k each) of the inverse iteration
-    double *M1 = lds, *M2 = lds + (size_t)nmax * nmax;
-    double *G2 = r.work, *WR = G2 + (size_t)nmax * nmax, *WP = WR + (size_t)a * a;
-    double *Y = WP + (size_t)(pq > 0 ? pq : 1) * (pq > 0 ? pq : 1);
-
-    // ---- U_R^{-1}, U_P^{-1} ----
-    for (int e = tid(); e < a * a; e += kDT) M2[e] = r.gb[(k + e % a) + (size_t)(k + e / a) * r.lda];
-    __syncthreads();
-    int ci = chol_upper(M2, a, a, &info, diag);
-    if (ci) {
-        if (tid() == 0) {
-            r.st->r_info = ci;
-            r.st->abort = 1;
-        }
-        return;
-    }
-    upper_inverse(M2, a, a, diag);
-    for (int e = tid(); e < a * a; e += kDT) WR[e] = M2[e];
-    __syncthreads();
-    int p = 0;
-    if (pq > 0) {
-        for (int e = tid(); e < pq * pq; e += kDT)
-            M2[e] = r.gb[(k + a + e % pq) + (size_t)(k + a + e / pq) * r.lda];
-        __syncthreads();
-        ci = chol_upper(M2, pq, pq, &info, diag);
-        if (ci == 0) {
-            upper_inverse(M2, pq, pq, diag);
-            for (int e = tid(); e < pq * pq; e += kDT) WP[e] = M2[e];
-            p = pq;
-        }
-        if (tid() == 0) r.st->p_info = ci;
-        __syncthreads();
-    }
-    const int n = k + a + p;
-    auto blk = [&](int i) { return i < k ? 0 : (i < k + a ? 1 : 2); };
-
-    // ---- raw pencil (M1: A, M2: B) ----
-    for (int e = tid(); e < 2 * n * n; e += kDT) {
-        const int pass = e / (n * n), f = e % (n * n);
-        const int i = f % n, j = f / n, bi = blk(i), bj = blk(j);
-        const double *g = pass == 0 ? r.ga : r.gb;
-        double v;
-        if (bi == 0 && bj == 0 && r.explicit_x)
-            v = 0.5 * (g[i + (size_t)j * r.lda] + g[j + (size_t)i * r.lda]);
-        else if (bi == 0 && bj == 0)
-            v = (i == j) ? (pass == 0 ? r.lam_in[i] : 1.0) : 0.0;
-        else if (bi > bj)
-            v = g[i + (size_t)j * r.lda];
-        else if (bi < bj)
-            v = g[j + (size_t)i * r.lda];
-        else
-            v = g[i + (size_t)j * r.lda];
-        (pass == 0 ? M1 : M2)[i + j * ld] = v;
-    }
-    __syncthreads();
-    // M <- M S (thread per row of either matrix), then M <- S' M (per column)
-    for (int t = tid(); t < 2 * n; t += kDT) {
-        double *M = t < n ? M1 : M2;
-        const int i = t % n;
-        right_mul_upper(M + i + (size_t)k * ld, ld, WR, a);
-        if (p) right_mul_upper(M + i + (size_t)(k + a) * ld, ld, WP, p);
-    }
-    __syncthreads();
-    for (int t = tid(); t < 2 * n; t += kDT) {
-        double *M = t < n ? M1 : M2;
-        const int j = t % n;
-        right_mul_upper(M + k + (size_t)j * ld, 1, WR, a);
-        if (p) right_mul_upper(M + (k + a) + (size_t)j * ld, 1, WP, p);
-    }
-    __syncthreads();
-    for (int e = tid(); e < 2 * n * n; e += kDT) {
-        const int pass = e / (n * n), f = e % (n * n);
-        const int i = f % n, j = f / n;
-        if (i <= j) continue;
-        double *M = pass == 0 ? M1 : M2;
-        const int bi = blk(i), bj = blk(j);
-        double v;
-        if (bi == bj && bi > 0)
-            v = pass == 0 ? 0.5 * (M[i + j * ld] + M[j + i * ld]) : 0.0;
-        else
-            v = M[i + j * ld];
-        M[i + j * ld] = v;
-        M[j + i * ld] = v;
-    }
-    __syncthreads();
-    for (int i = (r.explicit_x ? k : 0) + tid(); i < n; i += kDT) M2[i + i * ld] = 1.0;
-    __syncthreads();
-
-    // ---- B = U'U, C = U^{-T} A U^{-1} ----
-    int bi = chol_upper(M2, ld, n, &info, diag);
-    if (bi) {
-        if (tid() == 0) {
-            r.st->gevp_info = n + bi;
-            r.st->abort = 1;
-        }
-        return;
-    }
-    for (int i = tid(); i < n; i += kDT) {  // rows: solve y U = a
-        double *row = M1 + i;
-        for (int j = 0; j < n; j++) {
-            double s = row[j * ld];
-            for (int l = 0; l < j; l++) s -= row[l * ld] * M2[l + j * ld];
-            row[j * ld] = s / M2[j + j * ld];
-        }
-    }
-    __syncthreads();
-    for (int j = tid(); j < n; j += kDT) {  // columns: solve U' z = y
-        double *col = M1 + (size_t)j * ld;
-        for (int i = 0; i < n; i++) {
-            double s = col[i];
-            for (int l = 0; l < i; l++) s -= M2[l + i * ld] * col[l];
-            col[i] = s / M2[i + i * ld];
-        }
-    }
-    __syncthreads();
-    for (int e = tid(); e < n * n; e += kDT) {
-        const int i = e % n, j = e / n;
-        if (i < j) {
-            const double v = 0.5 * (M1[i + j * ld] + M1[j + i * ld]);
-            M1[i + j * ld] = v;
-            M1[j + i * ld] = v;
-        }
-    }
-    for (int e = tid(); e < n * n; e += kDT) {
-        const int i = e % n, j = e / n;
-        if (i > j) M2[i + j * ld] = 0.0;
-    }
-    __syncthreads();
-    upper_inverse(M2, ld, n, diag);
-    for (int e = tid(); e < n * n; e += kDT) G2[e] = M2[(e % n) + (e / n) * ld];
-    __syncthreads();
-
-    // ---- Householder tridiagonalisation C = Q T Q' ----
-    for (int j = 0; j + 2 < n; j++) {
-        const int m = n - j - 1;
-        double *cj = M1 + (j + 1) + (size_t)j * ld;  // column j below the diagonal
-        if (tid() < 64) {
-            double s2 = 0.0;
-            for (int i = tid(); i < m; i += 64) s2 += cj[i] * cj[i];
-            s2 = wave_sum(s2);
-            const double x0 = cj[0];
-            const double alpha = s2 > 0.0 ? -copysign(sqrt(s2), x0) : 0.0;
-            const double v0 = x0 - alpha;
-            const double vtv = s2 - x0 * x0 + v0 * v0;
-            const double tau = (s2 > 0.0 && vtv > 0.0) ? 2.0 / vtv : 0.0;
-            for (int i = tid(); i < m; i += 64) vb[i] = i == 0 ? v0 : cj[i];
-            if (tid() == 0) {
-                tauT[j] = tau;
-                eT[j] = tau != 0.0 ? alpha : x0;
-                dT[j] = M1[j + j * ld];
-            }
-        }
-        __syncthreads();
-        const double tau = tauT[j];
-        if (tau != 0.0) {
-            // p = tau C' v, 8 lanes per row
-            for (int q = tid(); q < m * 8; q += kDT) {
-                const int row = q >> 3, part = q & 7;
-                const double *cr = M1 + (j + 1 + row) + (size_t)(j + 1) * ld;
-                double s = 0.0;
-                for (int l = part; l < m; l += 8) s += cr[(size_t)l * ld] * vb[l];
-                s += __shfl_xor(s, 4, 8);
-                s += __shfl_xor(s, 2, 8);
-                s += __shfl_xor(s, 1, 8);
-                if (part == 0) wb[row] = tau * s;
-            }
-            __syncthreads();
-            if (tid() < 64) {
-                double s = 0.0;
-                for (int i = tid(); i < m; i += 64) s += wb[i] * vb[i];
-                s = wave_sum(s);
-                const double K = 0.5 * tau * s;
-                for (int i = tid(); i < m; i += 64) wb[i] -= K * vb[i];
-            }
-            __syncthreads();
-            for (int e = tid(); e < m * m; e += kDT) {
-                const int i = e % m, l = e / m;
-                M1[(j + 1 + i) + (size_t)(j + 1 + l) * ld] -= vb[i] * wb[l] + wb[i] * vb[l];
-            }
-            for (int i = tid(); i < m; i += kDT) cj[i] = vb[i];
-        }
-        __syncthreads();
-    }
-    if (tid() == 0) {
-        for (int j = n - 2 < 0 ? 0 : n - 2; j < n; j++) dT[j] = M1[j + j * ld];
-        if (n >= 2) eT[n - 2] = M1[(n - 1) + (size_t)(n - 2) * ld];
-        if (n >= 2) tauT[n - 2] = 0.0;
-        // Gershgorin interval and pivot floor
-        double lo = dT[0], hi = dT[0], tn = 0.0;
-        for (int i = 0; i < n; i++) {
-            const double rad = (i > 0 ? fabs(eT[i - 1]) : 0.0) + (i + 1 < n ? fabs(eT[i]) : 0.0);
-            lo = fmin(lo, dT[i] - rad);
-            hi = fmax(hi, dT[i] + rad);
-            tn = fmax(tn, fabs(dT[i]) + rad);
-        }
-        sc[0] = lo - 1e-14 * (tn + 1e-300);
-        sc[1] = hi + 1e-14 * (tn + 1e-300);
-        sc[2] = tn;
-    }
-    __syncthreads();
-
-    // ---- k smallest eigenvalues of T: multisection, 16 lanes per index ----
-    const double tnorm = sc[2];
-    const double pivmin = 1e-300 + 1e-30 * tnorm * tnorm;
-    for (int base = 0; base < k; base += kDT / 16) {
-        const int idx = base + tid() / 16, lane16 = tid() & 15;
-        double lo = sc[0], hi = sc[1];
-        for (int round = 0; round < 14; round++) {
-            const double x = lo + (hi - lo) * (double)(lane16 + 1) / 17.0;
-            const bool above = idx < k ? sturm_count(dT, eT, n, x, pivmin) > idx : false;
-            const unsigned long long bal = __ballot(above);
-            const unsigned seg = (unsigned)((bal >> (threadIdx.x & 48)) & 0xFFFFull);
-            const double step = (hi - lo) / 17.0;
-            if (seg) {
-                const int t = __builtin_ctz(seg);
-                const double nhi = lo + step * (t + 1);
-                lo = lo + step * t;
-                hi = nhi;
-            } else {
-                lo = lo + step * 16;
-            }
-        }
-        if (idx < k && lane16 == 0) lamk[idx] = 0.5 * (lo + hi);
-    }
-    __syncthreads();
-
-    // ---- inverse iteration, one thread per eigenvalue (Z, D in the M2 area) ----
-    double *Z = M2, *D = M2 + (size_t)n * k;
-    for (int c = tid(); c < k; c += kDT) {
-        double *z = Z + (size_t)c * n, *dd = D + (size_t)c * n;
-        const double lam = lamk[c];
-        const double tiny = 2.220446049250313e-16 * (tnorm + 1e-300);
-        double nrm = 0.0;
-        for (int j = 0; j < n; j++) {
-            const unsigned h = (unsigned)(j + 1) * 2654435761u ^ (unsigned)(c + 1) * 40503u;
-            z[j] = 0.5 + (double)(h % 1024u) / 1024.0;
-            nrm += z[j] * z[j];
-        }
-        nrm = 1.0 / sqrt(nrm);
-        for (int j = 0; j < n; j++) z[j] *= nrm;
-        for (int it = 0; it < 3; it++) {
-            double q = dT[0] - lam;
-            if (fabs(q) < tiny) q = q < 0.0 ? -tiny : tiny;
-            dd[0] = q;
-            double y = z[0];
-            for (int j = 1; j < n; j++) {
-                const double l = eT[j - 1] / q;
-                q = dT[j] - lam - l * eT[j - 1];
-                if (fabs(q) < tiny) q = q < 0.0 ? -tiny : tiny;
-                dd[j] = q;
-                y = z[j] - l * y;
-                z[j] = y;
-            }
-            double x = z[n - 1] / dd[n - 1];
-            z[n - 1] = x;
-            double mx = fabs(x);
-            for (int j = n - 2; j >= 0; j--) {
-                x = (z[j] - eT[j] * x) / dd[j];
-                z[j] = x;
-                mx = fmax(mx, fabs(x));
-            }
-            double s2 = 0.0;
-            const double inv = mx > 0.0 ? 1.0 / mx : 1.0;
-            for (int j = 0; j < n; j++) {
-                z[j] *= inv;
-                s2 += z[j] * z[j];
-            }
-            const double f = s2 > 0.0 ? 1.0 / sqrt(s2) : 0.0;
-            for (int j = 0; j < n; j++) z[j] *= f;
-        }
-    }
-    __syncthreads();
-    // ---- classical Gram-Schmidt, twice, in eigenvalue order ----
-    for (int c = 1; c < k; c++) {
-        double *zc = Z + (size_t)c * n;
-        for (int pass = 0; pass < 2; pass++) {
-            for (int q = tid(); q < c * 16; q += kDT) {
-                const int cp = q >> 4, part = q & 15;
-                const double *zp = Z + (size_t)cp * n;
-                double s = 0.0;
-                for (int j = part; j < n; j += 16) s += zp[j] * zc[j];
-                s += __shfl_xor(s, 8, 16);
-                s += __shfl_xor(s, 4, 16);
-                s += __shfl_xor(s, 2, 16);
-                s += __shfl_xor(s, 1, 16);
-                if (part == 0) proj[cp] = s;
-            }
-            __syncthreads();
-            for (int j = tid(); j < n; j += kDT) {
-                double s = zc[j];
-                for (int cp = 0; cp < c; cp++) s -= proj[cp] * Z[(size_t)cp * n + j];
-                zc[j] = s;
-            }
-            __syncthreads();
-        }
-        double s2 = 0.0;
-        if (tid() < 64) {
-            for (int j = tid(); j < n; j += 64) s2 += zc[j] * zc[j];
-            s2 = wave_sum(s2);
-            if (tid() == 0) sc[3] = s2 > 0.0 ? 1.0 / sqrt(s2) : 0.0;
-        }
-        __syncthreads();
-        for (int j = tid(); j < n; j += kDT) zc[j] *= sc[3];
-        __syncthreads();
-    }
-    // ---- Z <- Q Z (reflections n-3 .. 0), 32 lanes per column ----
-    for (int c = tid() >> 5; c < k; c += kDT / 32) {
-        const int l32 = tid() & 31;
-        double *z = Z + (size_t)c * n;
-        for (int j = n - 3; j >= 0; j--) {
-            const double tau = tauT[j];
-            if (tau == 0.0) continue;
-            const int m = n - j - 1;
-            const double *v = M1 + (j + 1) + (size_t)j * ld;
-            double s = 0.0;
-            for (int i = l32; i < m; i += 32) s += v[i] * z[j + 1 + i];
-            for (int o = 16; o > 0; o >>= 1) s += __shfl_xor(s, o, 32);
-            s *= tau;
-            for (int i = l32; i < m; i += 32) z[j + 1 + i] -= s * v[i];
-        }
-    }
-    __syncthreads();
-    // ---- Y = U^{-1} Z, coef = S Y ----
-    for (int e = tid(); e < n * k; e += kDT) {
-        const int i = e % n, c = e / n;
-        double s = 0.0;
-        for (int l = i; l < n; l++) s += G2[i + (size_t)l * n] * Z[(size_t)c * n + l];
-        Y[i + (size_t)c * n] = s;
-    }
-    for (int c = tid(); c < k; c += kDT) r.lam_out[c] = lamk[c];
-    __syncthreads();
-    for (int e = tid(); e < (k + a + pq) * k; e += kDT) {
-        const int i = e % (k + a + pq), c = e / (k + a + pq), b2 = blk(i);
-        double s;
-        if (i >= n) {
-            s = 0.0;
-        } else if (b2 == 0) {
-            s = Y[i + (size_t)c * n];
-        } else {
-            const int o = b2 == 1 ? k : k + a, m = b2 == 1 ? a : p, ii = i - o;
-            const double *W = b2 == 1 ? WR : WP;
-            s = 0.0;
-            for (int l = ii; l < m; l++) s += W[ii + l * m] * Y[(o + l) + (size_t)c * n];
-        }
-        r.coef[i + (size_t)c * r.ldc] = s;
-    }
-    if (tid() == 0) r.st->gevp_info = 0;
-}
-
 }  // namespace
 
 void chol_inv(double *g, int ld, int m, DevStatus *st, int which) {
@@ -1053,38 +641,31 @@ void rr_solve(const double *ga, const double *gb, int lda, int k, int a, int p, 
 
 void rr_solve_implicit(const double *ga, const double *gb, int lda, int k, int a, int p,
                        const double *lam_in, double *lam_out, double *coef, int ldc,
-                       DevStatus *st, int explicit_x) {
+                       DevStatus *st, int explicit_x, int retry_without_p) {
     const int nmax = k + a + p;
     BX_REQUIRE(nmax <= kMaxRR, "rr_solve_implicit: pencil of order %d (max %d)", nmax, kMaxRR);
     const int nE = (nmax + 1) / 2 * 2;
     const size_t words = (size_t)4 * nE * nE + (size_t)a * a + (size_t)(p > 0 ? p * p : 1) +
                          (size_t)nmax * k + 64;
     double *work = (double *)scratch(7, words * sizeof(double));
-    RR2Args r{ga, gb, lda, k, a, p, lam_in, lam_out, coef, ldc, work, st, explicit_x};
-    static bool attr_set = false;
-    if (!attr_set) {
-        BX_HIP(hipFuncSetAttribute((const void *)k_rr_implicit,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, kRRLdsBytes));
-        BX_HIP(hipFuncSetAttribute((const void *)k_rr_fast,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, kRRLdsBytes));
-        attr_set = true;
-    }
+    RR2Args r{ga, gb, lda, k, a, p, lam_in, lam_out, coef, ldc, work, st, explicit_x,
+              retry_without_p};
     static const bool force_jacobi = [] {
         const char *e = getenv("BLOPEX_AMD_RR");
         return e && e[0] == 'j';
     }();
-    if (nmax <= kFastN && k <= kFastK && !force_jacobi && !g_force_jacobi) {
-        // LDS: the A pencil, then the B pencil or the inverse-iteration
-        // vectors and pivots (2 nmax k), whichever is larger
-        const size_t second = (size_t)nmax * nmax > (size_t)2 * nmax * k ? (size_t)nmax * nmax
-                                                                          : (size_t)2 * nmax * k;
-        const size_t lds = ((size_t)nmax * nmax + second) * sizeof(double);
-        BX_REQUIRE(lds <= (size_t)kRRLdsBytes, "rr_solve_implicit: %zu bytes of LDS", lds);
-        hipLaunchKernelGGL(k_rr_fast, dim3(1), dim3(kDT), lds, stream(), r);
-    } else {
-        const bool in_lds = (size_t)2 * nE * nE * sizeof(double) <= kRRLdsBytes;
-        hipLaunchKernelGGL(k_rr_implicit, dim3(1), dim3(kDT), in_lds ? kRRLdsBytes : 0, stream(), r);
+    if (rr_fast_fits(k, a, p) && !force_jacobi && !g_force_jacobi) {
+        rr_fast(r);
+        return;
     }
+    static bool attr_set = false;
+    if (!attr_set) {
+        BX_HIP(hipFuncSetAttribute((const void *)k_rr_implicit,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, kRRLdsBytes));
+        attr_set = true;
+    }
+    const bool in_lds = (size_t)2 * nE * nE * sizeof(double) <= kRRLdsBytes;
+    hipLaunchKernelGGL(k_rr_implicit, dim3(1), dim3(kDT), in_lds ? kRRLdsBytes : 0, stream(), r);
     BX_HIP(hipGetLastError());
 }
 

@@ -3,19 +3,21 @@
 // MultiInnerProd calls per iteration of krylov/lobpcg.c:1092-1165, each a
 // column-by-column triple loop in multi_vector.c:354-406).
 //
-// A job is one 16-column tile of a left block L and up to kGramQ tiles of
-// right blocks; a wave runs one job over 64-row chunks:
-//   * the lane holding (kslot = l/16, column l%16) loads 16 consecutive rows
-//     r0 + 16*kslot .. +15 of its column with vector loads -- the MFMA sums
-//     over the 4 k-slots of each step, and any row order that L and Q share
-//     gives the same sum, so the 64 rows are assigned as 4 slots x 16 steps;
-//   * 16 MFMA steps per (L tile, Q tile) pair accumulate D = L' Q (16x16).
-// FP64 inputs accumulate on FP64 MFMA throughout.  FP32 inputs use FP32
-// MFMA for one 64-row chunk (products exact in the FP32 accumulator up to its
-// rounding over 64 terms) and add each chunk's partial into FP64 registers,
-// so FP32 rounding never accumulates over more than 64 rows.
-// Each workgroup writes one FP64 partial per output tile; a second kernel
-// sums the partials in a fixed order (bitwise reproducible results).
+// Workgroups (8 waves) take chunks of rows round-robin (32 rows FP64, 64
+// FP32), so at any time they stream one contiguous window of every column.
+// A chunk of each distinct block of the plan is copied into LDS with 16-byte
+// coalesced loads (column-major, stride an odd number of 16-byte pieces); the next chunk is in flight
+// in registers while the current one is multiplied.  Each wave owns a
+// register block of up to 2 L tiles x 3 Q tiles (16 columns each): per chunk
+// a lane reads its rows of those 5 tiles with 16-byte LDS reads and runs the
+// 6 tile products on MFMA 16x16x4.  The k-index of the MFMA runs over the
+// rows of the chunk lane-group-major (lane group g = lane / 16 holds rows
+// g*RC/4 .. g*RC/4 + RC/4 - 1), the same for both operands.
+// FP64 inputs accumulate in FP64 throughout; FP32 inputs use FP32 MFMA
+// within a chunk and add each chunk's tile into FP64 registers, so FP32
+// rounding never spans more than 64 rows.  Each workgroup writes one FP64
+// partial per tile; a second kernel sums the partials in a fixed order
+// (bitwise reproducible results).
 #include "kernels.h"
 #include "mfma.h"
 
@@ -23,163 +25,318 @@ namespace bx {
 
 namespace {
 
-constexpr int kGTh = 256;
+constexpr int kGTh = kGramWaves * 64;  // threads per workgroup
+constexpr int kSlots = kGramBL * kGramBQ;
+
+template <typename T>
+struct GramCfg;
+template <>
+struct GramCfg<double> {
+    static constexpr int RC = 32;  // rows per chunk
+};
+template <>
+struct GramCfg<float> {
+    static constexpr int RC = 64;
+};
 
 struct GramKArgs {
     GramPlan p;
     long ld;
-    double *part;  // [gridDim.x][ntiles][256]
+    double *part;  // [gridDim.x][kGramTiles][256]
+};
+
+// operand vector of one lane for one tile and chunk: RC/4 consecutive rows
+template <typename T>
+struct Rows;
+template <>
+struct Rows<double> {
+    static constexpr int N = 8;
+    double v[N];
+    __device__ __forceinline__ void load(const double *p) {
+#pragma unroll
+        for (int q = 0; q < N / 2; q++) {
+            const double2 x = reinterpret_cast<const double2 *>(p)[q];
+            v[2 * q] = x.x;
+            v[2 * q + 1] = x.y;
+        }
+    }
+};
+template <>
+struct Rows<float> {
+    static constexpr int N = 16;
+    float v[N];
+    __device__ __forceinline__ void load(const float *p) {
+#pragma unroll
+        for (int q = 0; q < N / 4; q++) {
+            const float4 x = reinterpret_cast<const float4 *>(p)[q];
+            v[4 * q] = x.x;
+            v[4 * q + 1] = x.y;
+            v[4 * q + 2] = x.z;
+            v[4 * q + 3] = x.w;
+        }
+    }
+};
+
+// The products of one staged chunk for an NL x NQ register block.
+template <typename T, int NL, int NQ>
+struct GramBlock;
+
+template <int NL, int NQ>
+struct GramBlock<double, NL, NQ> {
+    static __device__ __forceinline__ void run(const double *stage, const int *lo, const int *qo,
+                                               f64x4 (&acc)[kSlots]) {
+        Rows<double> x[NL], y[NQ];
+#pragma unroll
+        for (int i = 0; i < NL; i++) x[i].load(stage + lo[i]);
+#pragma unroll
+        for (int j = 0; j < NQ; j++) y[j].load(stage + qo[j]);
+#pragma unroll
+        for (int st = 0; st < Rows<double>::N; st++)
+#pragma unroll
+            for (int i = 0; i < NL; i++)
+#pragma unroll
+                for (int j = 0; j < NQ; j++)
+                    acc[i * kGramBQ + j] =
+                        Mfma<double>::mma(x[i].v[st], y[j].v[st], acc[i * kGramBQ + j]);
+    }
+};
+
+template <int NL, int NQ>
+struct GramBlock<float, NL, NQ> {
+    static __device__ __forceinline__ void run(const float *stage, const int *lo, const int *qo,
+                                               f64x4 (&acc)[kSlots]) {
+        Rows<float> x[NL], y[NQ];
+#pragma unroll
+        for (int i = 0; i < NL; i++) x[i].load(stage + lo[i]);
+#pragma unroll
+        for (int j = 0; j < NQ; j++) y[j].load(stage + qo[j]);
+        f32x4 c[NL * NQ];
+#pragma unroll
+        for (int t = 0; t < NL * NQ; t++) c[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int st = 0; st < Rows<float>::N; st++)
+#pragma unroll
+            for (int i = 0; i < NL; i++)
+#pragma unroll
+                for (int j = 0; j < NQ; j++)
+                    c[i * NQ + j] = Mfma<float>::mma(x[i].v[st], y[j].v[st], c[i * NQ + j]);
+#pragma unroll
+        for (int i = 0; i < NL; i++)
+#pragma unroll
+            for (int j = 0; j < NQ; j++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) acc[i * kGramBQ + j][r] += (double)c[i * NQ + j][r];
+    }
 };
 
 template <typename T>
-__device__ __forceinline__ void load16(const T *p, bool ok, T v[16]) {
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        if (ok) {
-            load4(p + 4 * q, v + 4 * q);
-        } else {
-            v[4 * q] = v[4 * q + 1] = v[4 * q + 2] = v[4 * q + 3] = T(0);
-        }
+__device__ __forceinline__ void gram_block(int nl, int nq, const T *stage, const int *lo,
+                                           const int *qo, f64x4 (&acc)[kSlots]) {
+    switch (nl * 4 + nq) {  // wave-uniform
+        case 5: GramBlock<T, 1, 1>::run(stage, lo, qo, acc); break;
+        case 6: GramBlock<T, 1, 2>::run(stage, lo, qo, acc); break;
+        case 7: GramBlock<T, 1, 3>::run(stage, lo, qo, acc); break;
+        case 9: GramBlock<T, 2, 1>::run(stage, lo, qo, acc); break;
+        case 10: GramBlock<T, 2, 2>::run(stage, lo, qo, acc); break;
+        case 11: GramBlock<T, 2, 3>::run(stage, lo, qo, acc); break;
+        default: break;
     }
 }
 
 template <typename T>
-struct GramAcc;
-
-template <>
-struct GramAcc<double> {
-    // accumulate straight into the FP64 accumulator
-    static __device__ __forceinline__ void chunk(const double *lv, const double *qv, f64x4 &acc) {
-#pragma unroll
-        for (int s = 0; s < 16; s++) acc = Mfma<double>::mma(lv[s], qv[s], acc);
-    }
-};
-
-template <>
-struct GramAcc<float> {
-    static __device__ __forceinline__ void chunk(const float *lv, const float *qv, f64x4 &acc) {
-        f32x4 c = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int s = 0; s < 16; s++) c = Mfma<float>::mma(lv[s], qv[s], c);
-        acc[0] += (double)c[0];
-        acc[1] += (double)c[1];
-        acc[2] += (double)c[2];
-        acc[3] += (double)c[3];
-    }
-};
-
-template <typename T>
-__global__ void __launch_bounds__(kGTh) k_gram_jobs(GramKArgs a) {
+__global__ void __launch_bounds__(kGTh) k_gram(GramKArgs a) {
+    constexpr int RC = GramCfg<T>::RC;
+    constexpr int EV = 16 / sizeof(T);  // elements per 16-byte piece
+    constexpr int S = RC + EV;          // column stride: an odd number of 16-byte pieces
+    constexpr int VPC = RC / EV;                     // pieces per column per chunk
+    constexpr int PRE = (kGramCols * VPC + kGTh - 1) / kGTh;
+    static_assert(PRE <= 9, "Gram staging: too many pieces per thread");
+    extern __shared__ __align__(16) unsigned char smem[];
     const GramPlan &p = a.p;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    T *stage = reinterpret_cast<T *>(smem);
+    const T **colp = reinterpret_cast<const T **>(smem + (size_t)(p.ncols + 16) * S * sizeof(T));
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar branches
     const int li = lane & 15, lk = lane >> 4;
-    const long nchunks = a.ld / 64;
-    for (int j = wave; j < p.njobs; j += kGTh / 64) {
-        const int lb = p.lblk[j];
-        const int lcol = p.ltile[j] * 16 + li;
-        const bool lok = lcol < p.blkw[lb];
-        const T *L = (const T *)p.blk[lb] + (size_t)(lok ? lcol : 0) * a.ld;
-        const int nq = p.nq[j];
-        f64x4 acc[kGramQ];
+    for (int c = tid; c < p.ncols; c += kGTh) {
+        int b = 0;
+        while (b + 1 < p.nblk && c >= p.bc0[b + 1]) b++;
+        colp[c] = (const T *)p.blk[b] + (size_t)(c - p.bc0[b]) * a.ld;
+    }
+    __syncthreads();
+
+    const int nvec = p.ncols * VPC;
+    const long nch = a.ld / RC;
+    const long gs = gridDim.x;
+    const bool act = wave < p.nwb;
+    const int nl = act ? p.wnl[wave] : 0, nq = act ? p.wnq[wave] : 0;
+    int lo[kGramBL], qo[kGramBQ];
 #pragma unroll
-        for (int q = 0; q < kGramQ; q++) acc[q] = f64x4{0.0, 0.0, 0.0, 0.0};
-        for (long ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
-            const long rb = ch * 64 + 16 * lk;
-            T lv[16];
-            load16(L + rb, lok, lv);
+    for (int i = 0; i < kGramBL; i++) lo[i] = ((act ? p.wl[wave][i] : 0) + li) * S + lk * (RC / 4);
 #pragma unroll
-            for (int q = 0; q < kGramQ; q++) {
-                if (q < nq) {
-                    const int qb = p.qblk[j][q];
-                    const int qcol = p.qtile[j][q] * 16 + li;
-                    const bool qok = qcol < p.blkw[qb];
-                    const T *Q = (const T *)p.blk[qb] + (size_t)(qok ? qcol : 0) * a.ld;
-                    T qv[16];
-                    load16(Q + rb, qok, qv);
-                    GramAcc<T>::chunk(lv, qv, acc[q]);
-                }
-            }
+    for (int j = 0; j < kGramBQ; j++) qo[j] = ((act ? p.wq[wave][j] : 0) + li) * S + lk * (RC / 4);
+    f64x4 acc[kSlots];
+#pragma unroll
+    for (int s = 0; s < kSlots; s++) acc[s] = f64x4{0.0, 0.0, 0.0, 0.0};
+
+    // the (up to 9) 16-byte pieces this thread copies per chunk, in named
+    // registers (an array of them would live in scratch memory); pieces past
+    // the plan load piece 0 and land in the spare columns
+    uint4 p0, p1, p2, p3, p4, p5, p6, p7, p8;
+#define BX_GRAM_PIECES(X) X(0, p0) X(1, p1) X(2, p2) X(3, p3) X(4, p4) X(5, p5) X(6, p6) X(7, p7) X(8, p8)
+#define BX_GRAM_LOAD(q, r)                                                                    \
+    if (q < PRE) {                                                                            \
+        const int e = tid + q * kGTh < nvec ? tid + q * kGTh : 0;                             \
+        r = *reinterpret_cast<const uint4 *>(colp[e / VPC] + chl * RC + (e % VPC) * EV);      \
+    }
+#define BX_GRAM_STORE(q, r)                                                                   \
+    if (q < PRE) {                                                                            \
+        const int e = tid + q * kGTh;                                                         \
+        const int o = e < nvec ? (e / VPC) * S + (e % VPC) * EV : (p.ncols + (tid & 15)) * S; \
+        *reinterpret_cast<uint4 *>(stage + o) = r;                                            \
+    }
+    if ((long)blockIdx.x < nch) {
+        const long chl = blockIdx.x;
+        BX_GRAM_PIECES(BX_GRAM_LOAD)
+    }
+    for (long ch = blockIdx.x; ch < nch; ch += gs) {
+        BX_GRAM_PIECES(BX_GRAM_STORE)
+        __syncthreads();
+        if (ch + gs < nch) {  // next chunk in flight during the products
+            const long chl = ch + gs;
+            BX_GRAM_PIECES(BX_GRAM_LOAD)
         }
+        gram_block<T>(nl, nq, stage, lo, qo, acc);
+        __syncthreads();
+    }
+#undef BX_GRAM_STORE
+#undef BX_GRAM_LOAD
+#undef BX_GRAM_PIECES
+    if (!act) return;
+    double *base = a.part + ((size_t)blockIdx.x * kGramTiles + (size_t)wave * kSlots) * 256;
 #pragma unroll
-        for (int q = 0; q < kGramQ; q++) {
-            if (q < nq) {
-                double *dst = a.part + ((size_t)blockIdx.x * p.ntiles + p.qout[j][q]) * 256;
+    for (int s = 0; s < kSlots; s++) {
+        if (s / kGramBQ >= nl || s % kGramBQ >= nq) continue;
 #pragma unroll
-                for (int r = 0; r < 4; r++) dst[Mfma<T>::drow(lk, r) * 16 + li] = acc[q][r];
-            }
-        }
+        for (int r = 0; r < 4; r++) base[s * 256 + Mfma<T>::drow(lk, r) * 16 + li] = acc[s][r];
     }
 }
 
 struct ReduceArgs {
-    double *dst[kGramOut];
-    int rows[kGramOut], cols[kGramOut];
-    int ntiles;
+    double *dst[kGramTiles], *mdst[kGramTiles];
+    unsigned char rows[kGramTiles], cols[kGramTiles];
+    unsigned char slot[kGramTiles];
     int nparts;
     long ldg;
     const double *part;
 };
 
+// one workgroup per stored tile: sum of the per-workgroup partials in a fixed order
 __global__ void __launch_bounds__(256) k_gram_sum(ReduceArgs a) {
     const int t = blockIdx.x, e = threadIdx.x;
     const int i = e / 16, jj = e % 16;
-    const double *src = a.part + (size_t)t * 256 + e;
-    const size_t stride = (size_t)a.ntiles * 256;
+    const double *src = a.part + (size_t)a.slot[t] * 256 + e;
+    const size_t stride = (size_t)kGramTiles * 256;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    int p = 0;
-    for (; p + 4 <= a.nparts; p += 4) {
-        s0 += src[(size_t)p * stride];
-        s1 += src[(size_t)(p + 1) * stride];
-        s2 += src[(size_t)(p + 2) * stride];
-        s3 += src[(size_t)(p + 3) * stride];
+    int q = 0;
+    for (; q + 4 <= a.nparts; q += 4) {
+        s0 += src[(size_t)q * stride];
+        s1 += src[(size_t)(q + 1) * stride];
+        s2 += src[(size_t)(q + 2) * stride];
+        s3 += src[(size_t)(q + 3) * stride];
     }
-    for (; p < a.nparts; p++) s0 += src[(size_t)p * stride];
+    for (; q < a.nparts; q++) s0 += src[(size_t)q * stride];
     const double s = (s0 + s1) + (s2 + s3);
-    if (i < a.rows[t] && jj < a.cols[t]) a.dst[t][i + (size_t)jj * a.ldg] = s;
+    if (i < a.rows[t] && jj < a.cols[t]) {
+        a.dst[t][i + (size_t)jj * a.ldg] = s;
+        if (a.mdst[t]) a.mdst[t][jj + (size_t)i * a.ldg] = s;
+    }
 }
 
-}  // namespace
-
-namespace {
 int plan_block(GramPlan &p, const void *blk, int width) {
     for (int b = 0; b < p.nblk; b++)
-        if (p.blk[b] == blk && p.blkw[b] == width) return b;
-    if (p.nblk >= kGramBlocks) return -1;
+        if (p.blk[b] == blk && p.bw[b] == width) return b;
+    if (p.nblk >= kGramBlocks || p.ncols + width > kGramCols) return -1;
     p.blk[p.nblk] = blk;
-    p.blkw[p.nblk] = width;
+    p.bw[p.nblk] = width;
+    p.bc0[p.nblk] = p.ncols;
+    p.ncols += width;
     return p.nblk++;
 }
+
+// Forms the register blocks from the queued products: for each left block,
+// its 16-column tiles in pairs, against all right tiles of its products in
+// threes.  Returns the number of register blocks (may exceed kGramWaves).
+int form_blocks(GramPlan &p, long ldg, bool fill) {
+    int nw = 0;
+    bool done[kGramBlocks] = {};
+    for (int q0 = 0; q0 < p.nprod; q0++) {
+        const int lb = p.pl[q0];
+        if (done[lb]) continue;
+        done[lb] = true;
+        // right tiles of every product with this left block
+        int qtp[64], qty[64], nqt = 0;
+        for (int q = 0; q < p.nprod; q++) {
+            if (p.pl[q] != lb) continue;
+            const int qt = (p.bw[p.pq[q]] + 15) / 16;
+            for (int y = 0; y < qt && nqt < 64; y++) {
+                qtp[nqt] = q;
+                qty[nqt] = y;
+                nqt++;
+            }
+        }
+        const int lw = p.bw[lb], lt = (lw + 15) / 16;
+        for (int x0 = 0; x0 < lt; x0 += kGramBL) {
+            const int nl = lt - x0 < kGramBL ? lt - x0 : kGramBL;
+            for (int j0 = 0; j0 < nqt; j0 += kGramBQ) {
+                const int nq = nqt - j0 < kGramBQ ? nqt - j0 : kGramBQ;
+                if (fill && nw < kGramWaves) {
+                    const int w = nw;
+                    p.wnl[w] = (unsigned char)nl;
+                    p.wnq[w] = (unsigned char)nq;
+                    for (int i = 0; i < kGramBL; i++)
+                        p.wl[w][i] = (short)(p.bc0[lb] + 16 * (x0 + (i < nl ? i : 0)));
+                    for (int j = 0; j < kGramBQ; j++) {
+                        const int jj = j0 + (j < nq ? j : 0);
+                        p.wq[w][j] = (short)(p.bc0[p.pq[qtp[jj]]] + 16 * qty[jj]);
+                    }
+                    for (int i = 0; i < kGramBL; i++)
+                        for (int j = 0; j < kGramBQ; j++) {
+                            const int s = w * kGramBL * kGramBQ + i * kGramBQ + j;
+                            p.dst[s] = p.mdst[s] = nullptr;
+                            p.rows[s] = p.cols[s] = 0;
+                            if (i >= nl || j >= nq) continue;
+                            const int q = qtp[j0 + j], x = x0 + i, y = qty[j0 + j];
+                            const int qw = p.bw[p.pq[q]];
+                            const bool sym = p.pq[q] == lb;
+                            if (sym && y < x) continue;  // stored as the transpose of (y, x)
+                            p.rows[s] = (unsigned char)(lw - 16 * x < 16 ? lw - 16 * x : 16);
+                            p.cols[s] = (unsigned char)(qw - 16 * y < 16 ? qw - 16 * y : 16);
+                            p.dst[s] = p.pg[q] + 16 * x + (size_t)16 * y * ldg;
+                            if (sym && x < y) p.mdst[s] = p.pg[q] + 16 * y + (size_t)16 * x * ldg;
+                        }
+                }
+                nw++;
+            }
+        }
+    }
+    if (fill) p.nwb = nw;
+    return nw;
+}
+
 }  // namespace
 
 bool gram_plan_product(GramPlan &p0, const void *L, int lw, const void *Q, int qw, double *g,
                        long ldg) {
     GramPlan p = p0;  // work on a copy: commit only if everything fits
+    if (p.nprod >= 16) return false;
     const int lb = plan_block(p, L, lw), qb = plan_block(p, Q, qw);
     if (lb < 0 || qb < 0) return false;
-    const int lt = (lw + 15) / 16, qt = (qw + 15) / 16;
-    for (int a = 0; a < lt; a++) {
-        for (int b = 0; b < qt; b++) {
-            int j = -1;
-            for (int c = 0; c < p.njobs; c++)
-                if (p.lblk[c] == lb && p.ltile[c] == a && p.nq[c] < kGramQ) j = c;
-            if (j < 0) {
-                if (p.njobs >= kGramJobs) return false;
-                j = p.njobs++;
-                p.lblk[j] = lb;
-                p.ltile[j] = a;
-                p.nq[j] = 0;
-            }
-            if (p.ntiles >= kGramOut) return false;
-            const int t = p.ntiles++;
-            p.qblk[j][p.nq[j]] = qb;
-            p.qtile[j][p.nq[j]] = b;
-            p.qout[j][p.nq[j]] = t;
-            p.nq[j]++;
-            p.dst[t] = g + 16 * a + (size_t)16 * b * ldg;
-            p.rows[t] = lw - 16 * a < 16 ? lw - 16 * a : 16;
-            p.cols[t] = qw - 16 * b < 16 ? qw - 16 * b : 16;
-        }
-    }
+    p.pl[p.nprod] = (short)lb;
+    p.pq[p.nprod] = (short)qb;
+    p.pg[p.nprod] = g;
+    p.nprod++;
+    if (form_blocks(p, ldg, false) > kGramWaves) return false;
     p0 = p;
     return true;
 }
@@ -193,38 +350,56 @@ void GramBatch::add(const void *L, int lw, const void *Q, int qw, double *g) {
 }
 
 void GramBatch::flush() {
-    if (p_.njobs) gram_run(prec_, ld_, p_, ldg_);
+    if (p_.nprod) gram_run(prec_, ld_, p_, ldg_);
     p_ = GramPlan();
 }
 
 void gram_run(int prec, long ld, GramPlan &p, long ldg) {
     BX_REQUIRE(ld % 64 == 0, "gram_run: leading dimension %ld not a multiple of 64", ld);
-    if (p.njobs == 0) return;
-    const long nchunks = ld / 64;
-    long parts = (long)num_cus() * 2;
+    if (p.nprod == 0) return;
+    form_blocks(p, ldg, true);
+    const size_t es = prec == HIPMV_FP32 ? sizeof(float) : sizeof(double);
+    const int rc = prec == HIPMV_FP32 ? GramCfg<float>::RC : GramCfg<double>::RC;
+    const int S = rc + (int)(16 / es);
+    const long nchunks = ld / rc;
+    const size_t lds = (size_t)(p.ncols + 16) * S * es + (size_t)p.ncols * sizeof(void *);
+    long parts = (long)num_cus();  // one 8-wave workgroup per CU (register bound)
     if (parts > nchunks) parts = nchunks;
     if (parts < 1) parts = 1;
-    double *part = (double *)scratch(0, (size_t)parts * p.ntiles * 256 * sizeof(double));
+    double *part = (double *)scratch(0, (size_t)parts * kGramTiles * 256 * sizeof(double));
     GramKArgs a;
     a.p = p;
     a.ld = ld;
     a.part = part;
+    static bool attr = false;
+    if (!attr) {
+        BX_HIP(hipFuncSetAttribute((const void *)k_gram<float>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        BX_HIP(hipFuncSetAttribute((const void *)k_gram<double>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr = true;
+    }
+    BX_REQUIRE(lds <= 160 * 1024, "gram_run: %zu bytes of LDS", lds);
     if (prec == HIPMV_FP32)
-        hipLaunchKernelGGL(k_gram_jobs<float>, dim3((unsigned)parts), dim3(kGTh), 0, stream(), a);
+        hipLaunchKernelGGL(k_gram<float>, dim3((unsigned)parts), dim3(kGTh), lds, stream(), a);
     else
-        hipLaunchKernelGGL(k_gram_jobs<double>, dim3((unsigned)parts), dim3(kGTh), 0, stream(), a);
+        hipLaunchKernelGGL(k_gram<double>, dim3((unsigned)parts), dim3(kGTh), lds, stream(), a);
     BX_HIP(hipGetLastError());
     ReduceArgs r;
-    for (int t = 0; t < p.ntiles; t++) {
-        r.dst[t] = p.dst[t];
-        r.rows[t] = p.rows[t];
-        r.cols[t] = p.cols[t];
+    int nt = 0;
+    for (int s = 0; s < p.nwb * kSlots; s++) {
+        if (!p.dst[s]) continue;
+        r.dst[nt] = p.dst[s];
+        r.mdst[nt] = p.mdst[s];
+        r.rows[nt] = p.rows[s];
+        r.cols[nt] = p.cols[s];
+        r.slot[nt] = (unsigned char)s;
+        nt++;
     }
-    r.ntiles = p.ntiles;
     r.nparts = (int)parts;
     r.ldg = ldg;
     r.part = part;
-    hipLaunchKernelGGL(k_gram_sum, dim3(p.ntiles), dim3(256), 0, stream(), r);
+    if (nt) hipLaunchKernelGGL(k_gram_sum, dim3(nt), dim3(256), 0, stream(), r);
     BX_HIP(hipGetLastError());
 }
 

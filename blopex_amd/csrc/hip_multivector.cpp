@@ -19,7 +19,7 @@ hip_LobpcgSettings default_settings() {
     hip_LobpcgSettings s;
     s.mode = HIPLOBPCG_FP64;
     s.switch_factor = 4.0;
-    s.stall_iterations = 8;
+    s.stall_iterations = 50;
     s.refine_iterations = 0;
     s.profile = 0;
     return s;

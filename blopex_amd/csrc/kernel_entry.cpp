@@ -85,7 +85,8 @@ int blopex_amd_kernel_gram(int prec, long n, int lw, int qw, const double *L, co
 }
 
 int blopex_amd_kernel_rr(int k, int a, int p, const double *ga, const double *gb, int lda,
-                         const double *lam_in, int explicit_x, int jacobi, double *lam_out,
+                         const double *lam_in, int explicit_x, int retry_without_p, int jacobi,
+                         double *lam_out,
                          double *coef, int *flags) {
     const int n = k + a + p;
     const size_t gbytes = (size_t)lda * n * sizeof(double);
@@ -98,7 +99,7 @@ int blopex_amd_kernel_rr(int k, int a, int p, const double *ga, const double *gb
     upload_async(dlam, lam_in, (size_t)k * 8);
     BX_HIP(hipMemsetAsync(st, 0, sizeof(DevStatus), stream()));
     g_force_jacobi = jacobi != 0;
-    rr_solve_implicit(dga, dgb, lda, k, a, p, dlam, dlo, dc, n, st, explicit_x);
+    rr_solve_implicit(dga, dgb, lda, k, a, p, dlam, dlo, dc, n, st, explicit_x, retry_without_p);
     g_force_jacobi = false;
     DevStatus h;
     download_sync(&h, st, sizeof(h));

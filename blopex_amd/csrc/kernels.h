@@ -56,25 +56,37 @@ struct UpdGroup {
 void block_update(int prec, long ld, const UpdGroup *groups, int ngroups, const double *C, int ldc,
                   int crows, int w, const DevStatus *st);
 
-// Fused Gram products G = L' Q over 16-column tiles, FP64 accumulation
-// (FP32 inputs: FP32 MFMA over 64 rows, then FP64).
-constexpr int kGramBlocks = 12;  // distinct n-row blocks
-constexpr int kGramJobs = 8;     // left tiles
-constexpr int kGramQ = 14;       // right tiles per left tile
-constexpr int kGramOut = 96;     // output tiles
+// Fused Gram products G = L' Q: every block of a plan is staged through LDS
+// a chunk of rows at a time, and each wave accumulates a register block of
+// up to 2 x 3 output tiles (16 x 16) on MFMA in FP64 (FP32 inputs: FP32 MFMA
+// over each chunk, then FP64).
+constexpr int kGramBlocks = 10;    // distinct n-row blocks per plan
+constexpr int kGramCols = 288;     // staged columns per plan (sum of block widths)
+constexpr int kGramWaves = 8;      // waves per workgroup = register blocks per plan
+constexpr int kGramBL = 2, kGramBQ = 3;                    // tiles of a register block
+constexpr int kGramTiles = kGramWaves * kGramBL * kGramBQ;  // tile slots per plan
 struct GramPlan {
     const void *blk[kGramBlocks];
-    int blkw[kGramBlocks];
-    int nblk;
-    int njobs;
-    int lblk[kGramJobs], ltile[kGramJobs], nq[kGramJobs];
-    int qblk[kGramJobs][kGramQ], qtile[kGramJobs][kGramQ], qout[kGramJobs][kGramQ];
-    int ntiles;
-    double *dst[kGramOut];  // output tile origin (FP64, leading dim ldg)
-    int rows[kGramOut], cols[kGramOut];
+    int bw[kGramBlocks];   // block widths
+    int bc0[kGramBlocks];  // first staged column of each block
+    int nblk, ncols;
+    int nwb;                                  // register blocks in use
+    short wl[kGramWaves][kGramBL];            // staged column of each L tile
+    short wq[kGramWaves][kGramBQ];            // staged column of each Q tile
+    unsigned char wnl[kGramWaves], wnq[kGramWaves];
+    // tile (w, i, j) = slot w * 6 + i * 3 + j
+    unsigned char rows[kGramTiles], cols[kGramTiles];  // valid part of the tile
+    double *dst[kGramTiles];   // tile origin (FP64, leading dim ldg); null: not stored
+    double *mdst[kGramTiles];  // origin of the transposed copy (symmetric products) or null
+    // products queued for the plan (blocks are formed when it runs)
+    int nprod;
+    short pl[16], pq[16];      // block index of L and Q
+    double *pg[16];            // output origin
 };
 // Adds the product L' Q (L: lw columns, Q: qw columns) landing at g (leading
-// dim ldg); returns false, leaving the plan unchanged, when it does not fit.
+// dim ldg); a product with Q == L (same block and width) is symmetric: only
+// its upper tiles are stored, each also as its transpose.  Returns false,
+// leaving the plan unchanged, when it does not fit.
 bool gram_plan_product(GramPlan &p, const void *L, int lw, const void *Q, int qw, double *g,
                        long ldg);
 void gram_run(int prec, long ld, GramPlan &p, long ldg);

@@ -15,8 +15,8 @@
 //
 // Precision: FP64 runs everything in FP64.  MIXED iterates with FP32 blocks
 // (FP64 Gram accumulation, FP64 small dense) until the FP32 residuals stop
-// improving or fall below switch_factor * atol, then restarts from the
-// current X in FP64 (re-orthonormalisation and Rayleigh-Ritz in FP64) and
+// improving or fall below switch_factor * atol (or an FP32 Rayleigh-Ritz
+// step breaks down), then restarts from the last FP32 X in FP64 (re-orthonormalisation and Rayleigh-Ritz in FP64) and
 // finishes there, so the final residuals and Ritz values are FP64 ones.
 #include <cfloat>
 #include <chrono>
@@ -140,6 +140,7 @@ struct HostOut {
 
 class Phase {
   public:
+    int fp32_exit_ = 0;  // why the FP32 phase ended (hip_LobpcgStats.fp32_exit)
     Phase(int prec, long n, int k, int ny, Op opA, void *Ad, Op opB, void *Bd, Op opT, void *Td,
           const hip_Multi_Vector *Yuser, lobpcg_Tolerance tol, int verb, HostOut out, KTimer *tm)
         : prec_(prec), n_(n), k_(k), ny_(ny), opA_(opA), Ad_(Ad), opB_(opB), Bd_(Bd), opT_(opT),
@@ -413,8 +414,7 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
     std::vector<int> mask(k, 1), act, prev_act(k);
     for (int i = 0; i < k; i++) prev_act[i] = i;
     int local_it = 0;
-    double best = maxres();
-    int since_best = 0;
+    std::vector<double> gm_hist;  // geometric mean of the k residual norms, per FP32 iteration
     int it = *it_io + 1;
     const int explicit_x = prec_ == HIPMV_FP32 ? 1 : 0;
     for (; flag == 0 && it <= maxit; it++) {
@@ -426,16 +426,23 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
         const int nr = (int)act.size();
         if (fp32_phase) {
             // leave FP32 on convergence, near the tolerance, or on a stall
+            // (LOBPCG residuals are not monotone: early on they rise for tens
+            // of iterations, so progress is judged window against window)
             const double mr = maxres();
-            if (mr < best * 0.9) {
-                best = mr;
-                since_best = 0;
-            } else {
-                since_best++;
+            double lg = 0.0;
+            for (int i = 0; i < k; i++) lg += log(out_.res[i] > 1e-300 ? out_.res[i] : 1e-300);
+            gm_hist.push_back(exp(lg / k));
+            const int W = set.stall_iterations, h = (int)gm_hist.size();
+            bool stalled = false;
+            if (W > 0 && h >= 2 * W) {
+                double m1 = HUGE_VAL, m2 = HUGE_VAL;
+                for (int t = h - 2 * W; t < h - W; t++) m1 = fmin(m1, gm_hist[t]);
+                for (int t = h - W; t < h; t++) m2 = fmin(m2, gm_hist[t]);
+                stalled = m2 > 0.98 * m1;
             }
-            if (nr == 0 || mr < set.switch_factor * tol_.absolute ||
-                since_best >= set.stall_iterations) {
+            if (nr == 0 || mr < set.switch_factor * tol_.absolute || stalled) {
                 *switched = true;
+                fp32_exit_ = stalled ? 1 : 0;
                 break;
             }
         }
@@ -495,8 +502,10 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
             run_gram(gb);
         }
         tm_->begin(kDense);
+        // FP32 iterates: explicit X blocks, and P is dropped rather than the
+        // phase abandoned when the basis has lost rank at FP32 accuracy
         rr_solve_implicit(GA_, GB_, ld3_, k, nr, preq, lam_, lam_new_, coef_, ld3_, st_,
-                          explicit_x);
+                          explicit_x, explicit_x);
         tm_->end();
         // P' = R C_R + P C_P and X' = X C_X + P' (likewise A-, B-blocks), in place
         UpdGroup g[3] = {};
@@ -527,6 +536,12 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
         diagdot_cols(prec_, n_, R_, ld_, colmap_identity(nr), R_, ld_, colmap_identity(nr), rn2_);
         tm_->end();
         read_tail(tail, &hs);
+        if (hs.abort && fp32_phase) {
+            // an FP32 breakdown ends the FP32 phase; X is still the last update
+            *switched = true;
+            fp32_exit_ = 2;
+            break;
+        }
         if (hs.abort) {
             if (hs.r_info) {
                 flag = hs.r_info;
@@ -612,6 +627,7 @@ int device_lobpcg(mv_MultiVectorPtr X, void *Ad, Op opA, void *Bd, Op opB, void 
         Phase ph(HIPMV_FP32, x->size, k, ny, opA, Ad, opB, Bd, opT, Td, y, tol, verb, out, &tm);
         flag = ph.run(x->data, &it, maxit, true, set, &switched);
         x->stats.iterations_fp32 = it;
+        x->stats.fp32_exit = ph.fp32_exit_;
     }
     if (set.mode != HIPLOBPCG_MIXED || (switched && flag == 0)) {
         const int it0 = it;

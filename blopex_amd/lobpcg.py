@@ -98,7 +98,7 @@ def interpreter() -> L.Interpreter:
 
 def solve_device(A: DeviceCSR, X: DeviceBlock, B=None, T=None, Y: DeviceBlock | None = None,
                  tol: float = 1e-6, rtol: float = 1e-50, maxit: int = 200, verbosity: int = 0,
-                 mode: str = "fp64", switch_factor: float = 4.0, stall_iterations: int = 8,
+                 mode: str = "fp64", switch_factor: float = 4.0, stall_iterations: int = 50,
                  refine_iterations: int = 0, histories: bool = True, profile: bool = False) -> Result:
     """lobpcg_solve_double on device-resident operands; X is overwritten."""
     lib = L.load()

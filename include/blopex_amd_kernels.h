@@ -29,11 +29,13 @@ int blopex_amd_kernel_gram(int precision, long n, int lw, int qw, const double *
    lower block triangle and the R-R / P-P blocks read): coefficients
    coef ((k+a+p) x k) for the raw blocks and the k smallest Ritz values.
    explicit_x: X'AX, X'BX from the buffers instead of diag(lam_in), I.
+   retry_without_p: a pencil that is not positive definite is solved again
+   without P (p_info = -1; tridiagonal solver only).
    jacobi != 0 forces the cyclic-Jacobi eigensolver instead of the
    tridiagonal one.  flags = {abort, r_info, p_info, gevp_info}. */
 int blopex_amd_kernel_rr(int k, int a, int p, const double *ga, const double *gb, int lda,
-                         const double *lam_in, int explicit_x, int jacobi, double *lam_out,
-                         double *coef, int *flags);
+                         const double *lam_in, int explicit_x, int retry_without_p, int jacobi,
+                         double *lam_out, double *coef, int *flags);
 
 /* For each lane l (0..63) and accumulator register r (0..3) of the 16x16x4
    MFMA used by the kernels (FP64 or FP32), the row and column of D it holds:

@@ -44,8 +44,9 @@ typedef struct hip_LobpcgSettings {
     int mode;                 /* HIPLOBPCG_FP64 (default) or HIPLOBPCG_MIXED */
     double switch_factor;     /* MIXED: leave FP32 once every active residual is below
                                  switch_factor * atol, or FP32 has stalled (default 4) */
-    int stall_iterations;     /* MIXED: iterations without a 10% drop of the largest active
-                                 residual that count as a stall (default 8) */
+    int stall_iterations;     /* MIXED: window W; FP32 has stalled when the smallest geometric
+                                 mean of the k residual norms over the last W iterations is
+                                 not 2% below that of the W before (default 50) */
     int refine_iterations;    /* FP64 iterations with all columns active after convergence
                                  (default 0) */
     int profile;              /* 1: time kernel classes with HIP events (adds event records
@@ -66,7 +67,9 @@ typedef struct hip_LobpcgStats {
     double seconds_other;       /* residuals, norms, preconditioner, B, gathers, copies */
     double spmm_bytes;          /* algorithmic bytes of the CSR products timed above */
     int spmm_calls;
-    int pad_;
+    int fp32_exit;              /* MIXED: why the FP32 phase ended: 0 converged or within
+                                   switch_factor * atol, 1 stall, 2 breakdown (R or pencil
+                                   not positive definite in FP32) */
 } hip_LobpcgStats;
 
 typedef struct hip_Multi_Vector {

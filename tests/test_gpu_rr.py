@@ -67,7 +67,7 @@ def reference(A, B, X, R, P, lam):
     return ev[:k], Bas @ Y[:, :k]
 
 
-def run_kernel(A, B, X, R, P, lam, jacobi, explicit_x=0):
+def run_kernel(A, B, X, R, P, lam, jacobi, explicit_x=0, retry=0):
     k, a, p = X.shape[1], R.shape[1], P.shape[1]
     Raw = np.hstack([X, R, P])
     n = k + a + p
@@ -81,7 +81,7 @@ def run_kernel(A, B, X, R, P, lam, jacobi, explicit_x=0):
     flags = np.zeros(4, dtype=np.int32)
     lam_in = np.ascontiguousarray(lam)
     L.load().blopex_amd_kernel_rr(k, a, p, ga.ctypes.data, gb.ctypes.data, lda, lam_in.ctypes.data,
-                                  explicit_x, jacobi, lam_out.ctypes.data, coef.ctypes.data,
+                                  explicit_x, retry, jacobi, lam_out.ctypes.data, coef.ctypes.data,
                                   flags.ctypes.data)
     return lam_out, Raw @ coef, flags, coef
 
@@ -140,3 +140,21 @@ def test_rr_reports_rank_deficient_r(jacobi):
     R[:, 2] = 0.0
     _, _, flags, _ = run_kernel(A, B, X, R, P, lam, jacobi)
     assert flags[0] == 1 and flags[1] == 3
+
+
+def test_rr_retries_without_p():
+    """X taken as B-orthonormal (implicit X block) while the data hold 2 X, and
+    a P column along X: the pencil is indefinite.  Plain, the step fails like
+    dsygv; with retry_without_p it is solved on [X R] alone (p_info = -1, P
+    coefficients zero)."""
+    say("rr retry without P")
+    A, B, X, R, P, lam = make_case(8, 8, 4, seed=11)
+    Xs = 2.0 * X
+    P[:, 0] = X[:, 0]
+    _, _, flags, _ = run_kernel(A, B, Xs, R, P, lam, 0)
+    assert flags[0] == 1 and flags[3] > 0
+    lg, Xg, flags, coef = run_kernel(A, B, Xs, R, P, lam, 0, retry=1)
+    assert flags[0] == 0 and flags[2] == -1 and flags[3] == 0
+    assert np.all(coef[16:] == 0.0)
+    ev, _ = reference(A, B, Xs, R, P[:, :0], lam)
+    assert np.max(np.abs(lg - ev) / np.abs(ev)) < 1e-12
