@@ -6,11 +6,11 @@
 // Workgroups (8 waves) take chunks of rows round-robin (32 rows FP64, 64
 // FP32), so at any time they stream one contiguous window of every column.
 // A chunk of each distinct block of the plan is copied into LDS with 16-byte
-// coalesced loads (column-major, stride an odd number of 16-byte pieces); the next chunk is in flight
-// in registers while the current one is multiplied.  Each wave owns a
-// register block of up to 2 L tiles x 3 Q tiles (16 columns each): per chunk
-// a lane reads its rows of those 5 tiles with 16-byte LDS reads and runs the
-// 6 tile products on MFMA 16x16x4.  The k-index of the MFMA runs over the
+// coalesced loads (column-major, stride an odd number of 16-byte pieces);
+// the next two chunks are in flight in registers while one is multiplied.  Each wave owns a
+// register block of up to 2 L tiles x 4 Q tiles (16 columns each): per half
+// chunk a lane reads its rows of those tiles with 16-byte LDS reads and runs
+// the tile products on MFMA 16x16x4.  The k-index of the MFMA runs over the
 // rows of the chunk lane-group-major (lane group g = lane / 16 holds rows
 // g*RC/4 .. g*RC/4 + RC/4 - 1), the same for both operands.
 // FP64 inputs accumulate in FP64 throughout; FP32 inputs use FP32 MFMA
@@ -26,7 +26,7 @@ namespace bx {
 namespace {
 
 constexpr int kGTh = kGramWaves * 64;  // threads per workgroup
-constexpr int kSlots = kGramBL * kGramBQ;
+constexpr int kSlots = kGramBL * kGramBQ;  // partial-buffer slots per wave
 
 template <typename T>
 struct GramCfg;
@@ -45,12 +45,13 @@ struct GramKArgs {
     double *part;  // [gridDim.x][kGramTiles][256]
 };
 
-// operand vector of one lane for one tile and chunk: RC/4 consecutive rows
+// operand vector of one lane for one tile and half a chunk: RC/8
+// consecutive rows (two 16-byte LDS reads)
 template <typename T>
 struct Rows;
 template <>
 struct Rows<double> {
-    static constexpr int N = 8;
+    static constexpr int N = 4;
     double v[N];
     __device__ __forceinline__ void load(const double *p) {
 #pragma unroll
@@ -63,7 +64,7 @@ struct Rows<double> {
 };
 template <>
 struct Rows<float> {
-    static constexpr int N = 16;
+    static constexpr int N = 8;
     float v[N];
     __device__ __forceinline__ void load(const float *p) {
 #pragma unroll
@@ -83,73 +84,88 @@ struct GramBlock;
 
 template <int NL, int NQ>
 struct GramBlock<double, NL, NQ> {
+    template <int BQ>
     static __device__ __forceinline__ void run(const double *stage, const int *lo, const int *qo,
-                                               f64x4 (&acc)[kSlots]) {
-        Rows<double> x[NL], y[NQ];
+                                               f64x4 (&acc)[kGramBL * BQ]) {
 #pragma unroll
-        for (int i = 0; i < NL; i++) x[i].load(stage + lo[i]);
+        for (int h = 0; h < 2; h++) {
+            Rows<double> x[NL], y[NQ];
 #pragma unroll
-        for (int j = 0; j < NQ; j++) y[j].load(stage + qo[j]);
+            for (int i = 0; i < NL; i++) x[i].load(stage + lo[i] + h * Rows<double>::N);
 #pragma unroll
-        for (int st = 0; st < Rows<double>::N; st++)
+            for (int j = 0; j < NQ; j++) y[j].load(stage + qo[j] + h * Rows<double>::N);
 #pragma unroll
-            for (int i = 0; i < NL; i++)
+            for (int st = 0; st < Rows<double>::N; st++)
 #pragma unroll
-                for (int j = 0; j < NQ; j++)
-                    acc[i * kGramBQ + j] =
-                        Mfma<double>::mma(x[i].v[st], y[j].v[st], acc[i * kGramBQ + j]);
+                for (int i = 0; i < NL; i++)
+#pragma unroll
+                    for (int j = 0; j < NQ; j++)
+                        acc[i * BQ + j] = Mfma<double>::mma(x[i].v[st], y[j].v[st], acc[i * BQ + j]);
+        }
     }
 };
 
 template <int NL, int NQ>
 struct GramBlock<float, NL, NQ> {
+    template <int BQ>
     static __device__ __forceinline__ void run(const float *stage, const int *lo, const int *qo,
-                                               f64x4 (&acc)[kSlots]) {
-        Rows<float> x[NL], y[NQ];
-#pragma unroll
-        for (int i = 0; i < NL; i++) x[i].load(stage + lo[i]);
-#pragma unroll
-        for (int j = 0; j < NQ; j++) y[j].load(stage + qo[j]);
+                                               f64x4 (&acc)[kGramBL * BQ]) {
         f32x4 c[NL * NQ];
 #pragma unroll
         for (int t = 0; t < NL * NQ; t++) c[t] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int st = 0; st < Rows<float>::N; st++)
+        for (int h = 0; h < 2; h++) {
+            Rows<float> x[NL], y[NQ];
 #pragma unroll
-            for (int i = 0; i < NL; i++)
+            for (int i = 0; i < NL; i++) x[i].load(stage + lo[i] + h * Rows<float>::N);
 #pragma unroll
-                for (int j = 0; j < NQ; j++)
-                    c[i * NQ + j] = Mfma<float>::mma(x[i].v[st], y[j].v[st], c[i * NQ + j]);
+            for (int j = 0; j < NQ; j++) y[j].load(stage + qo[j] + h * Rows<float>::N);
+#pragma unroll
+            for (int st = 0; st < Rows<float>::N; st++)
+#pragma unroll
+                for (int i = 0; i < NL; i++)
+#pragma unroll
+                    for (int j = 0; j < NQ; j++)
+                        c[i * NQ + j] = Mfma<float>::mma(x[i].v[st], y[j].v[st], c[i * NQ + j]);
+        }
 #pragma unroll
         for (int i = 0; i < NL; i++)
 #pragma unroll
             for (int j = 0; j < NQ; j++)
 #pragma unroll
-                for (int r = 0; r < 4; r++) acc[i * kGramBQ + j][r] += (double)c[i * NQ + j][r];
+                for (int r = 0; r < 4; r++) acc[i * BQ + j][r] += (double)c[i * NQ + j][r];
     }
 };
 
-template <typename T>
+template <typename T, int BQ>
 __device__ __forceinline__ void gram_block(int nl, int nq, const T *stage, const int *lo,
-                                           const int *qo, f64x4 (&acc)[kSlots]) {
-    switch (nl * 4 + nq) {  // wave-uniform
-        case 5: GramBlock<T, 1, 1>::run(stage, lo, qo, acc); break;
-        case 6: GramBlock<T, 1, 2>::run(stage, lo, qo, acc); break;
-        case 7: GramBlock<T, 1, 3>::run(stage, lo, qo, acc); break;
-        case 9: GramBlock<T, 2, 1>::run(stage, lo, qo, acc); break;
-        case 10: GramBlock<T, 2, 2>::run(stage, lo, qo, acc); break;
-        case 11: GramBlock<T, 2, 3>::run(stage, lo, qo, acc); break;
-        default: break;
+                                           const int *qo, f64x4 (&acc)[kGramBL * BQ]) {
+    switch (nl * 8 + nq) {  // wave-uniform
+        case 9: GramBlock<T, 1, 1>::template run<BQ>(stage, lo, qo, acc); break;
+        case 10: GramBlock<T, 1, 2>::template run<BQ>(stage, lo, qo, acc); break;
+        case 11: GramBlock<T, 1, 3>::template run<BQ>(stage, lo, qo, acc); break;
+        case 17: GramBlock<T, 2, 1>::template run<BQ>(stage, lo, qo, acc); break;
+        case 18: GramBlock<T, 2, 2>::template run<BQ>(stage, lo, qo, acc); break;
+        case 19: GramBlock<T, 2, 3>::template run<BQ>(stage, lo, qo, acc); break;
+        default:
+            if constexpr (BQ == 4) {
+                if (nl == 1) GramBlock<T, 1, 4>::template run<BQ>(stage, lo, qo, acc);
+                if (nl == 2) GramBlock<T, 2, 4>::template run<BQ>(stage, lo, qo, acc);
+            }
+            break;
     }
 }
 
-template <typename T>
+// PRE: 16-byte pieces per thread and chunk (6: plans up to 192 columns, two
+// chunks in flight; 9: up to 288 columns, one); BQ: right tiles per register
+// block (3 or 4)
+template <typename T, int PRE, int BQ>
 __global__ void __launch_bounds__(kGTh) k_gram(GramKArgs a) {
     constexpr int RC = GramCfg<T>::RC;
+    constexpr bool kTwo = PRE <= 6;
     constexpr int EV = 16 / sizeof(T);  // elements per 16-byte piece
     constexpr int S = RC + EV;          // column stride: an odd number of 16-byte pieces
     constexpr int VPC = RC / EV;                     // pieces per column per chunk
-    constexpr int PRE = (kGramCols * VPC + kGTh - 1) / kGTh;
     static_assert(PRE <= 9, "Gram staging: too many pieces per thread");
     extern __shared__ __align__(16) unsigned char smem[];
     const GramPlan &p = a.p;
@@ -170,20 +186,23 @@ __global__ void __launch_bounds__(kGTh) k_gram(GramKArgs a) {
     const long gs = gridDim.x;
     const bool act = wave < p.nwb;
     const int nl = act ? p.wnl[wave] : 0, nq = act ? p.wnq[wave] : 0;
-    int lo[kGramBL], qo[kGramBQ];
+    int lo[kGramBL], qo[BQ];
 #pragma unroll
     for (int i = 0; i < kGramBL; i++) lo[i] = ((act ? p.wl[wave][i] : 0) + li) * S + lk * (RC / 4);
 #pragma unroll
-    for (int j = 0; j < kGramBQ; j++) qo[j] = ((act ? p.wq[wave][j] : 0) + li) * S + lk * (RC / 4);
-    f64x4 acc[kSlots];
+    for (int j = 0; j < BQ; j++) qo[j] = ((act ? p.wq[wave][j] : 0) + li) * S + lk * (RC / 4);
+    f64x4 acc[kGramBL * BQ];
 #pragma unroll
-    for (int s = 0; s < kSlots; s++) acc[s] = f64x4{0.0, 0.0, 0.0, 0.0};
+    for (int s = 0; s < kGramBL * BQ; s++) acc[s] = f64x4{0.0, 0.0, 0.0, 0.0};
 
     // the (up to 9) 16-byte pieces this thread copies per chunk, in named
-    // registers (an array of them would live in scratch memory); pieces past
-    // the plan load piece 0 and land in the spare columns
-    uint4 p0, p1, p2, p3, p4, p5, p6, p7, p8;
-#define BX_GRAM_PIECES(X) X(0, p0) X(1, p1) X(2, p2) X(3, p3) X(4, p4) X(5, p5) X(6, p6) X(7, p7) X(8, p8)
+    // registers (an array of them would live in scratch memory), two chunks
+    // in flight (sets p and q); pieces past the plan load piece 0 and land in
+    // the spare columns
+    uint4 p0, p1, p2, p3, p4, p5, p6, p7, p8, q0, q1, q2, q3, q4, q5, q6, q7, q8;
+#define BX_GRAM_P(X) X(0, p0) X(1, p1) X(2, p2) X(3, p3) X(4, p4) X(5, p5) X(6, p6) X(7, p7) X(8, p8)
+#define BX_GRAM_Q(X) \
+    if (kTwo) { X(0, q0) X(1, q1) X(2, q2) X(3, q3) X(4, q4) X(5, q5) X(6, q6) X(7, q7) X(8, q8) }
 #define BX_GRAM_LOAD(q, r)                                                                    \
     if (q < PRE) {                                                                            \
         const int e = tid + q * kGTh < nvec ? tid + q * kGTh : 0;                             \
@@ -195,31 +214,50 @@ __global__ void __launch_bounds__(kGTh) k_gram(GramKArgs a) {
         const int o = e < nvec ? (e / VPC) * S + (e % VPC) * EV : (p.ncols + (tid & 15)) * S; \
         *reinterpret_cast<uint4 *>(stage + o) = r;                                            \
     }
-    if ((long)blockIdx.x < nch) {
-        const long chl = blockIdx.x;
-        BX_GRAM_PIECES(BX_GRAM_LOAD)
+    const long c0 = blockIdx.x;
+    if (c0 < nch) {
+        const long chl = c0;
+        BX_GRAM_P(BX_GRAM_LOAD)
     }
-    for (long ch = blockIdx.x; ch < nch; ch += gs) {
-        BX_GRAM_PIECES(BX_GRAM_STORE)
+    if (kTwo && c0 + gs < nch) {
+        const long chl = c0 + gs;
+        BX_GRAM_Q(BX_GRAM_LOAD)
+    }
+    const long step = kTwo ? 2 * gs : gs;
+    for (long ch = c0; ch < nch; ch += step) {
+        BX_GRAM_P(BX_GRAM_STORE)
         __syncthreads();
-        if (ch + gs < nch) {  // next chunk in flight during the products
-            const long chl = ch + gs;
-            BX_GRAM_PIECES(BX_GRAM_LOAD)
+        if (ch + step < nch) {
+            const long chl = ch + step;
+            BX_GRAM_P(BX_GRAM_LOAD)
         }
-        gram_block<T>(nl, nq, stage, lo, qo, acc);
+        gram_block<T, BQ>(nl, nq, stage, lo, qo, acc);
+        __syncthreads();
+        if (!kTwo || ch + gs >= nch) continue;
+        BX_GRAM_Q(BX_GRAM_STORE)
+        __syncthreads();
+        if (ch + 3 * gs < nch) {
+            const long chl = ch + 3 * gs;
+            BX_GRAM_Q(BX_GRAM_LOAD)
+        }
+        gram_block<T, BQ>(nl, nq, stage, lo, qo, acc);
         __syncthreads();
     }
+#undef BX_GRAM_Q
+#undef BX_GRAM_P
 #undef BX_GRAM_STORE
 #undef BX_GRAM_LOAD
-#undef BX_GRAM_PIECES
     if (!act) return;
     double *base = a.part + ((size_t)blockIdx.x * kGramTiles + (size_t)wave * kSlots) * 256;
 #pragma unroll
-    for (int s = 0; s < kSlots; s++) {
-        if (s / kGramBQ >= nl || s % kGramBQ >= nq) continue;
+    for (int i = 0; i < kGramBL; i++)
 #pragma unroll
-        for (int r = 0; r < 4; r++) base[s * 256 + Mfma<T>::drow(lk, r) * 16 + li] = acc[s][r];
-    }
+        for (int j = 0; j < BQ; j++) {
+            if (i >= nl || j >= nq) continue;
+            double *t = base + (i * kGramBQ + j) * 256;
+#pragma unroll
+            for (int r = 0; r < 4; r++) t[Mfma<T>::drow(lk, r) * 16 + li] = acc[i * BQ + j][r];
+        }
 }
 
 struct ReduceArgs {
@@ -266,8 +304,9 @@ int plan_block(GramPlan &p, const void *blk, int width) {
 
 // Forms the register blocks from the queued products: for each left block,
 // its 16-column tiles in pairs, against all right tiles of its products in
-// threes.  Returns the number of register blocks (may exceed kGramWaves).
-int form_blocks(GramPlan &p, long ldg, bool fill) {
+// groups of bq.  Returns the number of register blocks (may exceed
+// kGramWaves).
+int form_blocks(GramPlan &p, long ldg, bool fill, int bq) {
     int nw = 0;
     bool done[kGramBlocks] = {};
     for (int q0 = 0; q0 < p.nprod; q0++) {
@@ -288,8 +327,8 @@ int form_blocks(GramPlan &p, long ldg, bool fill) {
         const int lw = p.bw[lb], lt = (lw + 15) / 16;
         for (int x0 = 0; x0 < lt; x0 += kGramBL) {
             const int nl = lt - x0 < kGramBL ? lt - x0 : kGramBL;
-            for (int j0 = 0; j0 < nqt; j0 += kGramBQ) {
-                const int nq = nqt - j0 < kGramBQ ? nqt - j0 : kGramBQ;
+            for (int j0 = 0; j0 < nqt; j0 += bq) {
+                const int nq = nqt - j0 < bq ? nqt - j0 : bq;
                 if (fill && nw < kGramWaves) {
                     const int w = nw;
                     p.wnl[w] = (unsigned char)nl;
@@ -302,7 +341,7 @@ int form_blocks(GramPlan &p, long ldg, bool fill) {
                     }
                     for (int i = 0; i < kGramBL; i++)
                         for (int j = 0; j < kGramBQ; j++) {
-                            const int s = w * kGramBL * kGramBQ + i * kGramBQ + j;
+                            const int s = (w * kGramBL + i) * kGramBQ + j;
                             p.dst[s] = p.mdst[s] = nullptr;
                             p.rows[s] = p.cols[s] = 0;
                             if (i >= nl || j >= nq) continue;
@@ -336,7 +375,7 @@ bool gram_plan_product(GramPlan &p0, const void *L, int lw, const void *Q, int q
     p.pq[p.nprod] = (short)qb;
     p.pg[p.nprod] = g;
     p.nprod++;
-    if (form_blocks(p, ldg, false) > kGramWaves) return false;
+    if (form_blocks(p, ldg, false, kGramBQ) > kGramWaves) return false;
     p0 = p;
     return true;
 }
@@ -357,7 +396,10 @@ void GramBatch::flush() {
 void gram_run(int prec, long ld, GramPlan &p, long ldg) {
     BX_REQUIRE(ld % 64 == 0, "gram_run: leading dimension %ld not a multiple of 64", ld);
     if (p.nprod == 0) return;
-    form_blocks(p, ldg, true);
+    // groups of three right tiles spread the work over more waves; four when
+    // three would need more register blocks than there are waves
+    const int bq = form_blocks(p, ldg, false, 3) <= kGramWaves ? 3 : kGramBQ;
+    form_blocks(p, ldg, true, bq);
     const size_t es = prec == HIPMV_FP32 ? sizeof(float) : sizeof(double);
     const int rc = prec == HIPMV_FP32 ? GramCfg<float>::RC : GramCfg<double>::RC;
     const int S = rc + (int)(16 / es);
@@ -371,19 +413,27 @@ void gram_run(int prec, long ld, GramPlan &p, long ldg) {
     a.p = p;
     a.ld = ld;
     a.part = part;
-    static bool attr = false;
-    if (!attr) {
-        BX_HIP(hipFuncSetAttribute((const void *)k_gram<float>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        BX_HIP(hipFuncSetAttribute((const void *)k_gram<double>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        attr = true;
-    }
     BX_REQUIRE(lds <= 160 * 1024, "gram_run: %zu bytes of LDS", lds);
+    const int vpc = rc / (int)(16 / es);
+    const bool small = p.ncols * vpc <= 6 * kGTh;  // 6 pieces per thread suffice
+    const void *kern;
     if (prec == HIPMV_FP32)
-        hipLaunchKernelGGL(k_gram<float>, dim3((unsigned)parts), dim3(kGTh), lds, stream(), a);
+        kern = small ? (bq == 3 ? (const void *)k_gram<float, 6, 3> : (const void *)k_gram<float, 6, 4>)
+                     : (bq == 3 ? (const void *)k_gram<float, 9, 3> : (const void *)k_gram<float, 9, 4>);
     else
-        hipLaunchKernelGGL(k_gram<double>, dim3((unsigned)parts), dim3(kGTh), lds, stream(), a);
+        kern = small ? (bq == 3 ? (const void *)k_gram<double, 6, 3> : (const void *)k_gram<double, 6, 4>)
+                     : (bq == 3 ? (const void *)k_gram<double, 9, 3> : (const void *)k_gram<double, 9, 4>);
+    static const void *attr_done[8];
+    for (int i = 0; i < 8; i++) {
+        if (attr_done[i] == kern) break;
+        if (!attr_done[i]) {
+            BX_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            attr_done[i] = kern;
+            break;
+        }
+    }
+    void *args[] = {&a};
+    BX_HIP(hipLaunchKernel(kern, dim3((unsigned)parts), dim3(kGTh), args, lds, stream()));
     BX_HIP(hipGetLastError());
     ReduceArgs r;
     int nt = 0;

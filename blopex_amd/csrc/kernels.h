@@ -58,12 +58,12 @@ void block_update(int prec, long ld, const UpdGroup *groups, int ngroups, const 
 
 // Fused Gram products G = L' Q: every block of a plan is staged through LDS
 // a chunk of rows at a time, and each wave accumulates a register block of
-// up to 2 x 3 output tiles (16 x 16) on MFMA in FP64 (FP32 inputs: FP32 MFMA
+// up to 2 x 4 output tiles (16 x 16) on MFMA in FP64 (FP32 inputs: FP32 MFMA
 // over each chunk, then FP64).
 constexpr int kGramBlocks = 10;    // distinct n-row blocks per plan
 constexpr int kGramCols = 288;     // staged columns per plan (sum of block widths)
 constexpr int kGramWaves = 8;      // waves per workgroup = register blocks per plan
-constexpr int kGramBL = 2, kGramBQ = 3;                    // tiles of a register block
+constexpr int kGramBL = 2, kGramBQ = 4;                    // largest register block (tiles)
 constexpr int kGramTiles = kGramWaves * kGramBL * kGramBQ;  // tile slots per plan
 struct GramPlan {
     const void *blk[kGramBlocks];
@@ -74,7 +74,7 @@ struct GramPlan {
     short wl[kGramWaves][kGramBL];            // staged column of each L tile
     short wq[kGramWaves][kGramBQ];            // staged column of each Q tile
     unsigned char wnl[kGramWaves], wnq[kGramWaves];
-    // tile (w, i, j) = slot w * 6 + i * 3 + j
+    // tile (w, i, j) = slot (w * kGramBL + i) * kGramBQ + j
     unsigned char rows[kGramTiles], cols[kGramTiles];  // valid part of the tile
     double *dst[kGramTiles];   // tile origin (FP64, leading dim ldg); null: not stored
     double *mdst[kGramTiles];  // origin of the transposed copy (symmetric products) or null
