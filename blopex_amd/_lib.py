@@ -53,7 +53,9 @@ class Interpreter(ctypes.Structure):
 
 class CSRMatrix(ctypes.Structure):
     _fields_ = [("n", c_long), ("nnz", c_long), ("rowptr", c_void_p), ("colind", c_void_p),
-                ("val", c_void_p), ("val32", c_void_p)]
+                ("val", c_void_p), ("val32", c_void_p), ("ndiag", c_int),
+                ("diag_offset", c_int * 16), ("diag_ld", c_long), ("diag_val", c_void_p),
+                ("diag_val32", c_void_p)]
 
 
 class DiagMatrix(ctypes.Structure):

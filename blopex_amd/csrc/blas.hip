@@ -181,6 +181,43 @@ __global__ void k_rowsum(const double *part, int nparts, int count, double *out)
     out[j] = s;
 }
 
+// residual, its squared norm partials (the k_diagdot partition and order)
+// and the optional diagonal scaling, in one pass
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_resid_norm(long n, const T *bx, const T *ax, long ld,
+                                                         ColMap act, const double *lam, T *r,
+                                                         long ldr, const T *tdiag, double *part) {
+    __shared__ double red[kThreads / 64];
+    const int t = blockIdx.y;
+    const int c = act.idx[t];
+    const T l = (T)lam[c];
+    const T *bc = colp(bx, ld, c), *ac = colp(ax, ld, c);
+    T *rc = colp(r, ldr, t);
+    double s = 0.0;
+    for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < n; i += (long)gridDim.x * kThreads) {
+        const T v = l * bc[i] - ac[i];
+        s = fma((double)v, (double)v, s);
+        rc[i] = tdiag ? tdiag[i] * v : v;
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double u = 0.0;
+        for (int w = 0; w < kThreads / 64; w++) u += red[w];
+        part[(size_t)t * gridDim.x + blockIdx.x] = u;
+    }
+}
+
+// one thread per row walks the destinations in ascending order, so every
+// source is read before a later destination could overwrite it
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_compact(long n, T *x, long ld, ColMap src) {
+    for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < n; i += (long)gridDim.x * kThreads)
+        for (int t = 0; t < src.n; t++)
+            if (src.idx[t] != t) x[(size_t)t * ld + i] = x[(size_t)src.idx[t] * ld + i];
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kThreads) k_vecmat(long n, const T *x, long ldx, ColMap xm,
                                                      const double *c, long ldc, T *y, long ldy,
@@ -385,6 +422,45 @@ void vecmat_cols(int prec, long n, const void *x, long ldx, const ColMap &xm, co
         hipLaunchKernelGGL(k_vecmat<double>, g, dim3(kThreads), 0, stream(), n,
                            (const double *)x, ldx, xm, c, ldc, (double *)y, ldy, ym,
                            (int)accumulate);
+    BX_HIP(hipGetLastError());
+}
+
+void residual_norm_cols(int prec, long n, const void *bx, const void *ax, long ld,
+                        const ColMap &act, const double *lam, void *r, long ldr, const void *t,
+                        double *d) {
+    if (act.n == 0 || n == 0) return;
+    unsigned nparts = row_grid(n);
+    if (nparts > 512) nparts = 512;  // the diagdot_cols partition
+    double *part = (double *)scratch(1, (size_t)nparts * act.n * sizeof(double));
+    dim3 g(nparts, act.n);
+    if (prec == HIPMV_FP32)
+        hipLaunchKernelGGL(k_resid_norm<float>, g, dim3(kThreads), 0, stream(), n,
+                           (const float *)bx, (const float *)ax, ld, act, lam, (float *)r, ldr,
+                           (const float *)t, part);
+    else
+        hipLaunchKernelGGL(k_resid_norm<double>, g, dim3(kThreads), 0, stream(), n,
+                           (const double *)bx, (const double *)ax, ld, act, lam, (double *)r, ldr,
+                           (const double *)t, part);
+    BX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_rowsum, dim3((act.n + 63) / 64), dim3(64), 0, stream(), part, (int)nparts,
+                       act.n, d);
+    BX_HIP(hipGetLastError());
+}
+
+void compact_cols(int prec, long n, void *x, long ld, const ColMap &src) {
+    bool moves = false;
+    for (int t = 0; t < src.n; t++) {
+        BX_REQUIRE(src.idx[t] >= t && (t == 0 || src.idx[t] > src.idx[t - 1]),
+                   "compact_cols: sources must ascend from their destinations");
+        moves = moves || src.idx[t] != t;
+    }
+    if (!moves || n == 0) return;
+    dim3 g(row_grid(n));
+    if (prec == HIPMV_FP32)
+        hipLaunchKernelGGL(k_compact<float>, g, dim3(kThreads), 0, stream(), n, (float *)x, ld, src);
+    else
+        hipLaunchKernelGGL(k_compact<double>, g, dim3(kThreads), 0, stream(), n, (double *)x, ld,
+                           src);
     BX_HIP(hipGetLastError());
 }
 

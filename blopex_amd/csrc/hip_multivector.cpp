@@ -5,6 +5,7 @@
 // Slot semantics follow the serial backend (multi_vector.c): every operation
 // touches only the active columns, in ascending order; Gram results land in
 // caller (host) matrices with leading dimension gh; MultiVecMat overwrites.
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -319,6 +320,46 @@ int blopex_amd_is_hip_interpreter(const mv_InterfaceInterpreter *i) {
 
 // ---- operators -------------------------------------------------------------
 
+// Diagonal copy of A for the gather-free product (see hip_CSRMatrix): offsets
+// c - r of all nonzeros; at most HIP_CSR_MAX_DIAGONALS of them, at least half
+// of the diagonal slots filled, sorted distinct columns in every row.
+static void build_diagonals(hip_CSRMatrix *A, const int *rowptr, const int *colind,
+                            const double *val) {
+    const long n = A->n, nnz = A->nnz;
+    A->ndiag = 0;
+    if (n < 4 || nnz == 0) return;
+    std::vector<long> offs;
+    for (long r = 0; r < n; r++) {
+        for (int e = rowptr[r]; e < rowptr[r + 1]; e++) {
+            if (e > rowptr[r] && colind[e] <= colind[e - 1]) return;  // unsorted or repeated
+            const long o = (long)colind[e] - r;
+            if (std::find(offs.begin(), offs.end(), o) == offs.end()) {
+                if ((int)offs.size() == HIP_CSR_MAX_DIAGONALS) return;
+                offs.push_back(o);
+            }
+        }
+    }
+    if (2 * nnz < (long)offs.size() * n) return;  // too sparse for diagonals
+    std::sort(offs.begin(), offs.end());
+    const int nd = (int)offs.size();
+    const long ld = (n + 3) / 4 * 4;
+    std::vector<double> dv((size_t)nd * ld, 0.0);
+    for (long r = 0; r < n; r++)
+        for (int e = rowptr[r]; e < rowptr[r + 1]; e++) {
+            const int d = (int)(std::lower_bound(offs.begin(), offs.end(), (long)colind[e] - r) -
+                                offs.begin());
+            dv[(size_t)d * ld + r] = val[e];
+        }
+    A->ndiag = nd;
+    for (int d = 0; d < nd; d++) A->diag_offset[d] = (int)offs[d];
+    A->diag_ld = ld;
+    BX_HIP(hipMalloc(&A->diag_val, sizeof(double) * nd * ld));
+    BX_HIP(hipMalloc(&A->diag_val32, sizeof(float) * nd * ld));
+    BX_HIP(hipMemcpy(A->diag_val, dv.data(), sizeof(double) * nd * ld, hipMemcpyHostToDevice));
+    convert_block((long)nd * ld, 1, A->diag_val, (long)nd * ld, HIPMV_FP64, A->diag_val32,
+                  (long)nd * ld, HIPMV_FP32);
+}
+
 hip_CSRMatrix *hip_CSRMatrixCreate(long n, long nnz, const int *rowptr, const int *colind,
                                    const double *val) {
     BX_REQUIRE(n >= 0 && nnz >= 0, "hip_CSRMatrixCreate: bad sizes");
@@ -340,6 +381,7 @@ hip_CSRMatrix *hip_CSRMatrixCreate(long n, long nnz, const int *rowptr, const in
     BX_HIP(hipMemcpyAsync(A->colind, colind, sizeof(int) * nnz, hipMemcpyHostToDevice, s));
     BX_HIP(hipMemcpyAsync(A->val, val, sizeof(double) * nnz, hipMemcpyHostToDevice, s));
     convert_block(nnz, 1, A->val, nnz, HIPMV_FP64, A->val32, nnz, HIPMV_FP32);
+    build_diagonals(A, rowptr, colind, val);
     BX_HIP(hipStreamSynchronize(s));
     return A;
 }
@@ -351,6 +393,10 @@ void hip_CSRMatrixDestroy(hip_CSRMatrix *A) {
     BX_HIP(hipFree(A->colind));
     BX_HIP(hipFree(A->val));
     BX_HIP(hipFree(A->val32));
+    if (A->ndiag) {
+        BX_HIP(hipFree(A->diag_val));
+        BX_HIP(hipFree(A->diag_val32));
+    }
     free(A);
 }
 

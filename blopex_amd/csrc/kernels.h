@@ -35,6 +35,15 @@ void vecmat_cols(int prec, long n, const void *x, long ldx, const ColMap &xm, co
 // r(:, t) = lam[act[t]] * bx(:, act[t]) - ax(:, act[t]), t < act.n (lam on the device)
 void residual_cols(int prec, long n, const void *bx, const void *ax, long ld, const ColMap &act,
                    const double *lam, void *r, long ldr);
+// The same residual columns, their squared norms d[t] (device FP64, summed
+// exactly as diagdot_cols sums them) and, when t is not null, r(:, t) scaled
+// by the diagonal t (the Jacobi preconditioner) after the norms are taken:
+// one pass instead of residual_cols + diagdot_cols + the diagonal operator.
+void residual_norm_cols(int prec, long n, const void *bx, const void *ax, long ld,
+                        const ColMap &act, const double *lam, void *r, long ldr, const void *t,
+                        double *d);
+// in-place compaction: x(:, i) = x(:, src[i]) for i < src.n, src ascending with src[i] >= i
+void compact_cols(int prec, long n, void *x, long ld, const ColMap &src);
 // c (m x w) = alpha * op(a) * op(b), all device FP64 column-major, inner size l
 void small_mm(double alpha, const double *a, long lda, bool ta, const double *b, long ldb, bool tb,
               double *c, long ldc, int m, int w, int l);

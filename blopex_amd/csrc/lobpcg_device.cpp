@@ -147,6 +147,7 @@ class Phase {
           Td_(Td), Yu_(Yuser), tol_(tol), verb_(verb), out_(out), tm_(tm) {
         noB_ = opB == nullptr;
         noT_ = opT == nullptr;
+        fuseT_ = opT == HipDiagMatMultiVec;
         es_ = elem_size(prec);
         ld3_ = 3 * k;
         ld_ = (n + 63) / 64 * 64;  // padded leading dimension; padding rows stay zero
@@ -251,14 +252,21 @@ class Phase {
         tm_->resolve();
     }
     void gather(void *buf, const std::vector<int> &src) {
-        // in-place, one column per launch in ascending order (src[t] >= t)
-        for (size_t t = 0; t < src.size(); t++)
-            if (src[t] != (int)t) {
-                ColMap a = colmap_identity(1), b = colmap_identity(1);
-                a.idx[0] = src[t];
-                b.idx[0] = (int)t;
-                copy_cols(prec_, n_, buf, ld_, a, buf, ld_, b);
-            }
+        // in place (src ascending, src[t] >= t), one launch
+        ColMap m = colmap_identity((int)src.size());
+        for (size_t t = 0; t < src.size(); t++) m.idx[t] = src[t];
+        compact_cols(prec_, n_, buf, ld_, m);
+    }
+    // residuals R = BX diag(lam) - AX of the columns in m and their squared
+    // norms; with the library's diagonal preconditioner R is also scaled by it
+    // here (after the norms), instead of by a separate pass next iteration
+    void residuals(const void *BXp, const ColMap &m, const double *lam) {
+        const void *t = nullptr;
+        if (fuseT_) {
+            const hip_DiagMatrix *T = (const hip_DiagMatrix *)Td_;
+            t = prec_ == HIPMV_FP32 ? (const void *)T->d32 : (const void *)T->d;
+        }
+        residual_norm_cols(prec_, n_, BXp, AX_, ld_, m, lam, R_, ld_, t, rn2_);
     }
     void project_constraints(void *blk, int w);
     void store_hist(int col) {
@@ -292,7 +300,7 @@ class Phase {
     int verb_;
     HostOut out_;
     KTimer *tm_;
-    bool noB_, noT_;
+    bool noB_, noT_, fuseT_;
     size_t es_;
     int ld3_;
     long ld_;
@@ -383,8 +391,7 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
         update(g, ng, coef_, k, k);
         ColMap all = colmap_identity(k);
         tm_->begin(kOther);
-        residual_cols(prec_, n_, BXp, AX_, ld_, all, lam_new_, R_, ld_);
-        diagdot_cols(prec_, n_, R_, ld_, all, R_, ld_, all, rn2_);
+        residuals(BXp, all, lam_new_);
         tm_->end();
         read_tail(tail, &hs);
         if (hs.abort) {
@@ -467,7 +474,7 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
             }
             tm_->end();
         }
-        if (!noT_) {
+        if (!noT_ && !fuseT_) {
             apply(opT_, Td_, R_, W_, nr);
             std::swap(R_, W_);
         }
@@ -532,8 +539,7 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
         ColMap am = colmap_identity(nr);
         for (int t = 0; t < nr; t++) am.idx[t] = act[t];
         tm_->begin(kOther);
-        residual_cols(prec_, n_, BXp, AX_, ld_, am, lam_new_, R_, ld_);
-        diagdot_cols(prec_, n_, R_, ld_, colmap_identity(nr), R_, ld_, colmap_identity(nr), rn2_);
+        residuals(BXp, am, lam_new_);
         tm_->end();
         read_tail(tail, &hs);
         if (hs.abort && fp32_phase) {

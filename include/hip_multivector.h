@@ -111,6 +111,7 @@ int hip_Multi_VectorGetStats(const hip_Multi_Vector *x, hip_LobpcgStats *s);
 BlopexInt HipSetupInterpreter(mv_InterfaceInterpreter *ii);
 
 /* ---- operators ----------------------------------------------------------- */
+#define HIP_CSR_MAX_DIAGONALS 16
 typedef struct hip_CSRMatrix {
     long n;
     long nnz;
@@ -118,6 +119,15 @@ typedef struct hip_CSRMatrix {
     int *colind; /* device */
     double *val; /* device FP64 */
     float *val32;/* device FP32 copy (for FP32 blocks) */
+    /* diagonal copy, built when the nonzeros lie on at most
+       HIP_CSR_MAX_DIAGONALS diagonals filled to at least one half and the
+       rows have sorted, distinct columns (products then sum in the same
+       order as the CSR kernel); ndiag = 0 otherwise */
+    int ndiag;
+    int diag_offset[HIP_CSR_MAX_DIAGONALS]; /* column - row, ascending */
+    long diag_ld;                           /* >= n, multiple of 4 */
+    double *diag_val;                       /* device, ndiag x diag_ld, 0 off the matrix */
+    float *diag_val32;
 } hip_CSRMatrix;
 
 typedef struct hip_DiagMatrix {
