@@ -12,10 +12,13 @@ A step is one full solve to tolerance in MIXED mode (FP32 blocks, FP64
 Gram accumulation / small dense / finish), starting from X0 already in HBM.
 `value` is its time-to-tolerance (s, lower is better).  The same solve in
 FP64 mode is timed once after the steps, and the JSON line reports the
-ratio.  The roofline object is the CSR SpMM kernel (HIP events around
+ratio.  The roofline object is the SpMM kernel (the diagonal-copy kernel
+k_dia when the matrix has one, else the CSR kernel; HIP events around
 back-to-back launches on the library stream, k = 32 columns, FP32 -- the
-precision the mixed solve spends its iterations in); a profiled solve gives
-the per-kernel-class breakdown.  cpu_baseline is the reference's own
+precision the mixed solve spends its iterations in).  Its `traffic` (HBM
+bytes per launch) is read from the committed rocprofv3 counter summary of
+`bench.py --roofline-only` (FETCH_SIZE and WRITE_SIZE passes scaled by a
+512 MiB copy); a profiled solve gives the per-kernel-class breakdown.  cpu_baseline is the reference's own
 lobpcg_solve_double (oracle/_ref) on this host, timed on a bounded sample.
 
     python bench.py [--steps K] [--warmup W] [--n 128] [--k 32] [--no-cpu]
@@ -34,6 +37,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+COUNTERS = os.path.join(ROOT, "profiles", "spmm_roofline_128cube_k32_counters.txt")
 METRIC = ("time-to-residual 1e-6, k=32, 128³ Laplacian: mixed vs FP64 on GPU; SpMM GB/s")
 HBM_PEAK_GBS = 8000.0
 
@@ -66,7 +70,7 @@ class HipEvents:
 
 
 def spmm_roofline(lib, dA, A, k, prec, reps=30):
-    """Average duration of the CSR SpMM kernel over `reps` back-to-back launches."""
+    """Average duration of the SpMM kernel (HipCSRMatMultiVec) over `reps` back-to-back launches."""
     from blopex_amd import _lib as L
     from blopex_amd.lobpcg import DeviceBlock
 
@@ -258,11 +262,16 @@ def main():
     ms32, algo32 = spmm_roofline(lib, dA, A, k, L.HIPMV_FP32)
     ms64, algo64 = spmm_roofline(lib, dA, A, k, L.HIPMV_FP64)
     ach = algo32 / (ms32 * 1e-3) / 1e9
-    kname = (f"k_dia<float,{dA.ptr.contents.ndiag}> (SpMM on the diagonal copy of the CSR matrix, "
-             f"k={k}, FP32)" if dA.ptr.contents.ndiag else f"k_spmm<float,8> (CSR SpMM, k={k}, FP32)")
+    nd = dA.ptr.contents.ndiag
+    ksym = f"k_dia<float, {nd}>" if nd else "k_spmm<float, 8>"
+    kname = ksym + (f" (SpMM on the diagonal copy of the CSR matrix, k={k}, FP32)" if nd
+                    else f" (CSR SpMM, k={k}, FP32)")
+    from blopex_amd.profsum import read_traffic
+    traffic = read_traffic(COUNTERS, ksym) if (nside, k) == (128, 32) else None
     out["roofline"] = {"kernel": kname, "bound": "hbm",
                        "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                       "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
+                       "traffic_source": os.path.relpath(COUNTERS, ROOT) if traffic else None,
                        "algorithmic_bytes": algo32, "avg_launch_us": ms32 * 1e3}
     out["spmm_fp64"] = {"achieved": algo64 / (ms64 * 1e-3) / 1e9, "avg_launch_us": ms64 * 1e3,
                         "algorithmic_bytes": algo64}

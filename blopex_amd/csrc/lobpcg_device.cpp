@@ -216,7 +216,9 @@ class Phase {
         double bytes = 0.0;
         if (isA && op == HipCSRMatMultiVec) {
             const hip_CSRMatrix *A = (const hip_CSRMatrix *)d;
-            bytes = (double)A->nnz * (es_ + 4) + (double)(A->n + 1) * 4 + 2.0 * n_ * w * es_;
+            bytes = A->ndiag > 0 ? (double)A->ndiag * A->diag_ld * es_ + 2.0 * n_ * w * es_
+                                 : (double)A->nnz * (es_ + 4) + (double)(A->n + 1) * 4 +
+                                       2.0 * n_ * w * es_;
         }
         tm_->begin(isA ? kSpmm : kOther);
         op(d, &vx, &vy);

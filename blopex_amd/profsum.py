@@ -1,0 +1,132 @@
+"""Text summaries of rocprofv3 output (CSV format) for the profiles/ folder.
+
+    python3 -m blopex_amd.profsum stats  <trace-dir> [--title T] [--command C]...
+    python3 -m blopex_amd.profsum counters <fetch-dir> <write-dir> [--title T] [--command C]...
+
+`stats` reads `*_kernel_stats.csv` (rocprofv3 --kernel-trace --stats) and
+prints the per-kernel table.  `counters` reads the `*_counter_collection.csv`
+of one FETCH_SIZE pass and one WRITE_SIZE pass over the same program.  The
+program must also run a device-to-device copy of 512 MiB
+(`bench.py --roofline-only` does): its counter readings give the factor that
+turns each counter's KiB into bytes moved (counter reads x factor =
+512 MiB).  For every kernel the summary prints the scaled bytes per launch and
+a line `traffic <kernel> <bytes>` that bench.py reads back.
+"""
+from __future__ import annotations
+
+import argparse
+import collections
+import csv
+import glob
+import os
+import sys
+
+COPY_KERNEL = "__amd_rocclr_copyBuffer"
+COPY_BYTES = 512 << 20
+
+
+def short_name(name: str) -> str:
+    """`void bx::(anonymous namespace)::k_dia<float, 7>(long, ...)` -> `k_dia<float, 7>`."""
+    s = name.replace("(anonymous namespace)::", "")
+    if s.startswith("void "):
+        s = s[5:]
+    depth = 0
+    for i, ch in enumerate(s):
+        depth += (ch == "<") - (ch == ">")
+        if ch == "(" and depth == 0:
+            s = s[:i]
+            break
+    return s.split("::")[-1] if "<" not in s else s[:s.index("<")].split("::")[-1] + s[s.index("<"):]
+
+
+def _one(pattern: str) -> str:
+    hits = sorted(glob.glob(pattern, recursive=True))
+    if not hits:
+        raise SystemExit(f"no file matches {pattern}")
+    return hits[0]
+
+
+def stats(trace_dir: str) -> list[str]:
+    path = _one(os.path.join(trace_dir, "**", "*kernel_stats.csv"))
+    rows = list(csv.DictReader(open(path)))
+    out = [f"{'kernel':<48} {'calls':>6} {'avg us':>10} {'min us':>10} {'max us':>10} "
+           f"{'total ms':>10} {'%':>6}"]
+    for r in rows:
+        out.append(f"{short_name(r['Name'])[:48]:<48} {int(r['Calls']):>6} "
+                   f"{float(r['AverageNs']) / 1e3:>10.1f} {float(r['MinNs']) / 1e3:>10.1f} "
+                   f"{float(r['MaxNs']) / 1e3:>10.1f} {float(r['TotalDurationNs']) / 1e6:>10.3f} "
+                   f"{float(r['Percentage']):>6.2f}")
+    return out
+
+
+def _counter(pass_dir: str, counter: str) -> dict[str, list[float]]:
+    path = _one(os.path.join(pass_dir, "**", "*counter_collection.csv"))
+    vals: dict[str, list[float]] = collections.defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        if r["Counter_Name"] == counter:
+            vals[short_name(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+    return vals
+
+
+def counters(fetch_dir: str, write_dir: str) -> tuple[list[str], dict[str, float]]:
+    fetch = _counter(fetch_dir, "FETCH_SIZE")
+    write = _counter(write_dir, "WRITE_SIZE")
+    if COPY_KERNEL not in fetch or COPY_KERNEL not in write:
+        raise SystemExit(f"the passes hold no {COPY_KERNEL} dispatch to calibrate against")
+    mean = lambda v: sum(v) / len(v)
+    ff = COPY_BYTES / (mean(fetch[COPY_KERNEL]) * 1024.0)
+    fw = COPY_BYTES / (mean(write[COPY_KERNEL]) * 1024.0)
+    out = [f"calibration: {COPY_KERNEL} of {COPY_BYTES} B reads FETCH_SIZE "
+           f"{mean(fetch[COPY_KERNEL]):.0f} KiB -> factor {ff:.3f}; WRITE_SIZE "
+           f"{mean(write[COPY_KERNEL]):.0f} KiB -> factor {fw:.3f}",
+           "",
+           f"{'kernel':<40} {'disp':>5} {'FETCH KiB':>12} {'WRITE KiB':>12} {'read B':>14} "
+           f"{'written B':>14} {'traffic B':>14}"]
+    traffic = {}
+    for name in fetch:
+        if name == COPY_KERNEL or name not in write:
+            continue
+        fk, wk = mean(fetch[name]), mean(write[name])
+        rb, wb = fk * 1024.0 * ff, wk * 1024.0 * fw
+        traffic[name] = rb + wb
+        out.append(f"{name[:40]:<40} {len(fetch[name]):>5} {fk:>12.0f} {wk:>12.0f} {rb:>14.0f} "
+                   f"{wb:>14.0f} {rb + wb:>14.0f}")
+    out.append("")
+    out += [f"traffic {name} {b:.0f}" for name, b in traffic.items()]
+    return out, traffic
+
+
+def read_traffic(path: str, kernel: str) -> float | None:
+    """The `traffic <kernel> <bytes>` figure of a committed counter summary, or None."""
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic ") and line[8:].rsplit(" ", 1)[0] == kernel:
+            return float(line.rsplit(" ", 1)[1])
+    return None
+
+
+def main(argv=None) -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("what", choices=["stats", "counters"])
+    ap.add_argument("dirs", nargs="+")
+    ap.add_argument("--title", default="")
+    ap.add_argument("--command", action="append", default=[])
+    a = ap.parse_args(argv)
+    head = [a.title] if a.title else []
+    if a.command:
+        head.append("commands (from the repository root, TMPDIR=/tmp):")
+        head += [f"  {c}" for c in a.command]
+        head.append(f"summary: python3 -m blopex_amd.profsum {a.what} {' '.join(a.dirs)}")
+        head.append("")
+    if a.what == "stats":
+        body = stats(a.dirs[0])
+    else:
+        if len(a.dirs) != 2:
+            raise SystemExit("counters needs <fetch-dir> <write-dir>")
+        body, _ = counters(a.dirs[0], a.dirs[1])
+    sys.stdout.write("\n".join(head + body) + "\n")
+
+
+if __name__ == "__main__":
+    main()

@@ -65,7 +65,9 @@ typedef struct hip_LobpcgStats {
     double seconds_gram;        /* Gram products */
     double seconds_update;      /* block updates (block x small matrix, axpy) */
     double seconds_other;       /* residuals, norms, preconditioner, B, gathers, copies */
-    double spmm_bytes;          /* algorithmic bytes of the CSR products timed above */
+    double spmm_bytes;          /* algorithmic bytes of the A products timed above (value,
+                                   index and block streams; the diagonal copy has no index
+                                   stream) */
     int spmm_calls;
     int fp32_exit;              /* MIXED: why the FP32 phase ended: 0 converged or within
                                    switch_factor * atol, 1 stall, 2 breakdown (R or pencil
