@@ -273,6 +273,13 @@ def main():
                        "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
                        "traffic_source": os.path.relpath(COUNTERS, ROOT) if traffic else None,
                        "algorithmic_bytes": algo32, "avg_launch_us": ms32 * 1e3}
+    # the same launch priced with the CSR product's bytes (values, int32
+    # columns and row pointers, X once, Y once) -- the definition a CSR
+    # SpMM is quoted on; the diagonal copy needs fewer bytes than that
+    csr_bytes = A.nnz * 8 + (A.n + 1) * 4 + 2 * A.n * k * 4
+    out["roofline"]["csr_formula"] = {"bytes": csr_bytes,
+                                      "achieved": csr_bytes / (ms32 * 1e-3) / 1e9,
+                                      "frac": csr_bytes / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS}
     out["spmm_fp64"] = {"achieved": algo64 / (ms64 * 1e-3) / 1e9, "avg_launch_us": ms64 * 1e3,
                         "algorithmic_bytes": algo64}
 

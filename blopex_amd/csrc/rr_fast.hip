@@ -564,84 +564,124 @@ __global__ void __launch_bounds__(kT) k_rr_tri(RR2Args r) {
         __syncthreads();
     }
 
-    // ---- Householder tridiagonalisation of C (kept symmetric) ----
-    // H_j = I - tau v v', v = x - alpha e_1 over rows j+1..n-1 of column j;
-    // the next column is captured while the rank-2 update writes it
-    for (int i = tid; i < n - 1; i += kT) col0[i] = Am[1 + i];
-    __syncthreads();
-    for (int j = 0; j + 2 < n; j++) {
-        const double *cur = (j & 1) ? col1 : col0;
-        double *nxt = (j & 1) ? col0 : col1;
-        const int m = n - j - 1, o = j + 1;
-        double s = 0.0;
-        for (int i = lane; i < m; i += 64) s += cur[i] * cur[i];
-        s = wave_sum(s);
-        const double x0 = cur[0];
-        const double alpha = s > 0.0 ? -copysign(sqrt(s), x0) : 0.0;
-        const double v0 = x0 - alpha;
-        const double vtv = s - x0 * x0 + v0 * v0;
-        const double tau = (s > 0.0 && vtv > 0.0) ? 2.0 / vtv : 0.0;
-        if (tau != 0.0) {
-            // p = tau C v, 8 lanes per row, reading column row of C
-            const int row = tid >> 3, part = tid & 7;
-            double acc = 0.0;
-            if (row < m) {
-                const double *ccol = Am + o + (size_t)(o + row) * L;
-                for (int l = part; l < m; l += 8) acc += ccol[l] * (l == 0 ? v0 : cur[l]);
+    // ---- Householder tridiagonalisation of C, held in registers ----
+    // Waves 0-7 keep C: lane (hc = lane % 16, hr = lane / 16) of wave w owns
+    // rows w + 8 (hr + 4 q), q < 3, and columns hc + 16 b, b < 6, so a row's
+    // 96 columns lie in 16 lanes of one wave.  Step j: x = C[j+1:, j] (LDS),
+    // H_j = I - tau v v' with v = x - alpha e_1; p = tau C v (row sums in the
+    // wave), K = tau v'p / 2, w = p - K v, C -= v w' + w v'.  v goes to
+    // column j of Am below the diagonal, the next column to LDS.  Rows and
+    // columns outside j+1..n-1 have v = w = 0 and do not change.
+    {
+        constexpr int kHQ = 3, kHB = 6, kHW = 8;
+        const bool hw = wave < kHW;
+        const int hc = lane & 15, hr = lane >> 4;
+        auto hrow = [&](int q) { return wave + kHW * (hr + 4 * q); };
+        auto hcol = [&](int b) { return hc + 16 * b; };
+        double M[kHQ][kHB];
+#pragma unroll
+        for (int q = 0; q < kHQ; q++)
+#pragma unroll
+            for (int b = 0; b < kHB; b++) {
+                const int i = hrow(q), c = hcol(b);
+                M[q][b] = (hw && i < n && c < n) ? Am[i + c * L] : 0.0;
             }
-            acc = group_sum8(acc);
-            if (row < m && part == 0) {
-                const double pi = tau * acc;
-                pvec[row] = pi;
-                pv[row] = pi * (row == 0 ? v0 : cur[row]);
-            }
-        }
-        if (tid == 0) {
-            dT[j] = Am[j * (L + 1)];
-            eT[j] = tau != 0.0 ? alpha : x0;
-            tauT[j] = tau;
-        }
+        for (int i = tid; i < n - 1; i += kT) col0[i] = Am[1 + i];
         __syncthreads();
-        if (tau != 0.0) {
-            double s2 = 0.0;
-            for (int i = lane; i < m; i += 64) s2 += pv[i];
-            s2 = wave_sum(s2);
-            const double K = 0.5 * tau * s2;
-            double vi[3], wi[3];
+        for (int j = 0; j + 2 < n; j++) {
+            const double *cur = (j & 1) ? col1 : col0;
+            double *nxt = (j & 1) ? col0 : col1;
+            const int m = n - j - 1, o = j + 1;
+            double vr[kHQ], vc[kHB], pr[kHQ], tau = 0.0, xa = 0.0, xb = 0.0, v0 = 0.0;
+            if (hw) {
+                xa = lane < m ? cur[lane] : 0.0;
+                xb = lane + 64 < m ? cur[lane + 64] : 0.0;
+                const double s = wave_sum(xa * xa + xb * xb);
+                const double x0 = cur[0];
+                const double alpha = s > 0.0 ? -copysign(sqrt(s), x0) : 0.0;
+                v0 = x0 - alpha;
+                const double vtv = s - x0 * x0 + v0 * v0;
+                tau = (s > 0.0 && vtv > 0.0) ? 2.0 / vtv : 0.0;
+                auto vat = [&](int t) {
+                    const int u = t - o, uc = u < 0 ? 0 : (u < m ? u : m - 1);
+                    const double x = cur[uc];
+                    return u < 0 || u >= m ? 0.0 : (u == 0 ? v0 : x);
+                };
 #pragma unroll
-            for (int q = 0; q < 3; q++) {
-                const int i = tx + 32 * q;
-                vi[q] = i < m ? (i == 0 ? v0 : cur[i]) : 0.0;
-                wi[q] = i < m ? pvec[i] - K * vi[q] : 0.0;
-            }
-            for (int l = ty; l < m; l += 32) {
-                const double vl = l == 0 ? v0 : cur[l];
-                const double wl = pvec[l] - K * vl;
+                for (int q = 0; q < kHQ; q++) vr[q] = vat(hrow(q));
 #pragma unroll
-                for (int q = 0; q < 3; q++) {
-                    const int i = tx + 32 * q;
-                    if (i >= m) continue;
-                    double &cref = Am[(o + i) + (o + l) * L];
-                    const double nv = cref - (vi[q] * wl + wi[q] * vl);
-                    cref = nv;
-                    if (l == 0 && i >= 1) nxt[i - 1] = nv;
+                for (int b = 0; b < kHB; b++) vc[b] = vat(hcol(b));
+#pragma unroll
+                for (int q = 0; q < kHQ; q++) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int b = 0; b < kHB; b++) acc = fma(M[q][b], vc[b], acc);
+                    pr[q] = tau * group_sum16(acc);
+                    const int i = hrow(q);
+                    if (i < o) pr[q] = 0.0;
+                    if (hc == 0 && i < n) pvec[i] = pr[q];
+                }
+                if (wave == 0) {
+                    if (lane == 0) {
+                        eT[j] = tau != 0.0 ? alpha : x0;
+                        tauT[j] = tau;
+                    }
+                    // v into column j of Am (rows j+1..n-1)
+                    if (lane < m) Am[o + lane + (size_t)j * L] = lane == 0 ? v0 : xa;
+                    if (lane + 64 < m) Am[o + lane + 64 + (size_t)j * L] = xb;
                 }
             }
-            if (tid == 0) Am[o + j * L] = v0;
-        } else {
-            for (int i = tid; i < m - 1; i += kT) nxt[i] = Am[(o + 1 + i) + o * L];
+            __syncthreads();
+            if (hw) {
+                // the owner of C[j][j] (final since step j-1)
+#pragma unroll
+                for (int q = 0; q < kHQ; q++)
+#pragma unroll
+                    for (int b = 0; b < kHB; b++)
+                        if (hrow(q) == j && hcol(b) == j) dT[j] = M[q][b];
+                const double pa = lane < m ? pvec[o + lane] : 0.0;
+                const double pb = lane + 64 < m ? pvec[o + lane + 64] : 0.0;
+                const double K = 0.5 * tau * wave_sum((lane == 0 ? v0 : xa) * pa + xb * pb);
+                double wr[kHQ], wc[kHB];
+#pragma unroll
+                for (int q = 0; q < kHQ; q++) wr[q] = pr[q] - K * vr[q];
+#pragma unroll
+                for (int b = 0; b < kHB; b++) {
+                    const int c = hcol(b);
+                    const double pc = pvec[c < o ? o : (c < n ? c : n - 1)];
+                    wc[b] = c < o || c >= n ? 0.0 : pc - K * vc[b];
+                }
+#pragma unroll
+                for (int q = 0; q < kHQ; q++)
+#pragma unroll
+                    for (int b = 0; b < kHB; b++)
+                        M[q][b] = fma(-vr[q], wc[b], fma(-wr[q], vc[b], M[q][b]));
+                // next column: C[j+2:, j+1]
+#pragma unroll
+                for (int q = 0; q < kHQ; q++)
+#pragma unroll
+                    for (int b = 0; b < kHB; b++) {
+                        const int i = hrow(q);
+                        if (hcol(b) == o && i > o && i < n) nxt[i - o - 1] = M[q][b];
+                    }
+            }
+            __syncthreads();
+        }
+        // the last 2 x 2 block
+        if (hw) {
+#pragma unroll
+            for (int q = 0; q < kHQ; q++)
+#pragma unroll
+                for (int b = 0; b < kHB; b++) {
+                    const int i = hrow(q), c = hcol(b);
+                    if (n >= 2 && i == n - 2 && c == n - 2) dT[n - 2] = M[q][b];
+                    if (n >= 2 && i == n - 1 && c == n - 2) eT[n - 2] = M[q][b];
+                    if (i == n - 1 && c == n - 1) dT[n - 1] = M[q][b];
+                }
+            if (tid == 0 && n >= 2) tauT[n - 2] = 0.0;
         }
         __syncthreads();
     }
-    if (tid == 0) {
-        if (n >= 2) {
-            dT[n - 2] = Am[(n - 2) * (L + 1)];
-            eT[n - 2] = Am[(n - 1) + (n - 2) * L];
-            tauT[n - 2] = 0.0;
-        }
-        dT[n - 1] = Am[(n - 1) * (L + 1)];
-    }
-    __syncthreads();
 
     // ---- Gershgorin interval; T scaled by a power of two near its norm ----
     if (wave == 0) {
@@ -681,6 +721,29 @@ __global__ void __launch_bounds__(kT) k_rr_tri(RR2Args r) {
         const int idx = tid / G, lg = tid & (G - 1);
         if (idx < k) {
             double lo = sc[0], hi = sc[1];
+            // Bracket from the previous Ritz value e: the pencil contains the
+            // previous Ritz vectors, so its idx-th eigenvalue is about e or
+            // below.  Lane 0 tries e (plus a margin) as an upper bound, lane
+            // g > 0 tries e - |e| 16^-g as a lower bound; a bound is kept only
+            // where its Sturm count confirms it.
+            const double e = r.lam_in[idx] * sc[3];
+            if (e > lo && e < hi) {  // false for NaN
+                auto cand = [&](int g) {
+                    return g == 0 ? e + 16.0 * kEps * fabs(e) + 1e-3 * kEps
+                                  : e - fabs(e) * ldexp(1.0, -4 * g);
+                };
+                const bool above = sturm_count(dS, e2S, n, cand(lg)) > idx;
+                const unsigned long long bal = __ballot(above);
+                const unsigned seg = (unsigned)(bal >> (lane & ~(G - 1))) & gmask;
+                if (seg & 1u) hi = fmin(hi, cand(0));
+                const unsigned low = seg >> 1;  // lanes 1..G-1, points ascending
+                const int best = low ? __builtin_ctz(low) : G - 1;  // last lane not above
+                if (best >= 1) lo = fmax(lo, cand(best));
+                if (!(lo < hi)) {  // cannot happen with exact counts; start over
+                    lo = sc[0];
+                    hi = sc[1];
+                }
+            }
             for (int round = 0; round < 48; round++) {
                 const double w = hi - lo;
                 if (!(w > 2.0 * kEps * fmax(fabs(lo), fabs(hi)) + 1e-3 * kEps)) break;
@@ -833,30 +896,36 @@ __global__ void __launch_bounds__(kT) k_rr_tri(RR2Args r) {
     }
 
     // ---- Z <- Q Z: reflections n-3 .. 0, 32 lanes per column ----
+    // the lane keeps rows l32 + 32 q of its column in registers; v of
+    // reflection j is column j of Am below the diagonal (zero above)
     if ((tid >> 5) < k) {
         const int c = tid >> 5, l32 = tid & 31;
         double *z = Z + (size_t)c * L;
+        double zr[3];
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const int i = l32 + 32 * q;
+            zr[q] = i < n ? z[i] : 0.0;
+        }
+#pragma unroll 2
         for (int j = n - 3; j >= 0; j--) {
             const double tau = tauT[j];  // tau = 0 leaves z unchanged
-            const int m = n - j - 1;
-            const double *v = Am + (j + 1) + (size_t)j * L;
-            double s = 0.0;
-            double vv[3], zz[3];
+            double vv[3], s = 0.0;
 #pragma unroll
             for (int q = 0; q < 3; q++) {
-                const int i = l32 + 32 * q, ic = i < m ? i : m - 1;
-                vv[q] = v[ic];
-                zz[q] = z[j + 1 + ic];
+                const int i = l32 + 32 * q, ic = i < n ? i : n - 1;
+                const double v = Am[ic + (size_t)j * L];
+                vv[q] = i > j && i < n ? v : 0.0;
+                s = fma(vv[q], zr[q], s);
             }
-#pragma unroll
-            for (int q = 0; q < 3; q++)
-                if (l32 + 32 * q < m) s += vv[q] * zz[q];
             s = tau * group_sum32(s);
 #pragma unroll
-            for (int q = 0; q < 3; q++) {
-                const int i = l32 + 32 * q;
-                if (i < m) z[j + 1 + i] = zz[q] - s * vv[q];
-            }
+            for (int q = 0; q < 3; q++) zr[q] = fma(-s, vv[q], zr[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const int i = l32 + 32 * q;
+            if (i < n) z[i] = zr[q];
         }
     }
     __syncthreads();

@@ -158,3 +158,23 @@ def test_rr_retries_without_p():
     assert np.all(coef[16:] == 0.0)
     ev, _ = reference(A, B, Xs, R, P[:, :0], lam)
     assert np.max(np.abs(lg - ev) / np.abs(ev)) < 1e-12
+
+
+@pytest.mark.parametrize("hint", ["exact", "above", "below", "far_below", "tiny", "nan", "huge"])
+def test_rr_eigenvalue_hints(hint):
+    """The previous Ritz values (lam_in) only seed the eigenvalue brackets in
+    the explicit-X mode: wrong, NaN or huge hints must give the same Ritz
+    values as exact ones."""
+    say(f"rr hint {hint}")
+    k = 32
+    A, B, X, R, P, lam = make_case(k, 32, 32, seed=21, bdiag=True)
+    Bas = np.hstack([X, R, P])
+    ev = sl.eigh(Bas.T @ A @ Bas, Bas.T @ B @ Bas, eigvals_only=True)[:k]
+    rng = np.random.default_rng(5)
+    hints = {"exact": ev, "above": ev * (1 + 1e-6), "below": ev * (1 - 1e-9),
+             "far_below": ev * (1 - 0.3 * rng.random(k)), "tiny": np.full(k, 1e-300),
+             "nan": np.full(k, np.nan), "huge": np.full(k, 1e300)}
+    lg, Xg, flags, _ = run_kernel(A, B, X, R, P, hints[hint], 0, explicit_x=1)
+    assert flags[0] == 0
+    assert np.max(np.abs(lg - ev) / np.abs(ev)) < 1e-11
+    np.testing.assert_allclose(Xg.T @ B @ Xg, np.eye(k), atol=1e-11)
