@@ -12,8 +12,9 @@ A step is one full solve to tolerance in MIXED mode (FP32 blocks, FP64
 Gram accumulation / small dense / finish), starting from X0 already in HBM.
 `value` is its time-to-tolerance (s, lower is better).  The same solve in
 FP64 mode is timed once after the steps, and the JSON line reports the
-ratio.  The roofline object is the SpMM kernel (the diagonal-copy kernel
-k_dia when the matrix has one, else the CSR kernel; HIP events around
+ratio.  The roofline object is the SpMM kernel the library picks for this
+matrix (the z-marching stencil kernel k_zm here; the diagonal-copy kernel
+k_dia or the CSR kernel for other matrices; HIP events around
 back-to-back launches on the library stream, k = 32 columns, FP32 -- the
 precision the mixed solve spends its iterations in).  Its `traffic` (HBM
 bytes per launch) is read from the committed rocprofv3 counter summary of
@@ -70,7 +71,9 @@ class HipEvents:
 
 
 def spmm_roofline(lib, dA, A, k, prec, reps=30):
-    """Average duration of the SpMM kernel (HipCSRMatMultiVec) over `reps` back-to-back launches."""
+    """Average duration of the SpMM kernel (HipCSRMatMultiVec) over `reps` back-to-back launches;
+    algorithmic bytes of the format the kernel reads (the z-marching and diagonal kernels read
+    the diagonal copy, the CSR kernel values, columns and row pointers)."""
     from blopex_amd import _lib as L
     from blopex_amd.lobpcg import DeviceBlock
 
@@ -263,9 +266,12 @@ def main():
     ms64, algo64 = spmm_roofline(lib, dA, A, k, L.HIPMV_FP64)
     ach = algo32 / (ms32 * 1e-3) / 1e9
     nd = dA.ptr.contents.ndiag
-    ksym = f"k_dia<float, {nd}>" if nd else "k_spmm<float, 8>"
-    kname = ksym + (f" (SpMM on the diagonal copy of the CSR matrix, k={k}, FP32)" if nd
-                    else f" (CSR SpMM, k={k}, FP32)")
+    path = lib.hip_CSRMatrixProductKernel(dA.ptr, L.HIPMV_FP32, k, A.n)
+    ksym = {2: f"k_zm<float, 32, {dA.ptr.contents.grid[0]}>", 1: f"k_dia<float, {nd}>"}.get(
+        path, "k_spmm<float, 8>")
+    kname = ksym + {2: f" (z-marching 7-point stencil SpMM on the diagonal copy, k={k}, FP32)",
+                    1: f" (SpMM on the diagonal copy of the CSR matrix, k={k}, FP32)"}.get(
+        path, f" (CSR SpMM, k={k}, FP32)")
     from blopex_amd.profsum import read_traffic
     traffic = read_traffic(COUNTERS, ksym) if (nside, k) == (128, 32) else None
     out["roofline"] = {"kernel": kname, "bound": "hbm",

@@ -55,7 +55,7 @@ class CSRMatrix(ctypes.Structure):
     _fields_ = [("n", c_long), ("nnz", c_long), ("rowptr", c_void_p), ("colind", c_void_p),
                 ("val", c_void_p), ("val32", c_void_p), ("ndiag", c_int),
                 ("diag_offset", c_int * 16), ("diag_ld", c_long), ("diag_val", c_void_p),
-                ("diag_val32", c_void_p)]
+                ("diag_val32", c_void_p), ("grid", c_int * 3)]
 
 
 class DiagMatrix(ctypes.Structure):
@@ -92,6 +92,7 @@ _SIGS = {
     "HipSetupInterpreter": (c_int, [P(Interpreter)]),
     "hip_CSRMatrixCreate": (P(CSRMatrix), [c_long, c_long, c_void_p, c_void_p, c_void_p]),
     "hip_CSRMatrixDestroy": (None, [P(CSRMatrix)]),
+    "hip_CSRMatrixProductKernel": (c_int, [P(CSRMatrix), c_int, c_int, c_long]),
     "hip_DiagMatrixCreate": (P(DiagMatrix), [c_long, c_void_p]),
     "hip_DiagMatrixCreateJacobi": (P(DiagMatrix), [P(CSRMatrix)]),
     "hip_DiagMatrixDestroy": (None, [P(DiagMatrix)]),

@@ -3,6 +3,7 @@
 // diagonal of the Gram, block times small matrix) and precision conversion.
 // Reductions are two-stage (per-block FP64 partials, then a fixed-order sum),
 // so every result is reproducible run to run.
+#include "dpp.h"
 #include "kernels.h"
 
 namespace bx {
@@ -173,12 +174,15 @@ __global__ void __launch_bounds__(kThreads) k_diagdot(long n, const T *x, long l
     }
 }
 
-__global__ void k_rowsum(const double *part, int nparts, int count, double *out) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= count) return;
+// out[j] = sum of the nparts partials of column j: one wave per column, lane
+// l adds partials l, l + 64, ... and the lanes are summed in a fixed order
+__global__ void __launch_bounds__(64) k_rowsum(const double *part, int nparts, double *out) {
+    const int j = blockIdx.x, l = threadIdx.x;
+    const double *pj = part + (size_t)j * nparts;
     double s = 0.0;
-    for (int p = 0; p < nparts; p++) s += part[(size_t)j * nparts + p];
-    out[j] = s;
+    for (int p = l; p < nparts; p += 64) s += pj[p];
+    s = wave_sum(s);
+    if (l == 0) out[j] = s;
 }
 
 // residual, its squared norm partials (the k_diagdot partition and order)
@@ -406,8 +410,7 @@ void diagdot_cols(int prec, long n, const void *x, long ldx, const ColMap &xm, c
         hipLaunchKernelGGL(k_diagdot<double>, grid, dim3(kThreads), 0, stream(), n,
                            (const double *)x, ldx, xm, (const double *)y, ldy, ym, part);
     BX_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_rowsum, dim3((xm.n + 63) / 64), dim3(64), 0, stream(), part, (int)nparts,
-                       xm.n, d);
+    hipLaunchKernelGGL(k_rowsum, dim3(xm.n), dim3(64), 0, stream(), part, (int)nparts, d);
     BX_HIP(hipGetLastError());
 }
 
@@ -442,8 +445,7 @@ void residual_norm_cols(int prec, long n, const void *bx, const void *ax, long l
                            (const double *)bx, (const double *)ax, ld, act, lam, (double *)r, ldr,
                            (const double *)t, part);
     BX_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_rowsum, dim3((act.n + 63) / 64), dim3(64), 0, stream(), part, (int)nparts,
-                       act.n, d);
+    hipLaunchKernelGGL(k_rowsum, dim3(act.n), dim3(64), 0, stream(), part, (int)nparts, d);
     BX_HIP(hipGetLastError());
 }
 

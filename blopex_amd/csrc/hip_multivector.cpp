@@ -323,6 +323,30 @@ int blopex_amd_is_hip_interpreter(const mv_InterfaceInterpreter *i) {
 // Diagonal copy of A for the gather-free product (see hip_CSRMatrix): offsets
 // c - r of all nonzeros; at most HIP_CSR_MAX_DIAGONALS of them, at least half
 // of the diagonal slots filled, sorted distinct columns in every row.
+// A 7-point stencil on an nx x ny x nz grid (x fastest): diagonals at
+// -nx*ny, -nx, -1, 0, 1, nx, nx*ny with zero coefficients wherever the
+// neighbour lies outside the grid (the -1 entry of an x = 0 row would
+// otherwise couple to the end of the previous line).  dv: ndiag x diag_ld.
+static void find_grid(hip_CSRMatrix *A, const double *dv) {
+    A->grid[0] = A->grid[1] = A->grid[2] = 0;
+    if (A->ndiag != 7) return;
+    const int *o = A->diag_offset;
+    const long lx = o[5], pl = o[6], n = A->n;
+    if (o[2] != -1 || o[3] != 0 || o[4] != 1) return;
+    if (o[1] != -lx || o[0] != -pl || o[6] != pl || lx < 2 || pl % lx || pl / lx < 2 || n % pl)
+        return;
+    const long ny = pl / lx, nz = n / pl, ld = A->diag_ld;
+    for (long r = 0; r < n; r++) {
+        const long x = r % lx, y = (r / lx) % ny;
+        if ((x == 0 && dv[2 * ld + r] != 0.0) || (x == lx - 1 && dv[4 * ld + r] != 0.0) ||
+            (y == 0 && dv[1 * ld + r] != 0.0) || (y == ny - 1 && dv[5 * ld + r] != 0.0))
+            return;
+    }
+    A->grid[0] = (int)lx;
+    A->grid[1] = (int)ny;
+    A->grid[2] = (int)nz;
+}
+
 static void build_diagonals(hip_CSRMatrix *A, const int *rowptr, const int *colind,
                             const double *val) {
     const long n = A->n, nnz = A->nnz;
@@ -353,6 +377,7 @@ static void build_diagonals(hip_CSRMatrix *A, const int *rowptr, const int *coli
     A->ndiag = nd;
     for (int d = 0; d < nd; d++) A->diag_offset[d] = (int)offs[d];
     A->diag_ld = ld;
+    find_grid(A, dv.data());
     BX_HIP(hipMalloc(&A->diag_val, sizeof(double) * nd * ld));
     BX_HIP(hipMalloc(&A->diag_val32, sizeof(float) * nd * ld));
     BX_HIP(hipMemcpy(A->diag_val, dv.data(), sizeof(double) * nd * ld, hipMemcpyHostToDevice));
@@ -398,6 +423,12 @@ void hip_CSRMatrixDestroy(hip_CSRMatrix *A) {
         BX_HIP(hipFree(A->diag_val32));
     }
     free(A);
+}
+
+int hip_CSRMatrixProductKernel(const hip_CSRMatrix *A, int precision, BlopexInt ncols, long ld) {
+    BX_REQUIRE(A && (precision == HIPMV_FP64 || precision == HIPMV_FP32) && ld >= A->n,
+               "hip_CSRMatrixProductKernel: bad arguments");
+    return spmm_kernel(precision, A, ld, (int)ncols);
 }
 
 void HipCSRMatMultiVec(void *A_, void *X_, void *Y_) {

@@ -61,9 +61,12 @@ struct UpdGroup {
     void *out_full;
 };
 // Up to 3 groups in one launch; C device FP64 (crows x w, leading dim ldc);
-// a no-op when st->abort is set (st may be nullptr).
+// a no-op when st->abort is set (st may be nullptr).  tail_pos (host, w
+// entries, nullptr = identity): column of out_tail that receives output
+// column c, -1 to drop it (the tail can be compacted to the active columns
+// as it is written).
 void block_update(int prec, long ld, const UpdGroup *groups, int ngroups, const double *C, int ldc,
-                  int crows, int w, const DevStatus *st);
+                  int crows, int w, const DevStatus *st, const int *tail_pos = nullptr);
 
 // Fused Gram products G = L' Q: every block of a plan is staged through LDS
 // a chunk of rows at a time, and each wave accumulates a register block of
@@ -113,6 +116,9 @@ class GramBatch {
     GramPlan p_;
 };
 
+// kernel spmm_csr uses for ncols contiguous columns with leading dimension
+// ld: 0 CSR, 1 diagonal copy, 2 z-marching stencil kernel
+int spmm_kernel(int prec, const hip_CSRMatrix *A, long ld, int ncols);
 // CSR SpMM Y(:, ym) = A X(:, xm)
 void spmm_csr(int prec, const hip_CSRMatrix *A, const void *x, long ldx, const ColMap &xm, void *y,
               long ldy, const ColMap &ym);

@@ -230,9 +230,10 @@ class Phase {
         gb.flush();
         tm_->end();
     }
-    void update(const UpdGroup *g, int ng, const double *C, int crows, int w) {
+    void update(const UpdGroup *g, int ng, const double *C, int crows, int w,
+                const int *tail_pos = nullptr) {
         tm_->begin(kUpdate);
-        block_update(prec_, ld_, g, ng, C, ld3_, crows, w, st_);
+        block_update(prec_, ld_, g, ng, C, ld3_, crows, w, st_, tail_pos);
         tm_->end();
     }
     // x <- x * c (in place), c a w x w block of the Gram buffer
@@ -456,23 +457,22 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
             }
         }
         if (nr < 1) break;
-        // compact R (laid out by prev_act) and P/AP/BP (rebuilt at full width k
-        // by every update) to the active columns
-        if (act != prev_act || (local_it > 0 && nr < k)) {
+        // R, P, AP and BP are laid out by prev_act (the update writes P' only
+        // for the columns active when it ran); compact them when columns
+        // have converged since (the active set only shrinks)
+        if (act != prev_act) {
             tm_->begin(kOther);
-            if (act != prev_act) {
-                std::vector<int> src;
-                size_t q = 0;
-                for (int c : act) {
-                    while (prev_act[q] != c) q++;
-                    src.push_back((int)q);
-                }
-                gather(R_, src);
+            std::vector<int> src;
+            size_t q = 0;
+            for (int c : act) {
+                while (prev_act[q] != c) q++;
+                src.push_back((int)q);
             }
-            if (local_it > 0 && nr < k) {
-                gather(P_, act);
-                gather(AP_, act);
-                if (!noB_) gather(BP_, act);
+            gather(R_, src);
+            if (local_it > 0) {
+                gather(P_, src);
+                gather(AP_, src);
+                if (!noB_) gather(BP_, src);
             }
             tm_->end();
         }
@@ -536,7 +536,9 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
             g[ng].out_full = trip[q][0];
             ng++;
         }
-        update(g, ng, coef_, k + nr + preq, k);
+        std::vector<int> tail_pos(k, -1);  // P' kept for the active columns only
+        for (int t = 0; t < nr; t++) tail_pos[act[t]] = t;
+        update(g, ng, coef_, k + nr + preq, k, tail_pos.data());
         BXp = noB_ ? X_ : BX_;
         ColMap am = colmap_identity(nr);
         for (int t = 0; t < nr; t++) am.idx[t] = act[t];

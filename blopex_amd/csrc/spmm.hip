@@ -10,6 +10,19 @@
 // is the column order the CSR kernel sums in, and absent entries are zeros,
 // so both kernels give the same bits.
 //
+// 7-point stencils whose diagonal copy is laid out on an nx x ny x nz grid
+// (hip_CSRMatrix::grid, nx = 64, 128 or 256) march along z instead: a
+// 512-thread workgroup owns 512 / nx full x-lines, one row per thread, of a
+// chunk of consecutive planes.  X of planes z-1, z, z+1 at the thread's own
+// row stays in registers, so each X value comes from HBM once per chunk
+// instead of once for each of the three planes that use it; the x and y
+// neighbours come from the plane-z slab of the tile in LDS, whose two halo
+// lines are fetched together with X(z+1), when the workgroup of the
+// neighbouring tile -- on the same XCD, at the same plane -- reads them as
+// its own rows.  The sum runs over the same offsets in the same order, and
+// neighbours outside the grid (zero coefficients) enter as zeros, so the
+// bits equal those of the diagonal and CSR kernels.
+//
 // General CSR: one thread per row.  A row's (column, value) pairs are loaded once into
 // registers and reused for every block column, so the matrix streams from
 // HBM once per product; for the stencil matrices the X reads of neighbouring
@@ -134,6 +147,94 @@ __global__ void __launch_bounds__(kSpThreads) k_dia(long n, DiaOffsets off,
     }
 }
 
+constexpr int kZT = 512;  // threads of a z-marching workgroup
+
+template <typename T, int KC>
+struct ZmArgs {
+    const T *dval;  // 7 x dld: offsets -P, -nx, -1, 0, 1, nx, P
+    const T *x;     // first column of the chunk
+    T *y;
+    unsigned dld, ld;  // leading dimensions of dval and of x and y (elements)
+    int ny, nz, zc, ntiles, nc, xcd;
+};
+
+// Element offsets are 32-bit (the host checks that they fit), so loads take
+// one scalar base and a 32-bit vector offset.
+template <typename T, int KC, int NXT>
+__global__ void __launch_bounds__(kZT) k_zm(const ZmArgs<T, KC> a) {
+    constexpr int TY = kZT / NXT;            // x-lines per workgroup
+    constexpr int HPT = 2 * NXT * KC / kZT;  // halo values per thread
+    __shared__ T slab[(TY + 2) * KC * NXT];  // [line -1..TY][column][x]
+    const int t = threadIdx.x, xq = t % NXT, ly = t / NXT;
+    const long lin = a.xcd ? xcd_block(blockIdx.x, gridDim.x) : (long)blockIdx.x;
+    const int zc = (int)(lin / a.ntiles), tile = (int)(lin % a.ntiles);
+    const int y0 = tile * TY, z0 = zc * a.zc, yy = y0 + ly;
+    const unsigned plane = (unsigned)NXT * (unsigned)a.ny, ld = a.ld;
+    auto row = [&](int z, int yv, int xv) {
+        return (unsigned)z * plane + (unsigned)yv * NXT + (unsigned)xv;
+    };
+    T xm1[KC], x0v[KC], xp1[KC], hal[HPT];
+#pragma unroll
+    for (int j = 0; j < KC; j++) {
+        xm1[j] = (j < a.nc && z0 > 0) ? a.x[j * ld + row(z0 - 1, yy, xq)] : T(0);
+        x0v[j] = j < a.nc ? a.x[j * ld + row(z0, yy, xq)] : T(0);
+    }
+    // halo element e = t + kZT i: x = e % NXT, column (e / NXT) % KC, line
+    // y0 - 1 for e < NXT KC, y0 + TY after
+    auto load_halo = [&](int z) {
+#pragma unroll
+        for (int i = 0; i < HPT; i++) {
+            const int e = t + kZT * i, xv = e % NXT, j = (e / NXT) % KC;
+            const int yv = e < NXT * KC ? y0 - 1 : y0 + TY;
+            const bool ok = j < a.nc && yv >= 0 && yv < a.ny;
+            hal[i] = ok ? a.x[(unsigned)j * ld + row(z, yv, xv)] : T(0);
+        }
+    };
+    load_halo(z0);
+    for (int z = z0; z < z0 + a.zc; z++) {
+#pragma unroll
+        for (int j = 0; j < KC; j++) slab[((ly + 1) * KC + j) * NXT + xq] = x0v[j];
+#pragma unroll
+        for (int i = 0; i < HPT; i++) {
+            const int e = t + kZT * i, xv = e % NXT, j = (e / NXT) % KC;
+            slab[((e < NXT * KC ? 0 : TY + 1) * KC + j) * NXT + xv] = hal[i];
+        }
+        const bool up = z + 1 < a.nz;
+#pragma unroll
+        for (int j = 0; j < KC; j++)
+            xp1[j] = (j < a.nc && up) ? a.x[j * ld + row(z + 1, yy, xq)] : T(0);
+        if (up) load_halo(z + 1);
+        const unsigned r = row(z, yy, xq);
+        T dv[7];
+#pragma unroll
+        for (int d = 0; d < 7; d++) dv[d] = a.dval[d * a.dld + r];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < KC; j++) {
+            if (j >= a.nc) continue;
+            const T *sl = slab + j * NXT;
+            const T ym = sl[ly * KC * NXT + xq];
+            const T xmn = xq > 0 ? sl[(ly + 1) * KC * NXT + xq - 1] : T(0);
+            const T xpn = xq + 1 < NXT ? sl[(ly + 1) * KC * NXT + xq + 1] : T(0);
+            const T yp = sl[(ly + 2) * KC * NXT + xq];
+            T s = fma(dv[0], xm1[j], T(0));
+            s = fma(dv[1], ym, s);
+            s = fma(dv[2], xmn, s);
+            s = fma(dv[3], x0v[j], s);
+            s = fma(dv[4], xpn, s);
+            s = fma(dv[5], yp, s);
+            s = fma(dv[6], xp1[j], s);
+            a.y[j * ld + r] = s;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < KC; j++) {
+            xm1[j] = x0v[j];
+            x0v[j] = xp1[j];
+        }
+    }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kSpThreads) k_diag(long n, const T *__restrict__ d, const T *x,
                                                      long ldx, ColMap xm, T *y, long ldy, ColMap ym) {
@@ -149,12 +250,92 @@ unsigned xcd_grid(long n) {
     return (nb + kXcds - 1) / kXcds * kXcds;
 }
 
+// planes per z-marching chunk for this grid, or 0 when the diagonal kernel
+// is used: about one workgroup per CU, chunks of at least min_zc planes
+// (BLOPEX_AMD_ZM_MIN_ZC, default 4)
+int zm_chunk(const hip_CSRMatrix *A) {
+    const int nx = A->grid[0], ny = A->grid[1], nz = A->grid[2];
+    if (A->ndiag != 7 || nx == 0 || (nx != 64 && nx != 128 && nx != 256)) return 0;
+    const int ty = kZT / nx;
+    if (ny % ty) return 0;
+    const long ntiles = ny / ty;
+    int min_zc = 4;
+    if (const char *e = getenv("BLOPEX_AMD_ZM_MIN_ZC")) min_zc = atoi(e);
+    int zc = (int)(ntiles * nz / num_cus());
+    if (zc < 1) zc = 1;
+    while (nz % zc) zc--;
+    return zc >= min_zc ? zc : 0;
+}
+
+// the z-marching kernel takes contiguous column ranges
+bool contiguous(const ColMap &m) {
+    for (int j = 1; j < m.n; j++)
+        if (m.idx[j] != m.idx[0] + j) return false;
+    return true;
+}
+
+template <typename T, int KC, int NXT>
+void launch_zm(const hip_CSRMatrix *A, int zc, const T *dv, const T *x, long ldx, const ColMap &xm,
+               T *y, long ldy, const ColMap &ym) {
+    ZmArgs<T, KC> a{};
+    a.dval = dv;
+    a.dld = (unsigned)A->diag_ld;
+    a.ld = (unsigned)ldx;
+    a.ny = A->grid[1];
+    a.nz = A->grid[2];
+    a.zc = zc;
+    a.ntiles = A->grid[1] / (kZT / NXT);
+    const long nb = (long)a.ntiles * (a.nz / zc);
+    a.xcd = nb % kXcds == 0;
+    for (int c0 = 0; c0 < xm.n; c0 += KC) {
+        a.nc = xm.n - c0 < KC ? xm.n - c0 : KC;
+        a.x = x + (size_t)(xm.idx[0] + c0) * ldx;
+        a.y = y + (size_t)(ym.idx[0] + c0) * ldy;
+        hipLaunchKernelGGL((k_zm<T, KC, NXT>), dim3((unsigned)nb), dim3(kZT), 0, stream(), a);
+    }
+}
+
+template <typename T, int KC>
+void spmm_zm(const hip_CSRMatrix *A, int zc, const T *dv, const T *x, long ldx, const ColMap &xm,
+             T *y, long ldy, const ColMap &ym) {
+    switch (A->grid[0]) {
+        case 64: launch_zm<T, KC, 64>(A, zc, dv, x, ldx, xm, y, ldy, ym); break;
+        case 128: launch_zm<T, KC, 128>(A, zc, dv, x, ldx, xm, y, ldy, ym); break;
+        default: launch_zm<T, KC, 256>(A, zc, dv, x, ldx, xm, y, ldy, ym); break;
+    }
+}
+
 }  // namespace
+
+int spmm_zm_chunk(int prec, const hip_CSRMatrix *A, long ld, int ncols) {
+    // one leading dimension and 32-bit element offsets within a chunk of KC
+    // columns (32 FP32, 16 FP64)
+    const long kc = prec == HIPMV_FP32 ? 32 : 16;
+    if (7 * A->diag_ld >= (1L << 31) || (ncols < kc ? ncols : kc) * ld >= (1L << 31)) return 0;
+    return zm_chunk(A);
+}
+
+int spmm_kernel(int prec, const hip_CSRMatrix *A, long ld, int ncols) {
+    if (spmm_zm_chunk(prec, A, ld, ncols)) return 2;
+    return A->ndiag > 0 ? 1 : 0;
+}
 
 void spmm_csr(int prec, const hip_CSRMatrix *A, const void *x, long ldx, const ColMap &xm, void *y,
               long ldy, const ColMap &ym) {
     BX_REQUIRE(xm.n == ym.n, "spmm: active column counts differ (%d vs %d)", xm.n, ym.n);
     if (xm.n == 0 || A->n == 0) return;
+    const int zc = ldx == ldy && contiguous(xm) && contiguous(ym)
+                       ? spmm_zm_chunk(prec, A, ldx, xm.n)
+                       : 0;
+    if (zc) {
+        if (prec == HIPMV_FP32)
+            spmm_zm<float, 32>(A, zc, A->diag_val32, (const float *)x, ldx, xm, (float *)y, ldy, ym);
+        else
+            spmm_zm<double, 16>(A, zc, A->diag_val, (const double *)x, ldx, xm, (double *)y, ldy,
+                                ym);
+        BX_HIP(hipGetLastError());
+        return;
+    }
     if (A->ndiag > 0) {
         DiaOffsets off;
         for (int d = 0; d < HIP_CSR_MAX_DIAGONALS; d++) off.o[d] = d < A->ndiag ? A->diag_offset[d] : 0;

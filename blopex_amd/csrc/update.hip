@@ -32,6 +32,7 @@ struct UpdKArgs {
     int w;
     long ld;
     const DevStatus *st;
+    signed char tail_pos[64];  // out_tail column of each output column (-1: not stored)
 };
 
 template <typename T, int NOT>
@@ -52,16 +53,18 @@ __device__ __forceinline__ void upd_block(const T *in, int h, int crow, const T 
     }
 }
 
+// output column c of the lane's accumulators goes to column pos[o][r] of out
+// (skipped when negative)
 template <typename T, int NOT>
-__device__ __forceinline__ void upd_store(T *out, int w, long ld, long r0, int lk,
+__device__ __forceinline__ void upd_store(T *out, long ld, long r0, const int (&pos)[NOT][4],
                                           const typename Mfma<T>::acc (&acc)[NOT][4]) {
 #pragma unroll
     for (int o = 0; o < NOT; o++)
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const int c = o * 16 + Mfma<T>::drow(lk, r);
-            if (c < w) store4(out + (size_t)c * ld + r0, acc[o][0][r], acc[o][1][r], acc[o][2][r],
-                              acc[o][3][r]);
+            const int c = pos[o][r];
+            if (c >= 0) store4(out + (size_t)c * ld + r0, acc[o][0][r], acc[o][1][r], acc[o][2][r],
+                               acc[o][3][r]);
         }
 }
 
@@ -81,6 +84,15 @@ __global__ void __launch_bounds__(kUT) k_update(UpdKArgs a) {
     const int li = lane & 15, lk = lane >> 4;
     const long nchunks = a.ld / 64;
     const T *in0 = (const T *)g.in[0], *in1 = (const T *)g.in[1], *in2 = (const T *)g.in[2];
+    int pfull[NOT][4], ptail[NOT][4];
+#pragma unroll
+    for (int o = 0; o < NOT; o++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int c = o * 16 + Mfma<T>::drow(lk, r);
+            pfull[o][r] = c < a.w ? c : -1;
+            ptail[o][r] = c < a.w ? (int)a.tail_pos[c] : -1;
+        }
     for (long ch = (long)blockIdx.x * 4 + wave; ch < nchunks; ch += (long)gridDim.x * 4) {
         const long r0 = ch * 64 + 4 * li;
         typename Mfma<T>::acc acc[NOT][4];
@@ -90,9 +102,9 @@ __global__ void __launch_bounds__(kUT) k_update(UpdKArgs a) {
             for (int t = 0; t < 4; t++) acc[o][t] = typename Mfma<T>::acc{};
         if (in1) upd_block<T, NOT>(in1, g.h[1], g.crow[1], Cl, a.ld, r0, li, lk, acc);
         if (in2) upd_block<T, NOT>(in2, g.h[2], g.crow[2], Cl, a.ld, r0, li, lk, acc);
-        if (g.out_tail) upd_store<T, NOT>((T *)g.out_tail, a.w, a.ld, r0, lk, acc);
+        if (g.out_tail) upd_store<T, NOT>((T *)g.out_tail, a.ld, r0, ptail, acc);
         if (in0) upd_block<T, NOT>(in0, g.h[0], g.crow[0], Cl, a.ld, r0, li, lk, acc);
-        if (g.out_full) upd_store<T, NOT>((T *)g.out_full, a.w, a.ld, r0, lk, acc);
+        if (g.out_full) upd_store<T, NOT>((T *)g.out_full, a.ld, r0, pfull, acc);
     }
 }
 
@@ -145,7 +157,7 @@ void mfma_layout_probe(int prec, int *rows, int *cols) {
 }
 
 void block_update(int prec, long ld, const UpdGroup *groups, int ngroups, const double *C, int ldc,
-                  int crows, int w, const DevStatus *st) {
+                  int crows, int w, const DevStatus *st, const int *tail_pos) {
     BX_REQUIRE(ld % 64 == 0, "block_update: leading dimension %ld not a multiple of 64", ld);
     BX_REQUIRE(ngroups >= 1 && ngroups <= 3, "block_update: %d groups", ngroups);
     BX_REQUIRE(w >= 1 && w <= 64, "block_update: %d output columns (max 64)", w);
@@ -165,6 +177,11 @@ void block_update(int prec, long ld, const UpdGroup *groups, int ngroups, const 
     a.w = w;
     a.ld = ld;
     a.st = st;
+    for (int c = 0; c < 64; c++) {
+        const int q = c < w ? (tail_pos ? tail_pos[c] : c) : -1;
+        BX_REQUIRE(q >= -1 && q < w, "block_update: tail position %d of column %d", q, c);
+        a.tail_pos[c] = (signed char)q;
+    }
     const int NOT = (w + 15) / 16;
     if (prec == HIPMV_FP32) {
         if (NOT == 1) launch_update<float, 1>(a, ngroups);

@@ -130,6 +130,10 @@ typedef struct hip_CSRMatrix {
     long diag_ld;                           /* >= n, multiple of 4 */
     double *diag_val;                       /* device, ndiag x diag_ld, 0 off the matrix */
     float *diag_val32;
+    /* nx, ny, nz when the diagonals are a 7-point stencil on that grid (x
+       fastest; offsets -nx*ny, -nx, -1, 0, 1, nx, nx*ny, zero coefficients
+       for neighbours outside the grid); 0 otherwise */
+    int grid[3];
 } hip_CSRMatrix;
 
 typedef struct hip_DiagMatrix {
@@ -144,6 +148,10 @@ hip_CSRMatrix *hip_CSRMatrixCreate(long n, long nnz, const int *rowptr, const in
 void hip_CSRMatrixDestroy(hip_CSRMatrix *A);
 /* operatorA / operatorB callback: Y(active) = A * X(active) */
 void HipCSRMatMultiVec(void *A, void *X, void *Y);
+/* the kernel HipCSRMatMultiVec runs for ncols contiguous active columns of
+   blocks with leading dimension ld: 0 CSR, 1 diagonal copy, 2 z-marching
+   7-point stencil kernel (no counterpart in the reference) */
+int hip_CSRMatrixProductKernel(const hip_CSRMatrix *A, int precision, BlopexInt ncols, long ld);
 
 hip_DiagMatrix *hip_DiagMatrixCreate(long n, const double *d);
 /* Jacobi preconditioner T = diag(A)^{-1} */

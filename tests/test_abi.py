@@ -55,8 +55,10 @@ def test_struct_layouts():
     assert L.HipMultiVector.settings.offset == 48
     assert ctypes.sizeof(L.Settings) == 32 and ctypes.sizeof(L.Stats) == 80
     assert ctypes.sizeof(L.HipMultiVector) == 48 + 32 + 80
-    # hip_CSRMatrix: 6 words, ndiag + 16 offsets (68 bytes, padded to 72), diag_ld, 2 pointers
-    assert L.CSRMatrix.diag_ld.offset == 48 + 72 and ctypes.sizeof(L.CSRMatrix) == 48 + 72 + 24
+    # hip_CSRMatrix: 6 words, ndiag + 16 offsets (68 bytes, padded to 72), diag_ld, 2 pointers,
+    # grid[3] (padded to 16)
+    assert L.CSRMatrix.diag_ld.offset == 48 + 72 and L.CSRMatrix.grid.offset == 48 + 72 + 24
+    assert ctypes.sizeof(L.CSRMatrix) == 48 + 72 + 24 + 16
 
 
 def test_fortran_matrix_host_routines(lib):

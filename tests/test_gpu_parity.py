@@ -95,6 +95,87 @@ def test_spmm_diagonal_copy_matches_csr_kernel(prec):
         o.close()
 
 
+def _products(dA, X, prec, paths):
+    """A X through HipCSRMatMultiVec once per kernel setting in paths: "csr"
+    (ndiag cleared), "dia" (grid cleared) or "zm" (as built)."""
+    lib = L.load()
+    lib.HipCSRMatMultiVec.argtypes = [ctypes.c_void_p] * 3
+    c = dA.ptr.contents
+    nd, grid = c.ndiag, list(c.grid)
+    dX = bl.DeviceBlock.from_numpy(X, prec)
+    out = []
+    for path in paths:
+        c.ndiag = 0 if path == "csr" else nd
+        for i in range(3):
+            c.grid[i] = 0 if path != "zm" else grid[i]
+        dY = bl.DeviceBlock(X.shape[0], X.shape[1], prec)
+        lib.HipCSRMatMultiVec(ctypes.cast(dA.ptr, ctypes.c_void_p),
+                              ctypes.cast(dX.ptr, ctypes.c_void_p),
+                              ctypes.cast(dY.ptr, ctypes.c_void_p))
+        out.append(dY.to_numpy())
+        dY.close()
+    c.ndiag = nd
+    for i in range(3):
+        c.grid[i] = grid[i]
+    dX.close()
+    return out
+
+
+@pytest.mark.parametrize("prec", [L.HIPMV_FP64, L.HIPMV_FP32])
+@pytest.mark.parametrize("shape,k", [((64, 16, 24), 7), ((128, 8, 12), 40), ((256, 4, 8), 17)])
+def test_spmm_stencil_kernel_matches_csr_kernel(prec, shape, k, monkeypatch):
+    """The z-marching kernel of 7-point stencils (x lines of 64, 128 and 256
+    rows; several column chunks; one-plane chunks allowed here) gives the
+    same bits as the diagonal and CSR kernels."""
+    say(f"spmm z-marching vs CSR kernel {shape} k={k} prec={prec}")
+    monkeypatch.setenv("BLOPEX_AMD_ZM_MIN_ZC", "1")
+    A = pb.laplacian_7pt(*shape, sigma=1.0, seed=8)
+    X = np.random.default_rng(3).standard_normal((A.n, k))
+    dA = bl.DeviceCSR(A)
+    assert list(dA.ptr.contents.grid) == list(shape)
+    lib = L.load()
+    assert lib.hip_CSRMatrixProductKernel(dA.ptr, prec, k, A.n) == 2
+    zm, dia, csr = _products(dA, X, prec, ["zm", "dia", "csr"])
+    np.testing.assert_array_equal(zm, csr)
+    np.testing.assert_array_equal(dia, csr)
+    ref = po.spmm(A, X)
+    scale = np.abs(A.to_scipy()) @ np.abs(X)
+    assert np.max(np.abs(zm - ref) / scale) < (1e-13 if prec == L.HIPMV_FP64 else 1e-5)
+    dA.close()
+
+
+def test_spmm_stencil_kernel_at_128_cubed():
+    """At 128^3 the default settings pick the z-marching kernel for 32 FP32
+    columns; its product equals the CSR kernel's bit for bit."""
+    say("spmm z-marching 128^3 k=32 FP32")
+    A = pb.laplacian_7pt(128, 128, 128, sigma=1.0, seed=42)
+    dA = bl.DeviceCSR(A)
+    lib = L.load()
+    assert lib.hip_CSRMatrixProductKernel(dA.ptr, L.HIPMV_FP32, 32, A.n) == 2
+    X = np.random.default_rng(4).standard_normal((A.n, 32)).astype(np.float32).astype(np.float64)
+    zm, csr = _products(dA, X, L.HIPMV_FP32, ["zm", "csr"])
+    np.testing.assert_array_equal(zm, csr)
+    dA.close()
+
+
+def test_spmm_stencil_grid_detection():
+    """Only a true grid stencil gets grid dimensions: a coupling across the end
+    of an x line (same diagonals) turns the z-marching kernel off."""
+    say("spmm grid detection")
+    A = pb.laplacian_7pt(64, 4, 4, sigma=0.0, seed=1)
+    dA = bl.DeviceCSR(A)
+    assert list(dA.ptr.contents.grid) == [64, 4, 4]
+    dA.close()
+    S = A.to_scipy().tolil()
+    S[64, 63] = S[63, 64] = -0.5  # x = 0 of line 1 coupled to x = 63 of line 0
+    S = S.tocsr()
+    B = pb.CSR(S.indptr, S.indices, S.data)
+    dB = bl.DeviceCSR(B)
+    assert dB.ptr.contents.ndiag == 7 and list(dB.ptr.contents.grid) == [0, 0, 0]
+    assert L.load().hip_CSRMatrixProductKernel(dB.ptr, L.HIPMV_FP32, 8, B.n) == 1
+    dB.close()
+
+
 def _wrap(ii, blk):
     return L.load().mv_MultiVectorWrap(ctypes.byref(ii), ctypes.cast(blk.ptr, ctypes.c_void_p), 0)
 
