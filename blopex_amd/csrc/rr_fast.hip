@@ -12,7 +12,7 @@
 // replace dsygv itype 1 'V' 'L' (lobpcg.c:66-103):
 //   B_n = U'U (block Cholesky: the identity X block is skipped and its
 //   rank-k update done on MFMA), W = U^{-1}, C = W' A_n W (MFMA),
-//   C = Q T Q' (Householder, two barriers per column),
+//   C = Q T Q' (Householder with C in registers, two barriers per column),
 //   eigenvalues of T by multisection on the three-term Sturm recurrence,
 //   eigenvectors by inverse iteration (stored L D L' pivots), re-orthogonalised
 //   inside eigenvalue clusters (relative gap 1e-3, as LAPACK dstein),
@@ -567,120 +567,128 @@ __global__ void __launch_bounds__(kT) k_rr_tri(RR2Args r) {
     // ---- Householder tridiagonalisation of C, held in registers ----
     // Waves 0-7 keep C: lane (hc = lane % 16, hr = lane / 16) of wave w owns
     // rows w + 8 (hr + 4 q), q < 3, and columns hc + 16 b, b < 6, so a row's
-    // 96 columns lie in 16 lanes of one wave.  Step j: x = C[j+1:, j] (LDS),
-    // H_j = I - tau v v' with v = x - alpha e_1; p = tau C v (row sums in the
-    // wave), K = tau v'p / 2, w = p - K v, C -= v w' + w v'.  v goes to
-    // column j of Am below the diagonal, the next column to LDS.  Rows and
-    // columns outside j+1..n-1 have v = w = 0 and do not change.
+    // 96 columns lie in 16 lanes of one wave.  Step j: x = C[j+1:, j], kept in
+    // LDS by absolute row with zeros above (so every lane reads its rows and
+    // columns at fixed addresses); H_j = I - tau v v' with v = x - alpha e_{j+1};
+    // p = tau C v (row sums in the wave), K = tau v'p / 2, w = p - K v,
+    // C -= v w' + w v'.  v goes to column j of Am, the next column and the
+    // next diagonal entry to LDS.  Rows and columns up to j have v = w = 0 and
+    // do not change.
     {
         constexpr int kHQ = 3, kHB = 6, kHW = 8;
         const bool hw = wave < kHW;
         const int hc = lane & 15, hr = lane >> 4;
-        auto hrow = [&](int q) { return wave + kHW * (hr + 4 * q); };
-        auto hcol = [&](int b) { return hc + 16 * b; };
+        int hrow[kHQ], hcol[kHB];
+#pragma unroll
+        for (int q = 0; q < kHQ; q++) hrow[q] = wave + kHW * (hr + 4 * q);
+#pragma unroll
+        for (int b = 0; b < kHB; b++) hcol[b] = hc + 16 * b;
         double M[kHQ][kHB];
 #pragma unroll
         for (int q = 0; q < kHQ; q++)
 #pragma unroll
-            for (int b = 0; b < kHB; b++) {
-                const int i = hrow(q), c = hcol(b);
-                M[q][b] = (hw && i < n && c < n) ? Am[i + c * L] : 0.0;
-            }
-        for (int i = tid; i < n - 1; i += kT) col0[i] = Am[1 + i];
+            for (int b = 0; b < kHB; b++)
+                M[q][b] = (hw && hrow[q] < n && hcol[b] < n) ? Am[hrow[q] + hcol[b] * L] : 0.0;
+        // the owner lanes of column c pick its entries out of their registers
+        // (c is uniform: one scalar branch)
+#define BX_HH_COLUMN(c, cv)                                               \
+    switch (__builtin_amdgcn_readfirstlane((c) >> 4)) {                   \
+        case 0: cv[0] = M[0][0]; cv[1] = M[1][0]; cv[2] = M[2][0]; break; \
+        case 1: cv[0] = M[0][1]; cv[1] = M[1][1]; cv[2] = M[2][1]; break; \
+        case 2: cv[0] = M[0][2]; cv[1] = M[1][2]; cv[2] = M[2][2]; break; \
+        case 3: cv[0] = M[0][3]; cv[1] = M[1][3]; cv[2] = M[2][3]; break; \
+        case 4: cv[0] = M[0][4]; cv[1] = M[1][4]; cv[2] = M[2][4]; break; \
+        default: cv[0] = M[0][5]; cv[1] = M[1][5]; cv[2] = M[2][5]; break; \
+    }
+        for (int i = tid; i < kN; i += kT) {
+            col0[i] = i >= 1 && i < n ? Am[i] : 0.0;
+            col1[i] = 0.0;
+        }
+        if (tid == 0) dT[0] = Am[0];
         __syncthreads();
         for (int j = 0; j + 2 < n; j++) {
             const double *cur = (j & 1) ? col1 : col0;
             double *nxt = (j & 1) ? col0 : col1;
-            const int m = n - j - 1, o = j + 1;
-            double vr[kHQ], vc[kHB], pr[kHQ], tau = 0.0, xa = 0.0, xb = 0.0, v0 = 0.0;
+            const int o = j + 1;
+            double vr[kHQ], vc[kHB], pr[kHQ], tau = 0.0, alpha = 0.0, xa = 0.0, xb = 0.0;
             if (hw) {
-                xa = lane < m ? cur[lane] : 0.0;
-                xb = lane + 64 < m ? cur[lane + 64] : 0.0;
+                xa = cur[lane];
+                xb = lane < kN - 64 ? cur[lane + 64] : 0.0;
                 const double s = wave_sum(xa * xa + xb * xb);
-                const double x0 = cur[0];
-                const double alpha = s > 0.0 ? -copysign(sqrt(s), x0) : 0.0;
-                v0 = x0 - alpha;
+                const double x0 = cur[o];
+                alpha = s > 0.0 ? -copysign(sqrt(s), x0) : 0.0;
+                const double v0 = x0 - alpha;
                 const double vtv = s - x0 * x0 + v0 * v0;
                 tau = (s > 0.0 && vtv > 0.0) ? 2.0 / vtv : 0.0;
-                auto vat = [&](int t) {
-                    const int u = t - o, uc = u < 0 ? 0 : (u < m ? u : m - 1);
-                    const double x = cur[uc];
-                    return u < 0 || u >= m ? 0.0 : (u == 0 ? v0 : x);
-                };
 #pragma unroll
-                for (int q = 0; q < kHQ; q++) vr[q] = vat(hrow(q));
+                for (int q = 0; q < kHQ; q++) vr[q] = cur[hrow[q]] - (hrow[q] == o ? alpha : 0.0);
 #pragma unroll
-                for (int b = 0; b < kHB; b++) vc[b] = vat(hcol(b));
+                for (int b = 0; b < kHB; b++) vc[b] = cur[hcol[b]] - (hcol[b] == o ? alpha : 0.0);
 #pragma unroll
                 for (int q = 0; q < kHQ; q++) {
                     double acc = 0.0;
 #pragma unroll
                     for (int b = 0; b < kHB; b++) acc = fma(M[q][b], vc[b], acc);
-                    pr[q] = tau * group_sum16(acc);
-                    const int i = hrow(q);
-                    if (i < o) pr[q] = 0.0;
-                    if (hc == 0 && i < n) pvec[i] = pr[q];
+                    pr[q] = hrow[q] < o ? 0.0 : tau * group_sum16(acc);
+                    if (hc == 0) pvec[hrow[q]] = pr[q];
                 }
                 if (wave == 0) {
                     if (lane == 0) {
                         eT[j] = tau != 0.0 ? alpha : x0;
                         tauT[j] = tau;
+                        nxt[j] = 0.0;  // the next column is zero down to row j+1
+                        nxt[o] = 0.0;
                     }
-                    // v into column j of Am (rows j+1..n-1)
-                    if (lane < m) Am[o + lane + (size_t)j * L] = lane == 0 ? v0 : xa;
-                    if (lane + 64 < m) Am[o + lane + 64 + (size_t)j * L] = xb;
+                    // v into column j of Am (rows j+1..n-1; zeros above are ignored)
+                    if (lane < n) Am[lane + (size_t)j * L] = xa - (lane == o ? alpha : 0.0);
+                    if (lane + 64 < n)
+                        Am[lane + 64 + (size_t)j * L] = xb - (lane + 64 == o ? alpha : 0.0);
                 }
             }
             __syncthreads();
             if (hw) {
-                // the owner of C[j][j] (final since step j-1)
-#pragma unroll
-                for (int q = 0; q < kHQ; q++)
-#pragma unroll
-                    for (int b = 0; b < kHB; b++)
-                        if (hrow(q) == j && hcol(b) == j) dT[j] = M[q][b];
-                const double pa = lane < m ? pvec[o + lane] : 0.0;
-                const double pb = lane + 64 < m ? pvec[o + lane + 64] : 0.0;
-                const double K = 0.5 * tau * wave_sum((lane == 0 ? v0 : xa) * pa + xb * pb);
+                const double pa = pvec[lane], pb = lane < kN - 64 ? pvec[lane + 64] : 0.0;
+                const double K = 0.5 * tau * wave_sum((xa - (lane == o ? alpha : 0.0)) * pa +
+                                                      (xb - (lane + 64 == o ? alpha : 0.0)) * pb);
                 double wr[kHQ], wc[kHB];
 #pragma unroll
                 for (int q = 0; q < kHQ; q++) wr[q] = pr[q] - K * vr[q];
 #pragma unroll
-                for (int b = 0; b < kHB; b++) {
-                    const int c = hcol(b);
-                    const double pc = pvec[c < o ? o : (c < n ? c : n - 1)];
-                    wc[b] = c < o || c >= n ? 0.0 : pc - K * vc[b];
-                }
+                for (int b = 0; b < kHB; b++) wc[b] = pvec[hcol[b]] - K * vc[b];
 #pragma unroll
                 for (int q = 0; q < kHQ; q++)
 #pragma unroll
                     for (int b = 0; b < kHB; b++)
                         M[q][b] = fma(-vr[q], wc[b], fma(-wr[q], vc[b], M[q][b]));
-                // next column: C[j+2:, j+1]
+                // column o (rows > o) for the next step, and its diagonal entry
+                double cv[kHQ];
+                BX_HH_COLUMN(o, cv)
+                if (hc == (o & 15)) {
 #pragma unroll
-                for (int q = 0; q < kHQ; q++)
-#pragma unroll
-                    for (int b = 0; b < kHB; b++) {
-                        const int i = hrow(q);
-                        if (hcol(b) == o && i > o && i < n) nxt[i - o - 1] = M[q][b];
+                    for (int q = 0; q < kHQ; q++) {
+                        if (hrow[q] > o && hrow[q] < n) nxt[hrow[q]] = cv[q];
+                        if (hrow[q] == o) dT[o] = cv[q];
                     }
+                }
             }
             __syncthreads();
         }
-        // the last 2 x 2 block
-        if (hw) {
+        // the last diagonal entry and subdiagonal entry
+        if (hw && n >= 2) {
+            double cv[kHQ];
+            BX_HH_COLUMN(n - 1, cv)
+            if (hc == ((n - 1) & 15)) {
 #pragma unroll
-            for (int q = 0; q < kHQ; q++)
-#pragma unroll
-                for (int b = 0; b < kHB; b++) {
-                    const int i = hrow(q), c = hcol(b);
-                    if (n >= 2 && i == n - 2 && c == n - 2) dT[n - 2] = M[q][b];
-                    if (n >= 2 && i == n - 1 && c == n - 2) eT[n - 2] = M[q][b];
-                    if (i == n - 1 && c == n - 1) dT[n - 1] = M[q][b];
-                }
-            if (tid == 0 && n >= 2) tauT[n - 2] = 0.0;
+                for (int q = 0; q < kHQ; q++)
+                    if (hrow[q] == n - 1) dT[n - 1] = cv[q];
+            }
+        }
+        if (tid == 0 && n >= 2) {
+            eT[n - 2] = (n & 1 ? col1 : col0)[n - 1];
+            tauT[n - 2] = 0.0;
         }
         __syncthreads();
+#undef BX_HH_COLUMN
     }
 
     // ---- Gershgorin interval; T scaled by a power of two near its norm ----
