@@ -138,7 +138,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=128, help="grid side")
     ap.add_argument("--k", type=int, default=32)
-    ap.add_argument("--maxit", type=int, default=3000)
+    ap.add_argument("--maxit", type=int, default=6000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--roofline-only", action="store_true",

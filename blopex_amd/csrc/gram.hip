@@ -84,11 +84,12 @@ struct GramBlock;
 
 template <int NL, int NQ>
 struct GramBlock<double, NL, NQ> {
-    template <int BQ>
+    // NH: steps of Rows::N rows per lane group in the chunk (RC = 4 N NH)
+    template <int BQ, int NH>
     static __device__ __forceinline__ void run(const double *stage, const int *lo, const int *qo,
                                                f64x4 (&acc)[kGramBL * BQ]) {
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
+        for (int h = 0; h < NH; h++) {
             Rows<double> x[NL], y[NQ];
 #pragma unroll
             for (int i = 0; i < NL; i++) x[i].load(stage + lo[i] + h * Rows<double>::N);
@@ -107,14 +108,19 @@ struct GramBlock<double, NL, NQ> {
 
 template <int NL, int NQ>
 struct GramBlock<float, NL, NQ> {
-    template <int BQ>
+    // NH: steps of Rows::N rows per lane group in the chunk (RC = 4 N NH); the
+    // FP32 tiles are added into the FP64 ones after every two steps (64 rows)
+    template <int BQ, int NH>
     static __device__ __forceinline__ void run(const float *stage, const int *lo, const int *qo,
                                                f64x4 (&acc)[kGramBL * BQ]) {
+      static_assert(NH % 2 == 0, "whole 64-row groups");
+#pragma unroll
+      for (int h0 = 0; h0 < NH; h0 += 2) {
         f32x4 c[NL * NQ];
 #pragma unroll
         for (int t = 0; t < NL * NQ; t++) c[t] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
+        for (int h = h0; h < h0 + 2; h++) {
             Rows<float> x[NL], y[NQ];
 #pragma unroll
             for (int i = 0; i < NL; i++) x[i].load(stage + lo[i] + h * Rows<float>::N);
@@ -134,34 +140,37 @@ struct GramBlock<float, NL, NQ> {
             for (int j = 0; j < NQ; j++)
 #pragma unroll
                 for (int r = 0; r < 4; r++) acc[i * BQ + j][r] += (double)c[i * NQ + j][r];
+      }
     }
 };
 
-template <typename T, int BQ>
+template <typename T, int BQ, int NH>
 __device__ __forceinline__ void gram_block(int nl, int nq, const T *stage, const int *lo,
                                            const int *qo, f64x4 (&acc)[kGramBL * BQ]) {
     switch (nl * 8 + nq) {  // wave-uniform
-        case 9: GramBlock<T, 1, 1>::template run<BQ>(stage, lo, qo, acc); break;
-        case 10: GramBlock<T, 1, 2>::template run<BQ>(stage, lo, qo, acc); break;
-        case 11: GramBlock<T, 1, 3>::template run<BQ>(stage, lo, qo, acc); break;
-        case 17: GramBlock<T, 2, 1>::template run<BQ>(stage, lo, qo, acc); break;
-        case 18: GramBlock<T, 2, 2>::template run<BQ>(stage, lo, qo, acc); break;
-        case 19: GramBlock<T, 2, 3>::template run<BQ>(stage, lo, qo, acc); break;
+        case 9: GramBlock<T, 1, 1>::template run<BQ, NH>(stage, lo, qo, acc); break;
+        case 10: GramBlock<T, 1, 2>::template run<BQ, NH>(stage, lo, qo, acc); break;
+        case 11: GramBlock<T, 1, 3>::template run<BQ, NH>(stage, lo, qo, acc); break;
+        case 17: GramBlock<T, 2, 1>::template run<BQ, NH>(stage, lo, qo, acc); break;
+        case 18: GramBlock<T, 2, 2>::template run<BQ, NH>(stage, lo, qo, acc); break;
+        case 19: GramBlock<T, 2, 3>::template run<BQ, NH>(stage, lo, qo, acc); break;
         default:
             if constexpr (BQ == 4) {
-                if (nl == 1) GramBlock<T, 1, 4>::template run<BQ>(stage, lo, qo, acc);
-                if (nl == 2) GramBlock<T, 2, 4>::template run<BQ>(stage, lo, qo, acc);
+                if (nl == 1) GramBlock<T, 1, 4>::template run<BQ, NH>(stage, lo, qo, acc);
+                if (nl == 2) GramBlock<T, 2, 4>::template run<BQ, NH>(stage, lo, qo, acc);
             }
             break;
     }
 }
 
-// PRE: 16-byte pieces per thread and chunk (6: plans up to 192 columns, two
-// chunks in flight; 9: up to 288 columns, one); BQ: right tiles per register
-// block (3 or 4)
-template <typename T, int PRE, int BQ>
+// PRE: 16-byte pieces per thread and chunk (6: plans up to 192 columns at
+// the standard chunk, two chunks in flight; 9: up to 288 columns, one); BQ:
+// right tiles per register block (3 or 4); RC: rows per chunk (the standard
+// one, or twice that for narrow plans, so that each chunk round trip moves
+// more bytes)
+template <typename T, int PRE, int BQ, int RC>
 __global__ void __launch_bounds__(kGTh) k_gram(GramKArgs a) {
-    constexpr int RC = GramCfg<T>::RC;
+    constexpr int NH = RC / (4 * Rows<T>::N);
     constexpr bool kTwo = PRE <= 6;
     constexpr int EV = 16 / sizeof(T);  // elements per 16-byte piece
     constexpr int S = RC + EV;          // column stride: an odd number of 16-byte pieces
@@ -231,7 +240,7 @@ __global__ void __launch_bounds__(kGTh) k_gram(GramKArgs a) {
             const long chl = ch + step;
             BX_GRAM_P(BX_GRAM_LOAD)
         }
-        gram_block<T, BQ>(nl, nq, stage, lo, qo, acc);
+        gram_block<T, BQ, NH>(nl, nq, stage, lo, qo, acc);
         __syncthreads();
         if (!kTwo || ch + gs >= nch) continue;
         BX_GRAM_Q(BX_GRAM_STORE)
@@ -240,7 +249,7 @@ __global__ void __launch_bounds__(kGTh) k_gram(GramKArgs a) {
             const long chl = ch + 3 * gs;
             BX_GRAM_Q(BX_GRAM_LOAD)
         }
-        gram_block<T, BQ>(nl, nq, stage, lo, qo, acc);
+        gram_block<T, BQ, NH>(nl, nq, stage, lo, qo, acc);
         __syncthreads();
     }
 #undef BX_GRAM_Q
@@ -400,8 +409,13 @@ void gram_run(int prec, long ld, GramPlan &p, long ldg) {
     // three would need more register blocks than there are waves
     const int bq = form_blocks(p, ldg, false, 3) <= kGramWaves ? 3 : kGramBQ;
     form_blocks(p, ldg, true, bq);
-    const size_t es = prec == HIPMV_FP32 ? sizeof(float) : sizeof(double);
-    const int rc = prec == HIPMV_FP32 ? GramCfg<float>::RC : GramCfg<double>::RC;
+    const bool f32 = prec == HIPMV_FP32;
+    const size_t es = f32 ? sizeof(float) : sizeof(double);
+    // narrow plans (at most 96 columns, three right tiles) take chunks of
+    // twice the standard rows: their chunk loop is bound by the round trip
+    const int rc0 = f32 ? GramCfg<float>::RC : GramCfg<double>::RC;
+    const bool narrow = bq == 3 && p.ncols <= 96 && ld % (2 * rc0) == 0;
+    const int rc = narrow ? 2 * rc0 : rc0;
     const int S = rc + (int)(16 / es);
     const long nchunks = ld / rc;
     const size_t lds = (size_t)(p.ncols + 16) * S * es + (size_t)p.ncols * sizeof(void *);
@@ -416,15 +430,18 @@ void gram_run(int prec, long ld, GramPlan &p, long ldg) {
     BX_REQUIRE(lds <= 160 * 1024, "gram_run: %zu bytes of LDS", lds);
     const int vpc = rc / (int)(16 / es);
     const bool small = p.ncols * vpc <= 6 * kGTh;  // 6 pieces per thread suffice
+    BX_REQUIRE(!narrow || small, "gram_run: narrow plan of %d columns", p.ncols);
     const void *kern;
-    if (prec == HIPMV_FP32)
-        kern = small ? (bq == 3 ? (const void *)k_gram<float, 6, 3> : (const void *)k_gram<float, 6, 4>)
-                     : (bq == 3 ? (const void *)k_gram<float, 9, 3> : (const void *)k_gram<float, 9, 4>);
+    if (f32)
+        kern = narrow ? (const void *)k_gram<float, 6, 3, 128>
+               : small ? (bq == 3 ? (const void *)k_gram<float, 6, 3, 64> : (const void *)k_gram<float, 6, 4, 64>)
+                       : (bq == 3 ? (const void *)k_gram<float, 9, 3, 64> : (const void *)k_gram<float, 9, 4, 64>);
     else
-        kern = small ? (bq == 3 ? (const void *)k_gram<double, 6, 3> : (const void *)k_gram<double, 6, 4>)
-                     : (bq == 3 ? (const void *)k_gram<double, 9, 3> : (const void *)k_gram<double, 9, 4>);
-    static const void *attr_done[8];
-    for (int i = 0; i < 8; i++) {
+        kern = narrow ? (const void *)k_gram<double, 6, 3, 64>
+               : small ? (bq == 3 ? (const void *)k_gram<double, 6, 3, 32> : (const void *)k_gram<double, 6, 4, 32>)
+                       : (bq == 3 ? (const void *)k_gram<double, 9, 3, 32> : (const void *)k_gram<double, 9, 4, 32>);
+    static const void *attr_done[10];
+    for (int i = 0; i < 10; i++) {
         if (attr_done[i] == kern) break;
         if (!attr_done[i]) {
             BX_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
