@@ -120,6 +120,136 @@ __device__ int sturm_count(const double *d, const double *e2, int n, double x) {
     return cnt;
 }
 
+// Householder tridiagonalisation of the n x n matrix C (LDS, leading
+// dimension L), held in registers: waves 0..HW-1 keep C, lane (hc = lane % 16,
+// hr = lane / 16) of wave w owning rows w + HW (hr + 4 q), q < HQ, and
+// columns hc + 16 b, b < HB (4 HW HQ >= n, 16 HB >= n), so a row's columns lie
+// in 16 lanes of one wave.  Step j: x = C[j+1:, j], kept in LDS by absolute
+// row with zeros above (so every lane reads its rows and columns at fixed
+// addresses); H_j = I - tau v v' with v = x - alpha e_{j+1}; p = tau C v (row
+// sums in the wave), K = tau v'p / 2, w = p - K v, C -= v w' + w v'.  v goes
+// to column j of C in LDS, the next column and the next diagonal entry to
+// LDS.  Rows and columns up to j have v = w = 0 and do not change.  Every
+// thread of the workgroup calls this (two barriers per column).
+template <int HW, int HQ, int HB>
+__device__ __forceinline__ void householder(int n, int L, double *Am, double *col0, double *col1,
+                                            double *pvec, double *dT, double *eT, double *tauT) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const bool hw = wave < HW;
+    const int hc = lane & 15, hr = lane >> 4;
+    int hrow[HQ], hcol[HB];
+#pragma unroll
+    for (int q = 0; q < HQ; q++) hrow[q] = wave + HW * (hr + 4 * q);
+#pragma unroll
+    for (int b = 0; b < HB; b++) hcol[b] = hc + 16 * b;
+    double M[HQ][HB];
+#pragma unroll
+    for (int q = 0; q < HQ; q++)
+#pragma unroll
+        for (int b = 0; b < HB; b++)
+            M[q][b] = (hw && hrow[q] < n && hcol[b] < n) ? Am[hrow[q] + hcol[b] * L] : 0.0;
+    // the owner lanes of column c (uniform) pick its entries out of their registers
+    auto column = [&](int c, double (&cv)[HQ]) {
+        const int bs = __builtin_amdgcn_readfirstlane(c >> 4);
+#pragma unroll
+        for (int q = 0; q < HQ; q++) cv[q] = M[q][HB - 1];
+#pragma unroll
+        for (int b = 0; b + 1 < HB; b++)
+            if (bs == b) {
+#pragma unroll
+                for (int q = 0; q < HQ; q++) cv[q] = M[q][b];
+            }
+    };
+    for (int i = tid; i < kN; i += kT) {
+        col0[i] = i >= 1 && i < n ? Am[i] : 0.0;
+        col1[i] = 0.0;
+    }
+    if (tid == 0) dT[0] = Am[0];
+    __syncthreads();
+    for (int j = 0; j + 2 < n; j++) {
+        const double *cur = (j & 1) ? col1 : col0;
+        double *nxt = (j & 1) ? col0 : col1;
+        const int o = j + 1;
+        double vr[HQ], vc[HB], pr[HQ], tau = 0.0, alpha = 0.0, xa = 0.0, xb = 0.0;
+        if (hw) {
+            xa = cur[lane];
+            xb = lane < kN - 64 ? cur[lane + 64] : 0.0;
+            const double s = wave_sum(xa * xa + xb * xb);
+            const double x0 = cur[o];
+            alpha = s > 0.0 ? -copysign(sqrt(s), x0) : 0.0;
+            const double v0 = x0 - alpha;
+            const double vtv = s - x0 * x0 + v0 * v0;
+            tau = (s > 0.0 && vtv > 0.0) ? 2.0 / vtv : 0.0;
+#pragma unroll
+            for (int q = 0; q < HQ; q++) vr[q] = cur[hrow[q]] - (hrow[q] == o ? alpha : 0.0);
+#pragma unroll
+            for (int b = 0; b < HB; b++) vc[b] = cur[hcol[b]] - (hcol[b] == o ? alpha : 0.0);
+#pragma unroll
+            for (int q = 0; q < HQ; q++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int b = 0; b < HB; b++) acc = fma(M[q][b], vc[b], acc);
+                pr[q] = hrow[q] < o ? 0.0 : tau * group_sum16(acc);
+                if (hc == 0) pvec[hrow[q]] = pr[q];
+            }
+            if (wave == 0) {
+                if (lane == 0) {
+                    eT[j] = tau != 0.0 ? alpha : x0;
+                    tauT[j] = tau;
+                    nxt[j] = 0.0;  // the next column is zero down to row j+1
+                    nxt[o] = 0.0;
+                }
+                // v into column j of Am (rows j+1..n-1; zeros above are ignored)
+                if (lane < n) Am[lane + (size_t)j * L] = xa - (lane == o ? alpha : 0.0);
+                if (lane + 64 < n)
+                    Am[lane + 64 + (size_t)j * L] = xb - (lane + 64 == o ? alpha : 0.0);
+            }
+        }
+        __syncthreads();
+        if (hw) {
+            const double pa = pvec[lane], pb = lane < kN - 64 ? pvec[lane + 64] : 0.0;
+            const double K = 0.5 * tau * wave_sum((xa - (lane == o ? alpha : 0.0)) * pa +
+                                                  (xb - (lane + 64 == o ? alpha : 0.0)) * pb);
+            double wr[HQ], wc[HB];
+#pragma unroll
+            for (int q = 0; q < HQ; q++) wr[q] = pr[q] - K * vr[q];
+#pragma unroll
+            for (int b = 0; b < HB; b++) wc[b] = pvec[hcol[b]] - K * vc[b];
+#pragma unroll
+            for (int q = 0; q < HQ; q++)
+#pragma unroll
+                for (int b = 0; b < HB; b++)
+                    M[q][b] = fma(-vr[q], wc[b], fma(-wr[q], vc[b], M[q][b]));
+            // column o (rows > o) for the next step, and its diagonal entry
+            double cv[HQ];
+            column(o, cv);
+            if (hc == (o & 15)) {
+#pragma unroll
+                for (int q = 0; q < HQ; q++) {
+                    if (hrow[q] > o && hrow[q] < n) nxt[hrow[q]] = cv[q];
+                    if (hrow[q] == o) dT[o] = cv[q];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // the last diagonal entry and subdiagonal entry
+    if (hw && n >= 2) {
+        double cv[HQ];
+        column(n - 1, cv);
+        if (hc == ((n - 1) & 15)) {
+#pragma unroll
+            for (int q = 0; q < HQ; q++)
+                if (hrow[q] == n - 1) dT[n - 1] = cv[q];
+        }
+    }
+    if (tid == 0 && n >= 2) {
+        eT[n - 2] = (n & 1 ? col1 : col0)[n - 1];
+        tauT[n - 2] = 0.0;
+    }
+    __syncthreads();
+}
+
 __global__ void __launch_bounds__(kT) k_rr_tri(RR2Args r) {
     extern __shared__ double lds[];
     __shared__ double dT[kN], eT[kN], dS[kN], e2S[kN], tauT[kN], invu[kN], sdg[kN];
@@ -565,131 +695,12 @@ __global__ void __launch_bounds__(kT) k_rr_tri(RR2Args r) {
     }
 
     // ---- Householder tridiagonalisation of C, held in registers ----
-    // Waves 0-7 keep C: lane (hc = lane % 16, hr = lane / 16) of wave w owns
-    // rows w + 8 (hr + 4 q), q < 3, and columns hc + 16 b, b < 6, so a row's
-    // 96 columns lie in 16 lanes of one wave.  Step j: x = C[j+1:, j], kept in
-    // LDS by absolute row with zeros above (so every lane reads its rows and
-    // columns at fixed addresses); H_j = I - tau v v' with v = x - alpha e_{j+1};
-    // p = tau C v (row sums in the wave), K = tau v'p / 2, w = p - K v,
-    // C -= v w' + w v'.  v goes to column j of Am, the next column and the
-    // next diagonal entry to LDS.  Rows and columns up to j have v = w = 0 and
-    // do not change.
-    {
-        constexpr int kHQ = 3, kHB = 6, kHW = 8;
-        const bool hw = wave < kHW;
-        const int hc = lane & 15, hr = lane >> 4;
-        int hrow[kHQ], hcol[kHB];
-#pragma unroll
-        for (int q = 0; q < kHQ; q++) hrow[q] = wave + kHW * (hr + 4 * q);
-#pragma unroll
-        for (int b = 0; b < kHB; b++) hcol[b] = hc + 16 * b;
-        double M[kHQ][kHB];
-#pragma unroll
-        for (int q = 0; q < kHQ; q++)
-#pragma unroll
-            for (int b = 0; b < kHB; b++)
-                M[q][b] = (hw && hrow[q] < n && hcol[b] < n) ? Am[hrow[q] + hcol[b] * L] : 0.0;
-        // the owner lanes of column c pick its entries out of their registers
-        // (c is uniform: one scalar branch)
-#define BX_HH_COLUMN(c, cv)                                               \
-    switch (__builtin_amdgcn_readfirstlane((c) >> 4)) {                   \
-        case 0: cv[0] = M[0][0]; cv[1] = M[1][0]; cv[2] = M[2][0]; break; \
-        case 1: cv[0] = M[0][1]; cv[1] = M[1][1]; cv[2] = M[2][1]; break; \
-        case 2: cv[0] = M[0][2]; cv[1] = M[1][2]; cv[2] = M[2][2]; break; \
-        case 3: cv[0] = M[0][3]; cv[1] = M[1][3]; cv[2] = M[2][3]; break; \
-        case 4: cv[0] = M[0][4]; cv[1] = M[1][4]; cv[2] = M[2][4]; break; \
-        default: cv[0] = M[0][5]; cv[1] = M[1][5]; cv[2] = M[2][5]; break; \
-    }
-        for (int i = tid; i < kN; i += kT) {
-            col0[i] = i >= 1 && i < n ? Am[i] : 0.0;
-            col1[i] = 0.0;
-        }
-        if (tid == 0) dT[0] = Am[0];
-        __syncthreads();
-        for (int j = 0; j + 2 < n; j++) {
-            const double *cur = (j & 1) ? col1 : col0;
-            double *nxt = (j & 1) ? col0 : col1;
-            const int o = j + 1;
-            double vr[kHQ], vc[kHB], pr[kHQ], tau = 0.0, alpha = 0.0, xa = 0.0, xb = 0.0;
-            if (hw) {
-                xa = cur[lane];
-                xb = lane < kN - 64 ? cur[lane + 64] : 0.0;
-                const double s = wave_sum(xa * xa + xb * xb);
-                const double x0 = cur[o];
-                alpha = s > 0.0 ? -copysign(sqrt(s), x0) : 0.0;
-                const double v0 = x0 - alpha;
-                const double vtv = s - x0 * x0 + v0 * v0;
-                tau = (s > 0.0 && vtv > 0.0) ? 2.0 / vtv : 0.0;
-#pragma unroll
-                for (int q = 0; q < kHQ; q++) vr[q] = cur[hrow[q]] - (hrow[q] == o ? alpha : 0.0);
-#pragma unroll
-                for (int b = 0; b < kHB; b++) vc[b] = cur[hcol[b]] - (hcol[b] == o ? alpha : 0.0);
-#pragma unroll
-                for (int q = 0; q < kHQ; q++) {
-                    double acc = 0.0;
-#pragma unroll
-                    for (int b = 0; b < kHB; b++) acc = fma(M[q][b], vc[b], acc);
-                    pr[q] = hrow[q] < o ? 0.0 : tau * group_sum16(acc);
-                    if (hc == 0) pvec[hrow[q]] = pr[q];
-                }
-                if (wave == 0) {
-                    if (lane == 0) {
-                        eT[j] = tau != 0.0 ? alpha : x0;
-                        tauT[j] = tau;
-                        nxt[j] = 0.0;  // the next column is zero down to row j+1
-                        nxt[o] = 0.0;
-                    }
-                    // v into column j of Am (rows j+1..n-1; zeros above are ignored)
-                    if (lane < n) Am[lane + (size_t)j * L] = xa - (lane == o ? alpha : 0.0);
-                    if (lane + 64 < n)
-                        Am[lane + 64 + (size_t)j * L] = xb - (lane + 64 == o ? alpha : 0.0);
-                }
-            }
-            __syncthreads();
-            if (hw) {
-                const double pa = pvec[lane], pb = lane < kN - 64 ? pvec[lane + 64] : 0.0;
-                const double K = 0.5 * tau * wave_sum((xa - (lane == o ? alpha : 0.0)) * pa +
-                                                      (xb - (lane + 64 == o ? alpha : 0.0)) * pb);
-                double wr[kHQ], wc[kHB];
-#pragma unroll
-                for (int q = 0; q < kHQ; q++) wr[q] = pr[q] - K * vr[q];
-#pragma unroll
-                for (int b = 0; b < kHB; b++) wc[b] = pvec[hcol[b]] - K * vc[b];
-#pragma unroll
-                for (int q = 0; q < kHQ; q++)
-#pragma unroll
-                    for (int b = 0; b < kHB; b++)
-                        M[q][b] = fma(-vr[q], wc[b], fma(-wr[q], vc[b], M[q][b]));
-                // column o (rows > o) for the next step, and its diagonal entry
-                double cv[kHQ];
-                BX_HH_COLUMN(o, cv)
-                if (hc == (o & 15)) {
-#pragma unroll
-                    for (int q = 0; q < kHQ; q++) {
-                        if (hrow[q] > o && hrow[q] < n) nxt[hrow[q]] = cv[q];
-                        if (hrow[q] == o) dT[o] = cv[q];
-                    }
-                }
-            }
-            __syncthreads();
-        }
-        // the last diagonal entry and subdiagonal entry
-        if (hw && n >= 2) {
-            double cv[kHQ];
-            BX_HH_COLUMN(n - 1, cv)
-            if (hc == ((n - 1) & 15)) {
-#pragma unroll
-                for (int q = 0; q < kHQ; q++)
-                    if (hrow[q] == n - 1) dT[n - 1] = cv[q];
-            }
-        }
-        if (tid == 0 && n >= 2) {
-            eT[n - 2] = (n & 1 ? col1 : col0)[n - 1];
-            tauT[n - 2] = 0.0;
-        }
-        __syncthreads();
-#undef BX_HH_COLUMN
-    }
+    // pencils up to 48: 4 waves (one per SIMD) with 9 entries per lane; up
+    // to 96: 8 waves with 18
+    if (n <= 48)
+        householder<4, 3, 3>(n, L, Am, col0, col1, pvec, dT, eT, tauT);
+    else
+        householder<8, 3, 6>(n, L, Am, col0, col1, pvec, dT, eT, tauT);
 
     // ---- Gershgorin interval; T scaled by a power of two near its norm ----
     if (wave == 0) {
@@ -732,13 +743,13 @@ __global__ void __launch_bounds__(kT) k_rr_tri(RR2Args r) {
             // Bracket from the previous Ritz value e: the pencil contains the
             // previous Ritz vectors, so its idx-th eigenvalue is about e or
             // below.  Lane 0 tries e (plus a margin) as an upper bound, lane
-            // g > 0 tries e - |e| 16^-g as a lower bound; a bound is kept only
-            // where its Sturm count confirms it.
+            // g > 0 tries e - |e| 2^(2 - 6g) as a lower bound; a bound is kept
+            // only where its Sturm count confirms it.
             const double e = r.lam_in[idx] * sc[3];
             if (e > lo && e < hi) {  // false for NaN
                 auto cand = [&](int g) {
                     return g == 0 ? e + 16.0 * kEps * fabs(e) + 1e-3 * kEps
-                                  : e - fabs(e) * ldexp(1.0, -4 * g);
+                                  : e - fabs(e) * ldexp(1.0, 2 - 6 * g);
                 };
                 const bool above = sturm_count(dS, e2S, n, cand(lg)) > idx;
                 const unsigned long long bal = __ballot(above);
@@ -868,7 +879,12 @@ __global__ void __launch_bounds__(kT) k_rr_tri(RR2Args r) {
     }
     __syncthreads();
 
-    // ---- Gram-Schmidt (twice) inside clusters of close eigenvalues ----
+    // ---- Gram-Schmidt inside clusters of close eigenvalues ----
+    // Each vector against the earlier (unit) vectors of its cluster; the
+    // dot products include its own, so |z - P z|^2 = |z|^2 - sum proj^2 is
+    // known without another reduction: a second pass runs only when the
+    // first removed more than three quarters of |z|^2 (twice is enough),
+    // and the last pass also normalises.
     {
         const double ortol = 1e-3 * tnorm;
         for (int c = 1; c < k; c++) {
@@ -878,28 +894,26 @@ __global__ void __launch_bounds__(kT) k_rr_tri(RR2Args r) {
             double *zc = Z + (size_t)c * L;
             for (int pass = 0; pass < 2; pass++) {
                 {
-                    const int cp = c0 + (tid >> 4), part = tid & 15;
+                    const int cp = c0 + (tid >> 4), part = tid & 15;  // cp == c: |z|^2
                     double s = 0.0;
-                    if (cp < c)
+                    if (cp <= c)
                         for (int j = part; j < n; j += 16) s += Z[(size_t)cp * L + j] * zc[j];
                     s = group_sum16(s);
-                    if (cp < c && part == 0) proj[cp] = s;
+                    if (cp <= c && part == 0) proj[cp] = s;
                 }
                 __syncthreads();
+                double rem = proj[c];
+                for (int cp = c0; cp < c; cp++) rem -= proj[cp] * proj[cp];
+                const bool again = pass == 0 && !(rem > 0.25 * proj[c]);  // uniform
+                const double f = again ? 1.0 : (rem > 0.0 ? 1.0 / sqrt(rem) : 0.0);
                 for (int j = tid; j < n; j += kT) {
-                    double s = zc[j];
-                    for (int cp = c0; cp < c; cp++) s -= proj[cp] * Z[(size_t)cp * L + j];
-                    zc[j] = s;
+                    double v = zc[j];
+                    for (int cp = c0; cp < c; cp++) v -= proj[cp] * Z[(size_t)cp * L + j];
+                    zc[j] = v * f;
                 }
                 __syncthreads();
+                if (!again) break;
             }
-            double s2 = 0.0;
-            for (int j = lane; j < n; j += 64) s2 += zc[j] * zc[j];
-            s2 = wave_sum(s2);
-            const double f = s2 > 0.0 ? 1.0 / sqrt(s2) : 0.0;
-            __syncthreads();
-            for (int j = tid; j < n; j += kT) zc[j] *= f;
-            __syncthreads();
         }
     }
 
