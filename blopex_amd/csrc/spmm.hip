@@ -11,7 +11,7 @@
 // so both kernels give the same bits.
 //
 // 7-point stencils whose diagonal copy is laid out on an nx x ny x nz grid
-// (hip_CSRMatrix::grid, nx = 64, 128 or 256) march along z instead: a
+// (hip_CSRMatrix::grid, nx = 64 or 128) march along z instead: a
 // 512-thread workgroup owns 512 / nx full x-lines, one row per thread, of a
 // chunk of consecutive planes.  X of planes z-1, z, z+1 at the thread's own
 // row stays in registers, so each X value comes from HBM once per chunk
@@ -173,12 +173,23 @@ __global__ void __launch_bounds__(kZT) k_zm(const ZmArgs<T, KC> a) {
     auto row = [&](int z, int yv, int xv) {
         return (unsigned)z * plane + (unsigned)yv * NXT + (unsigned)xv;
     };
-    T xm1[KC], x0v[KC], xp1[KC], hal[HPT];
+    // X(z-1), X(z), X(z+1) in use and X(z+2) in flight, the halo lines and
+    // the coefficients one plane ahead: every load has a whole step to land
+    T xm1[KC], x0v[KC], xp1[KC], xp2[KC], hal[HPT], dv[7], dvn[7];
+    auto load_plane = [&](int z, T (&v)[KC]) {
 #pragma unroll
-    for (int j = 0; j < KC; j++) {
-        xm1[j] = (j < a.nc && z0 > 0) ? a.x[j * ld + row(z0 - 1, yy, xq)] : T(0);
-        x0v[j] = j < a.nc ? a.x[j * ld + row(z0, yy, xq)] : T(0);
-    }
+        for (int j = 0; j < KC; j++)
+            v[j] = (j < a.nc && z >= 0 && z < a.nz) ? a.x[j * ld + row(z, yy, xq)] : T(0);
+    };
+    auto load_dv = [&](int z, T (&v)[7]) {
+        const unsigned r = row(z < a.nz ? z : a.nz - 1, yy, xq);
+#pragma unroll
+        for (int d = 0; d < 7; d++) v[d] = a.dval[d * a.dld + r];
+    };
+    load_plane(z0 - 1, xm1);
+    load_plane(z0, x0v);
+    load_plane(z0 + 1, xp1);
+    load_dv(z0, dvn);
     // halo element e = t + kZT i: x = e % NXT, column (e / NXT) % KC, line
     // y0 - 1 for e < NXT KC, y0 + TY after
     auto load_halo = [&](int z) {
@@ -187,7 +198,7 @@ __global__ void __launch_bounds__(kZT) k_zm(const ZmArgs<T, KC> a) {
             const int e = t + kZT * i, xv = e % NXT, j = (e / NXT) % KC;
             const int yv = e < NXT * KC ? y0 - 1 : y0 + TY;
             const bool ok = j < a.nc && yv >= 0 && yv < a.ny;
-            hal[i] = ok ? a.x[(unsigned)j * ld + row(z, yv, xv)] : T(0);
+            hal[i] = ok ? a.x[(unsigned)j * ld + row(z, ok ? yv : 0, xv)] : T(0);
         }
     };
     load_halo(z0);
@@ -199,15 +210,14 @@ __global__ void __launch_bounds__(kZT) k_zm(const ZmArgs<T, KC> a) {
             const int e = t + kZT * i, xv = e % NXT, j = (e / NXT) % KC;
             slab[((e < NXT * KC ? 0 : TY + 1) * KC + j) * NXT + xv] = hal[i];
         }
-        const bool up = z + 1 < a.nz;
 #pragma unroll
-        for (int j = 0; j < KC; j++)
-            xp1[j] = (j < a.nc && up) ? a.x[j * ld + row(z + 1, yy, xq)] : T(0);
-        if (up) load_halo(z + 1);
+        for (int d = 0; d < 7; d++) dv[d] = dvn[d];
+        load_plane(z + 2, xp2);
+        if (z + 1 < a.nz) {
+            load_halo(z + 1);
+            load_dv(z + 1, dvn);
+        }
         const unsigned r = row(z, yy, xq);
-        T dv[7];
-#pragma unroll
-        for (int d = 0; d < 7; d++) dv[d] = a.dval[d * a.dld + r];
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < KC; j++) {
@@ -231,6 +241,7 @@ __global__ void __launch_bounds__(kZT) k_zm(const ZmArgs<T, KC> a) {
         for (int j = 0; j < KC; j++) {
             xm1[j] = x0v[j];
             x0v[j] = xp1[j];
+            xp1[j] = xp2[j];
         }
     }
 }
@@ -255,7 +266,7 @@ unsigned xcd_grid(long n) {
 // (BLOPEX_AMD_ZM_MIN_ZC, default 4)
 int zm_chunk(const hip_CSRMatrix *A) {
     const int nx = A->grid[0], ny = A->grid[1], nz = A->grid[2];
-    if (A->ndiag != 7 || nx == 0 || (nx != 64 && nx != 128 && nx != 256)) return 0;
+    if (A->ndiag != 7 || (nx != 64 && nx != 128)) return 0;
     const int ty = kZT / nx;
     if (ny % ty) return 0;
     const long ntiles = ny / ty;
@@ -300,8 +311,7 @@ void spmm_zm(const hip_CSRMatrix *A, int zc, const T *dv, const T *x, long ldx, 
              T *y, long ldy, const ColMap &ym) {
     switch (A->grid[0]) {
         case 64: launch_zm<T, KC, 64>(A, zc, dv, x, ldx, xm, y, ldy, ym); break;
-        case 128: launch_zm<T, KC, 128>(A, zc, dv, x, ldx, xm, y, ldy, ym); break;
-        default: launch_zm<T, KC, 256>(A, zc, dv, x, ldx, xm, y, ldy, ym); break;
+        default: launch_zm<T, KC, 128>(A, zc, dv, x, ldx, xm, y, ldy, ym); break;
     }
 }
 

@@ -122,11 +122,11 @@ def _products(dA, X, prec, paths):
 
 
 @pytest.mark.parametrize("prec", [L.HIPMV_FP64, L.HIPMV_FP32])
-@pytest.mark.parametrize("shape,k", [((64, 16, 24), 7), ((128, 8, 12), 40), ((256, 4, 8), 17)])
+@pytest.mark.parametrize("shape,k", [((64, 16, 24), 7), ((128, 8, 12), 40), ((64, 8, 40), 33)])
 def test_spmm_stencil_kernel_matches_csr_kernel(prec, shape, k, monkeypatch):
-    """The z-marching kernel of 7-point stencils (x lines of 64, 128 and 256
-    rows; several column chunks; one-plane chunks allowed here) gives the
-    same bits as the diagonal and CSR kernels."""
+    """The z-marching kernel of 7-point stencils (x lines of 64 and 128 rows;
+    several column chunks; one-plane chunks allowed here) gives the same bits
+    as the diagonal and CSR kernels."""
     say(f"spmm z-marching vs CSR kernel {shape} k={k} prec={prec}")
     monkeypatch.setenv("BLOPEX_AMD_ZM_MIN_ZC", "1")
     A = pb.laplacian_7pt(*shape, sigma=1.0, seed=8)
@@ -174,6 +174,12 @@ def test_spmm_stencil_grid_detection():
     assert dB.ptr.contents.ndiag == 7 and list(dB.ptr.contents.grid) == [0, 0, 0]
     assert L.load().hip_CSRMatrixProductKernel(dB.ptr, L.HIPMV_FP32, 8, B.n) == 1
     dB.close()
+    # x lines of 256 rows: the diagonal kernel
+    C = pb.laplacian_7pt(256, 4, 4, sigma=0.0, seed=1)
+    dC = bl.DeviceCSR(C)
+    assert list(dC.ptr.contents.grid) == [256, 4, 4]
+    assert L.load().hip_CSRMatrixProductKernel(dC.ptr, L.HIPMV_FP32, 8, C.n) == 1
+    dC.close()
 
 
 def _wrap(ii, blk):
