@@ -212,8 +212,8 @@ __global__ void __launch_bounds__(kZT) k_zm(const ZmArgs<T, KC> a) {
         }
 #pragma unroll
         for (int d = 0; d < 7; d++) dv[d] = dvn[d];
-        load_plane(z + 2, xp2);
-        if (z + 1 < a.nz) {
+        if (z + 1 < z0 + a.zc) {  // the next step of this chunk
+            load_plane(z + 2, xp2);
             load_halo(z + 1);
             load_dv(z + 1, dvn);
         }
