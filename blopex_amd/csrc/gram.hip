@@ -295,7 +295,7 @@ __global__ void __launch_bounds__(256) k_gram_sum(ReduceArgs a) {
     for (; q < a.nparts; q++) s0 += src[(size_t)q * stride];
     const double s = (s0 + s1) + (s2 + s3);
     if (i < a.rows[t] && jj < a.cols[t]) {
-        a.dst[t][i + (size_t)jj * a.ldg] = s;
+        if (a.dst[t]) a.dst[t][i + (size_t)jj * a.ldg] = s;
         if (a.mdst[t]) a.mdst[t][jj + (size_t)i * a.ldg] = s;
     }
 }
@@ -360,6 +360,10 @@ int form_blocks(GramPlan &p, long ldg, bool fill, int bq) {
                             if (sym && y < x) continue;  // stored as the transpose of (y, x)
                             p.rows[s] = (unsigned char)(lw - 16 * x < 16 ? lw - 16 * x : 16);
                             p.cols[s] = (unsigned char)(qw - 16 * y < 16 ? qw - 16 * y : 16);
+                            if (p.pt[q]) {  // only the transpose is stored
+                                p.mdst[s] = p.pg[q] + 16 * y + (size_t)16 * x * ldg;
+                                continue;
+                            }
                             p.dst[s] = p.pg[q] + 16 * x + (size_t)16 * y * ldg;
                             if (sym && x < y) p.mdst[s] = p.pg[q] + 16 * y + (size_t)16 * x * ldg;
                         }
@@ -375,14 +379,16 @@ int form_blocks(GramPlan &p, long ldg, bool fill, int bq) {
 }  // namespace
 
 bool gram_plan_product(GramPlan &p0, const void *L, int lw, const void *Q, int qw, double *g,
-                       long ldg) {
+                       long ldg, bool transposed) {
     GramPlan p = p0;  // work on a copy: commit only if everything fits
     if (p.nprod >= 16) return false;
     const int lb = plan_block(p, L, lw), qb = plan_block(p, Q, qw);
     if (lb < 0 || qb < 0) return false;
+    BX_REQUIRE(!(transposed && lb == qb), "gram_plan_product: a symmetric product stored transposed");
     p.pl[p.nprod] = (short)lb;
     p.pq[p.nprod] = (short)qb;
     p.pg[p.nprod] = g;
+    p.pt[p.nprod] = transposed ? 1 : 0;
     p.nprod++;
     if (form_blocks(p, ldg, false, kGramBQ) > kGramWaves) return false;
     p0 = p;
@@ -394,6 +400,14 @@ void GramBatch::add(const void *L, int lw, const void *Q, int qw, double *g) {
     if (gram_plan_product(p_, L, lw, Q, qw, g, ldg_)) return;
     flush();
     BX_REQUIRE(gram_plan_product(p_, L, lw, Q, qw, g, ldg_),
+               "Gram product of %d x %d columns does not fit one plan", lw, qw);
+}
+
+void GramBatch::add_transposed(const void *L, int lw, const void *Q, int qw, double *g) {
+    if (lw == 0 || qw == 0) return;
+    if (gram_plan_product(p_, L, lw, Q, qw, g, ldg_, true)) return;
+    flush();
+    BX_REQUIRE(gram_plan_product(p_, L, lw, Q, qw, g, ldg_, true),
                "Gram product of %d x %d columns does not fit one plan", lw, qw);
 }
 
@@ -455,7 +469,7 @@ void gram_run(int prec, long ld, GramPlan &p, long ldg) {
     ReduceArgs r;
     int nt = 0;
     for (int s = 0; s < p.nwb * kSlots; s++) {
-        if (!p.dst[s]) continue;
+        if (!p.dst[s] && !p.mdst[s]) continue;
         r.dst[nt] = p.dst[s];
         r.mdst[nt] = p.mdst[s];
         r.rows[nt] = p.rows[s];

@@ -94,13 +94,14 @@ struct GramPlan {
     int nprod;
     short pl[16], pq[16];      // block index of L and Q
     double *pg[16];            // output origin
+    unsigned char pt[16];      // 1: store (L'Q)' = Q'L at pg
 };
 // Adds the product L' Q (L: lw columns, Q: qw columns) landing at g (leading
 // dim ldg); a product with Q == L (same block and width) is symmetric: only
 // its upper tiles are stored, each also as its transpose.  Returns false,
 // leaving the plan unchanged, when it does not fit.
 bool gram_plan_product(GramPlan &p, const void *L, int lw, const void *Q, int qw, double *g,
-                       long ldg);
+                       long ldg, bool transposed = false);
 void gram_run(int prec, long ld, GramPlan &p, long ldg);
 
 // Accumulates products and runs the plan whenever it is full.
@@ -108,6 +109,9 @@ class GramBatch {
   public:
     GramBatch(int prec, long ld, long ldg) : prec_(prec), ld_(ld), ldg_(ldg), p_() {}
     void add(const void *L, int lw, const void *Q, int qw, double *g);
+    // g (qw x lw) receives Q'L, computed as the transpose of L'Q: lets a
+    // product join the register blocks of L
+    void add_transposed(const void *L, int lw, const void *Q, int qw, double *g);
     void flush();
 
   private:

@@ -498,7 +498,13 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
             }
             gb.add(R_, nr, AX_, k, GA_ + r0);
             gb.add(R_, nr, AR_, nr, GA_ + r0 + (size_t)r0 * ld3_);
-            gb.add(R_, nr, BXp, k, GB_ + r0);
+            // with the explicit X blocks, R'BX as the transpose of (BX)'R: for
+            // B = I, X then leads 6 right tiles, R 6 and P 12, which makes 8
+            // register blocks of 2 x 3 tiles, two on each SIMD
+            if (explicit_x)
+                gb.add_transposed(BXp, k, R_, nr, GB_ + r0);
+            else
+                gb.add(R_, nr, BXp, k, GB_ + r0);
             gb.add(R_, nr, BRp, nr, GB_ + r0 + (size_t)r0 * ld3_);
             if (preq) {
                 gb.add(P_, nr, AX_, k, GA_ + p0);
