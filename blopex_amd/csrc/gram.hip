@@ -428,8 +428,10 @@ void gram_run(int prec, long ld, GramPlan &p, long ldg) {
     // narrow plans (at most 96 columns, three right tiles) take chunks of
     // twice the standard rows: their chunk loop is bound by the round trip
     const int rc0 = f32 ? GramCfg<float>::RC : GramCfg<double>::RC;
-    const bool narrow = bq == 3 && p.ncols <= 96 && ld % (2 * rc0) == 0;
-    const int rc = narrow ? 2 * rc0 : rc0;
+    // (four times for at most 72 columns, one chunk in flight)
+    const bool narrow4 = bq == 3 && p.ncols <= 72 && ld % (4 * rc0) == 0;
+    const bool narrow = !narrow4 && bq == 3 && p.ncols <= 96 && ld % (2 * rc0) == 0;
+    const int rc = narrow4 ? 4 * rc0 : (narrow ? 2 * rc0 : rc0);
     const int S = rc + (int)(16 / es);
     const long nchunks = ld / rc;
     const size_t lds = (size_t)(p.ncols + 16) * S * es + (size_t)p.ncols * sizeof(void *);
@@ -445,17 +447,21 @@ void gram_run(int prec, long ld, GramPlan &p, long ldg) {
     const int vpc = rc / (int)(16 / es);
     const bool small = p.ncols * vpc <= 6 * kGTh;  // 6 pieces per thread suffice
     BX_REQUIRE(!narrow || small, "gram_run: narrow plan of %d columns", p.ncols);
+    BX_REQUIRE(!narrow4 || p.ncols * vpc <= 9 * kGTh, "gram_run: narrow plan of %d columns",
+               p.ncols);
     const void *kern;
     if (f32)
-        kern = narrow ? (const void *)k_gram<float, 6, 3, 128>
+        kern = narrow4  ? (const void *)k_gram<float, 9, 3, 256>
+               : narrow ? (const void *)k_gram<float, 6, 3, 128>
                : small ? (bq == 3 ? (const void *)k_gram<float, 6, 3, 64> : (const void *)k_gram<float, 6, 4, 64>)
                        : (bq == 3 ? (const void *)k_gram<float, 9, 3, 64> : (const void *)k_gram<float, 9, 4, 64>);
     else
-        kern = narrow ? (const void *)k_gram<double, 6, 3, 64>
+        kern = narrow4  ? (const void *)k_gram<double, 9, 3, 128>
+               : narrow ? (const void *)k_gram<double, 6, 3, 64>
                : small ? (bq == 3 ? (const void *)k_gram<double, 6, 3, 32> : (const void *)k_gram<double, 6, 4, 32>)
                        : (bq == 3 ? (const void *)k_gram<double, 9, 3, 32> : (const void *)k_gram<double, 9, 4, 32>);
-    static const void *attr_done[10];
-    for (int i = 0; i < 10; i++) {
+    static const void *attr_done[12];
+    for (int i = 0; i < 12; i++) {
         if (attr_done[i] == kern) break;
         if (!attr_done[i]) {
             BX_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));

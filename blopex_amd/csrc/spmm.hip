@@ -164,7 +164,9 @@ template <typename T, int KC, int NXT>
 __global__ void __launch_bounds__(kZT) k_zm(const ZmArgs<T, KC> a) {
     constexpr int TY = kZT / NXT;            // x-lines per workgroup
     constexpr int HPT = 2 * NXT * KC / kZT;  // halo values per thread
-    __shared__ T slab[(TY + 2) * KC * NXT];  // [line -1..TY][column][x]
+    constexpr int LINE = KC * NXT;           // one x-line of all columns
+    __shared__ T slab[TY * LINE];            // plane z: [line][column][x]
+    __shared__ T hbuf[2][2 * LINE];          // halo lines y0-1, y0+TY of planes z, z+1
     const int t = threadIdx.x, xq = t % NXT, ly = t / NXT;
     const long lin = a.xcd ? xcd_block(blockIdx.x, gridDim.x) : (long)blockIdx.x;
     const int zc = (int)(lin / a.ntiles), tile = (int)(lin % a.ntiles);
@@ -173,8 +175,12 @@ __global__ void __launch_bounds__(kZT) k_zm(const ZmArgs<T, KC> a) {
     auto row = [&](int z, int yv, int xv) {
         return (unsigned)z * plane + (unsigned)yv * NXT + (unsigned)xv;
     };
-    // X(z-1), X(z), X(z+1) in use and X(z+2) in flight, the halo lines and
-    // the coefficients one plane ahead: every load has a whole step to land
+    // X(z-1), X(z), X(z+1) in use and X(z+2) in flight, the coefficients one
+    // plane ahead: every load has a whole step to land.  The halo lines of
+    // plane z+2 are fetched in the step in which the neighbouring workgroups
+    // fetch them as their own X(z+2), so that one of the two reads finds them
+    // in L2; they land in registers and go to LDS one step later.
+    // Coefficients and Y stream past L2 (non-temporal).
     T xm1[KC], x0v[KC], xp1[KC], xp2[KC], hal[HPT], dv[7], dvn[7];
     auto load_plane = [&](int z, T (&v)[KC]) {
 #pragma unroll
@@ -184,49 +190,55 @@ __global__ void __launch_bounds__(kZT) k_zm(const ZmArgs<T, KC> a) {
     auto load_dv = [&](int z, T (&v)[7]) {
         const unsigned r = row(z < a.nz ? z : a.nz - 1, yy, xq);
 #pragma unroll
-        for (int d = 0; d < 7; d++) v[d] = a.dval[d * a.dld + r];
+        for (int d = 0; d < 7; d++) v[d] = __builtin_nontemporal_load(a.dval + d * a.dld + r);
+    };
+    // halo element e = t + kZT i: x = e % NXT, column (e / NXT) % KC, line
+    // y0 - 1 for e < LINE, y0 + TY after; stored at hbuf[.][e]
+    auto load_halo = [&](int z) {
+#pragma unroll
+        for (int i = 0; i < HPT; i++) {
+            const int e = t + kZT * i, xv = e % NXT, j = (e / NXT) % KC;
+            const int yv = e < LINE ? y0 - 1 : y0 + TY;
+            const bool ok = j < a.nc && yv >= 0 && yv < a.ny;
+            hal[i] = ok ? a.x[(unsigned)j * ld + row(z, ok ? yv : 0, xv)] : T(0);
+        }
+    };
+    auto store_halo = [&](T *h) {
+#pragma unroll
+        for (int i = 0; i < HPT; i++) h[t + kZT * i] = hal[i];
     };
     load_plane(z0 - 1, xm1);
     load_plane(z0, x0v);
     load_plane(z0 + 1, xp1);
     load_dv(z0, dvn);
-    // halo element e = t + kZT i: x = e % NXT, column (e / NXT) % KC, line
-    // y0 - 1 for e < NXT KC, y0 + TY after
-    auto load_halo = [&](int z) {
-#pragma unroll
-        for (int i = 0; i < HPT; i++) {
-            const int e = t + kZT * i, xv = e % NXT, j = (e / NXT) % KC;
-            const int yv = e < NXT * KC ? y0 - 1 : y0 + TY;
-            const bool ok = j < a.nc && yv >= 0 && yv < a.ny;
-            hal[i] = ok ? a.x[(unsigned)j * ld + row(z, ok ? yv : 0, xv)] : T(0);
-        }
-    };
     load_halo(z0);
+    store_halo(hbuf[z0 & 1]);
+    if (z0 + 1 < z0 + a.zc) load_halo(z0 + 1);
     for (int z = z0; z < z0 + a.zc; z++) {
 #pragma unroll
-        for (int j = 0; j < KC; j++) slab[((ly + 1) * KC + j) * NXT + xq] = x0v[j];
-#pragma unroll
-        for (int i = 0; i < HPT; i++) {
-            const int e = t + kZT * i, xv = e % NXT, j = (e / NXT) % KC;
-            slab[((e < NXT * KC ? 0 : TY + 1) * KC + j) * NXT + xv] = hal[i];
-        }
+        for (int j = 0; j < KC; j++) slab[ly * LINE + j * NXT + xq] = x0v[j];
+        if (z + 1 < z0 + a.zc) store_halo(hbuf[(z + 1) & 1]);
 #pragma unroll
         for (int d = 0; d < 7; d++) dv[d] = dvn[d];
         if (z + 1 < z0 + a.zc) {  // the next step of this chunk
             load_plane(z + 2, xp2);
-            load_halo(z + 1);
             load_dv(z + 1, dvn);
         }
+        if (z + 2 < z0 + a.zc) load_halo(z + 2);
         const unsigned r = row(z, yy, xq);
+        // lines y - 1 and y + 1 of this wave (ly is uniform in a wave)
+        const T *hb = hbuf[z & 1];
+        const T *lm = ly > 0 ? slab + (ly - 1) * LINE : hb;
+        const T *lp = ly + 1 < TY ? slab + (ly + 1) * LINE : hb + LINE;
+        const T *lc = slab + ly * LINE;
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < KC; j++) {
             if (j >= a.nc) continue;
-            const T *sl = slab + j * NXT;
-            const T ym = sl[ly * KC * NXT + xq];
-            const T xmn = xq > 0 ? sl[(ly + 1) * KC * NXT + xq - 1] : T(0);
-            const T xpn = xq + 1 < NXT ? sl[(ly + 1) * KC * NXT + xq + 1] : T(0);
-            const T yp = sl[(ly + 2) * KC * NXT + xq];
+            const T ym = lm[j * NXT + xq];
+            const T xmn = xq > 0 ? lc[j * NXT + xq - 1] : T(0);
+            const T xpn = xq + 1 < NXT ? lc[j * NXT + xq + 1] : T(0);
+            const T yp = lp[j * NXT + xq];
             T s = fma(dv[0], xm1[j], T(0));
             s = fma(dv[1], ym, s);
             s = fma(dv[2], xmn, s);
@@ -234,7 +246,7 @@ __global__ void __launch_bounds__(kZT) k_zm(const ZmArgs<T, KC> a) {
             s = fma(dv[4], xpn, s);
             s = fma(dv[5], yp, s);
             s = fma(dv[6], xp1[j], s);
-            a.y[j * ld + r] = s;
+            __builtin_nontemporal_store(s, a.y + j * ld + r);
         }
         __syncthreads();
 #pragma unroll

@@ -107,7 +107,7 @@ def _products(dA, X, prec, paths):
     for path in paths:
         c.ndiag = 0 if path == "csr" else nd
         for i in range(3):
-            c.grid[i] = 0 if path != "zm" else grid[i]
+            c.grid[i] = grid[i] if path == "zm" else 0
         dY = bl.DeviceBlock(X.shape[0], X.shape[1], prec)
         lib.HipCSRMatMultiVec(ctypes.cast(dA.ptr, ctypes.c_void_p),
                               ctypes.cast(dX.ptr, ctypes.c_void_p),
@@ -156,6 +156,29 @@ def test_spmm_stencil_kernel_at_128_cubed():
     zm, csr = _products(dA, X, L.HIPMV_FP32, ["zm", "csr"])
     np.testing.assert_array_equal(zm, csr)
     dA.close()
+
+
+@pytest.mark.parametrize("prec", [L.HIPMV_FP64, L.HIPMV_FP32])
+def test_spmm_stencil_nonsymmetric(prec, monkeypatch):
+    """A stencil whose coefficients are not symmetric keeps its grid and the
+    z-marching kernel; the product equals the CSR kernel's."""
+    say(f"spmm z-marching non-symmetric stencil prec={prec}")
+    monkeypatch.setenv("BLOPEX_AMD_ZM_MIN_ZC", "1")
+    A = pb.laplacian_7pt(64, 16, 8, sigma=1.0, seed=5)
+    S = A.to_scipy().tolil()
+    for r, c in ((70, 71), (64 * 16 * 3 + 5, 64 * 16 * 4 + 5), (64 * 5 + 9, 64 * 6 + 9)):
+        S[r, c] = S[r, c] * 0.75  # x, z and y couplings, one direction only
+    S = S.tocsr()
+    B = pb.CSR(S.indptr, S.indices, S.data)
+    dB = bl.DeviceCSR(B)
+    assert list(dB.ptr.contents.grid) == [64, 16, 8]
+    assert L.load().hip_CSRMatrixProductKernel(dB.ptr, prec, 24, B.n) == 2
+    X = np.random.default_rng(6).standard_normal((B.n, 24))
+    zm, csr = _products(dB, X, prec, ["zm", "csr"])
+    np.testing.assert_array_equal(zm, csr)
+    ref = S @ X
+    assert np.max(np.abs(zm - ref)) < (1e-12 if prec == L.HIPMV_FP64 else 1e-4)
+    dB.close()
 
 
 def test_spmm_stencil_grid_detection():
