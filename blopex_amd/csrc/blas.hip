@@ -192,13 +192,33 @@ __global__ void __launch_bounds__(kThreads) k_resid_norm(long n, const T *bx, co
                                                          ColMap act, const double *lam, T *r,
                                                          long ldr, const T *tdiag, double *part) {
     __shared__ double red[kThreads / 64];
-    const int t = blockIdx.y;
+    // blockIdx.x: column (fastest, so the workgroups of one row part run
+    // together and share the tdiag lines in L2); blockIdx.y: row part
+    const int t = blockIdx.x, np = gridDim.y;
     const int c = act.idx[t];
     const T l = (T)lam[c];
     const T *bc = colp(bx, ld, c), *ac = colp(ax, ld, c);
     T *rc = colp(r, ldr, t);
     double s = 0.0;
-    for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < n; i += (long)gridDim.x * kThreads) {
+    const long stride = (long)np * kThreads;
+    long i = (long)blockIdx.y * kThreads + threadIdx.x;
+    // four rows of the part in flight at a time, summed in row order
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        T b[4], a4[4], d[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            b[u] = bc[i + u * stride];
+            a4[u] = ac[i + u * stride];
+            d[u] = tdiag ? tdiag[i + u * stride] : T(1);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const T v = l * b[u] - a4[u];
+            s = fma((double)v, (double)v, s);
+            rc[i + u * stride] = tdiag ? d[u] * v : v;
+        }
+    }
+    for (; i < n; i += stride) {
         const T v = l * bc[i] - ac[i];
         s = fma((double)v, (double)v, s);
         rc[i] = tdiag ? tdiag[i] * v : v;
@@ -209,7 +229,7 @@ __global__ void __launch_bounds__(kThreads) k_resid_norm(long n, const T *bx, co
     if (threadIdx.x == 0) {
         double u = 0.0;
         for (int w = 0; w < kThreads / 64; w++) u += red[w];
-        part[(size_t)t * gridDim.x + blockIdx.x] = u;
+        part[(size_t)t * np + blockIdx.y] = u;
     }
 }
 
@@ -435,7 +455,7 @@ void residual_norm_cols(int prec, long n, const void *bx, const void *ax, long l
     unsigned nparts = row_grid(n);
     if (nparts > 512) nparts = 512;  // the diagdot_cols partition
     double *part = (double *)scratch(1, (size_t)nparts * act.n * sizeof(double));
-    dim3 g(nparts, act.n);
+    dim3 g(act.n, nparts);
     if (prec == HIPMV_FP32)
         hipLaunchKernelGGL(k_resid_norm<float>, g, dim3(kThreads), 0, stream(), n,
                            (const float *)bx, (const float *)ax, ld, act, lam, (float *)r, ldr,

@@ -7,7 +7,8 @@
 // FP32), so at any time they stream one contiguous window of every column.
 // A chunk of each distinct block of the plan is copied into LDS with 16-byte
 // coalesced loads (column-major, stride an odd number of 16-byte pieces);
-// the next two chunks are in flight in registers while one is multiplied.  Each wave owns a
+// while one chunk is multiplied, the next is written into a second LDS stage
+// and the one after is in flight in registers.  Each wave owns a
 // register block of up to 2 L tiles x 4 Q tiles (16 columns each): per half
 // chunk a lane reads its rows of those tiles with 16-byte LDS reads and runs
 // the tile products on MFMA 16x16x4.  The k-index of the MFMA runs over the
@@ -164,14 +165,17 @@ __device__ __forceinline__ void gram_block(int nl, int nq, const T *stage, const
 }
 
 // PRE: 16-byte pieces per thread and chunk (6: plans up to 192 columns at
-// the standard chunk, two chunks in flight; 9: up to 288 columns, one); BQ:
+// the standard chunk, two LDS stages; 9: up to 288 columns, one); BQ:
 // right tiles per register block (3 or 4); RC: rows per chunk (the standard
 // one, or twice that for narrow plans, so that each chunk round trip moves
 // more bytes)
 template <typename T, int PRE, int BQ, int RC>
 __global__ void __launch_bounds__(kGTh) k_gram(GramKArgs a) {
     constexpr int NH = RC / (4 * Rows<T>::N);
-    constexpr bool kTwo = PRE <= 6;
+    // PRE <= 6: two LDS stages; a chunk is multiplied out of one while the
+    // next is written into the other, one barrier per chunk.  PRE 9: one
+    // stage (LDS would not hold two), store - barrier - multiply - barrier.
+    constexpr bool kDB = PRE <= 6;
     constexpr int EV = 16 / sizeof(T);  // elements per 16-byte piece
     constexpr int S = RC + EV;          // column stride: an odd number of 16-byte pieces
     constexpr int VPC = RC / EV;                     // pieces per column per chunk
@@ -179,7 +183,9 @@ __global__ void __launch_bounds__(kGTh) k_gram(GramKArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const GramPlan &p = a.p;
     T *stage = reinterpret_cast<T *>(smem);
-    const T **colp = reinterpret_cast<const T **>(smem + (size_t)(p.ncols + 16) * S * sizeof(T));
+    const int sbuf = (p.ncols + 16) * S;  // elements of one stage
+    const T **colp =
+        reinterpret_cast<const T **>(smem + (size_t)(kDB ? 2 : 1) * sbuf * sizeof(T));
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar branches
     const int li = lane & 15, lk = lane >> 4;
@@ -205,13 +211,10 @@ __global__ void __launch_bounds__(kGTh) k_gram(GramKArgs a) {
     for (int s = 0; s < kGramBL * BQ; s++) acc[s] = f64x4{0.0, 0.0, 0.0, 0.0};
 
     // the (up to 9) 16-byte pieces this thread copies per chunk, in named
-    // registers (an array of them would live in scratch memory), two chunks
-    // in flight (sets p and q); pieces past the plan load piece 0 and land in
-    // the spare columns
-    uint4 p0, p1, p2, p3, p4, p5, p6, p7, p8, q0, q1, q2, q3, q4, q5, q6, q7, q8;
+    // registers (an array of them would live in scratch memory); pieces past
+    // the plan load piece 0 and land in the spare columns
+    uint4 p0, p1, p2, p3, p4, p5, p6, p7, p8;
 #define BX_GRAM_P(X) X(0, p0) X(1, p1) X(2, p2) X(3, p3) X(4, p4) X(5, p5) X(6, p6) X(7, p7) X(8, p8)
-#define BX_GRAM_Q(X) \
-    if (kTwo) { X(0, q0) X(1, q1) X(2, q2) X(3, q3) X(4, q4) X(5, q5) X(6, q6) X(7, q7) X(8, q8) }
 #define BX_GRAM_LOAD(q, r)                                                                    \
     if (q < PRE) {                                                                            \
         const int e = tid + q * kGTh < nvec ? tid + q * kGTh : 0;                             \
@@ -221,38 +224,51 @@ __global__ void __launch_bounds__(kGTh) k_gram(GramKArgs a) {
     if (q < PRE) {                                                                            \
         const int e = tid + q * kGTh;                                                         \
         const int o = e < nvec ? (e / VPC) * S + (e % VPC) * EV : (p.ncols + (tid & 15)) * S; \
-        *reinterpret_cast<uint4 *>(stage + o) = r;                                            \
+        *reinterpret_cast<uint4 *>(dstg + o) = r;                                             \
     }
     const long c0 = blockIdx.x;
     if (c0 < nch) {
         const long chl = c0;
         BX_GRAM_P(BX_GRAM_LOAD)
     }
-    if (kTwo && c0 + gs < nch) {
-        const long chl = c0 + gs;
-        BX_GRAM_Q(BX_GRAM_LOAD)
-    }
-    const long step = kTwo ? 2 * gs : gs;
-    for (long ch = c0; ch < nch; ch += step) {
-        BX_GRAM_P(BX_GRAM_STORE)
-        __syncthreads();
-        if (ch + step < nch) {
-            const long chl = ch + step;
+    if (kDB) {
+        // chunk ch is in stage cur; the registers hold chunk ch + gs
+        if (c0 < nch) {
+            T *dstg = stage;
+            BX_GRAM_P(BX_GRAM_STORE)
+        }
+        if (c0 + gs < nch) {
+            const long chl = c0 + gs;
             BX_GRAM_P(BX_GRAM_LOAD)
         }
-        gram_block<T, BQ, NH>(nl, nq, stage, lo, qo, acc);
         __syncthreads();
-        if (!kTwo || ch + gs >= nch) continue;
-        BX_GRAM_Q(BX_GRAM_STORE)
-        __syncthreads();
-        if (ch + 3 * gs < nch) {
-            const long chl = ch + 3 * gs;
-            BX_GRAM_Q(BX_GRAM_LOAD)
+        int cur = 0;
+        for (long ch = c0; ch < nch; ch += gs) {
+            if (ch + gs < nch) {
+                T *dstg = stage + (cur ^ 1) * sbuf;
+                BX_GRAM_P(BX_GRAM_STORE)
+            }
+            if (ch + 2 * gs < nch) {
+                const long chl = ch + 2 * gs;
+                BX_GRAM_P(BX_GRAM_LOAD)
+            }
+            gram_block<T, BQ, NH>(nl, nq, stage + cur * sbuf, lo, qo, acc);
+            __syncthreads();
+            cur ^= 1;
         }
-        gram_block<T, BQ, NH>(nl, nq, stage, lo, qo, acc);
-        __syncthreads();
+    } else {
+        T *dstg = stage;
+        for (long ch = c0; ch < nch; ch += gs) {
+            BX_GRAM_P(BX_GRAM_STORE)
+            __syncthreads();
+            if (ch + gs < nch) {
+                const long chl = ch + gs;
+                BX_GRAM_P(BX_GRAM_LOAD)
+            }
+            gram_block<T, BQ, NH>(nl, nq, stage, lo, qo, acc);
+            __syncthreads();
+        }
     }
-#undef BX_GRAM_Q
 #undef BX_GRAM_P
 #undef BX_GRAM_STORE
 #undef BX_GRAM_LOAD
@@ -434,7 +450,11 @@ void gram_run(int prec, long ld, GramPlan &p, long ldg) {
     const int rc = narrow4 ? 4 * rc0 : (narrow ? 2 * rc0 : rc0);
     const int S = rc + (int)(16 / es);
     const long nchunks = ld / rc;
-    const size_t lds = (size_t)(p.ncols + 16) * S * es + (size_t)p.ncols * sizeof(void *);
+    const int vpc = rc / (int)(16 / es);
+    const bool small = p.ncols * vpc <= 6 * kGTh;  // 6 pieces per thread suffice
+    // the 6-piece kernels keep two stages in LDS
+    const size_t lds = (size_t)(!narrow4 && small ? 2 : 1) * (p.ncols + 16) * S * es +
+                       (size_t)p.ncols * sizeof(void *);
     long parts = (long)num_cus();  // one 8-wave workgroup per CU (register bound)
     if (parts > nchunks) parts = nchunks;
     if (parts < 1) parts = 1;
@@ -444,8 +464,6 @@ void gram_run(int prec, long ld, GramPlan &p, long ldg) {
     a.ld = ld;
     a.part = part;
     BX_REQUIRE(lds <= 160 * 1024, "gram_run: %zu bytes of LDS", lds);
-    const int vpc = rc / (int)(16 / es);
-    const bool small = p.ncols * vpc <= 6 * kGTh;  // 6 pieces per thread suffice
     BX_REQUIRE(!narrow || small, "gram_run: narrow plan of %d columns", p.ncols);
     BX_REQUIRE(!narrow4 || p.ncols * vpc <= 9 * kGTh, "gram_run: narrow plan of %d columns",
                p.ncols);

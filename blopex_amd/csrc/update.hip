@@ -35,20 +35,33 @@ struct UpdKArgs {
     signed char tail_pos[64];  // out_tail column of each output column (-1: not stored)
 };
 
+// U groups of 4 input columns per step: U loads in flight per lane before
+// their MFMAs wait on the first (one at a time leaves the kernel bound by
+// load latency, not bandwidth)
 template <typename T, int NOT>
 __device__ __forceinline__ void upd_block(const T *in, int h, int crow, const T *Cl, long ld,
                                           long r0, int li, int lk,
                                           typename Mfma<T>::acc (&acc)[NOT][4]) {
     constexpr int WT = NOT * 16;
-    for (int s = 0; s < h; s += 4) {
-        const int kc = s + lk;
-        T v[4] = {T(0), T(0), T(0), T(0)};
-        if (kc < h) load4(in + (size_t)kc * ld + r0, v);
+    constexpr int U = 4;
+    for (int s = 0; s < h; s += 4 * U) {
+        T v[U][4];
 #pragma unroll
-        for (int o = 0; o < NOT; o++) {
-            const T cf = (kc < h) ? Cl[(crow + kc) * WT + o * 16 + li] : T(0);
+        for (int u = 0; u < U; u++) {
+            const int kc = s + 4 * u + lk;
 #pragma unroll
-            for (int t = 0; t < 4; t++) acc[o][t] = Mfma<T>::mma(cf, v[t], acc[o][t]);
+            for (int t = 0; t < 4; t++) v[u][t] = T(0);
+            if (kc < h) load4(in + (size_t)kc * ld + r0, v[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int kc = s + 4 * u + lk;
+#pragma unroll
+            for (int o = 0; o < NOT; o++) {
+                const T cf = (kc < h) ? Cl[(crow + kc) * WT + o * 16 + li] : T(0);
+#pragma unroll
+                for (int t = 0; t < 4; t++) acc[o][t] = Mfma<T>::mma(cf, v[u][t], acc[o][t]);
+            }
         }
     }
 }
@@ -117,9 +130,25 @@ void launch_update(const UpdKArgs &a, int ngroups) {
                                    hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024));
         attr = true;
     }
+    // as many workgroups (over all groups) as are resident at once: the loop
+    // over row chunks is persistent, and a second round would run as a tail
+    static size_t occ_lds[8];
+    static int occ_val[8], occ_n = 0;
+    int per_cu = 0;
+    for (int i = 0; i < occ_n; i++)
+        if (occ_lds[i] == lds) per_cu = occ_val[i];
+    if (per_cu == 0) {
+        BX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_update<T, NOT>,
+                                                            kUT, lds));
+        if (per_cu < 1) per_cu = 1;
+        if (occ_n < 8) {
+            occ_lds[occ_n] = lds;
+            occ_val[occ_n++] = per_cu;
+        }
+    }
     const long nchunks = a.ld / 64;
     long blocks = (nchunks + 3) / 4;
-    const long cap = (long)num_cus() * 4;
+    const long cap = (long)num_cus() * per_cu / ngroups;
     if (blocks > cap) blocks = cap;
     if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL((k_update<T, NOT>), dim3((unsigned)blocks, ngroups), dim3(kUT), lds, stream(),
