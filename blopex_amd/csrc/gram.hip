@@ -302,6 +302,19 @@ __global__ void __launch_bounds__(256) k_gram_sum(ReduceArgs a) {
     const size_t stride = (size_t)kGramTiles * 256;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
     int q = 0;
+    // sixteen loads in flight, added in the same order as four at a time
+    for (; q + 16 <= a.nparts; q += 16) {
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) v[u] = src[(size_t)(q + u) * stride];
+#pragma unroll
+        for (int u = 0; u < 16; u += 4) {
+            s0 += v[u];
+            s1 += v[u + 1];
+            s2 += v[u + 2];
+            s3 += v[u + 3];
+        }
+    }
     for (; q + 4 <= a.nparts; q += 4) {
         s0 += src[(size_t)q * stride];
         s1 += src[(size_t)(q + 1) * stride];
@@ -330,9 +343,18 @@ int plan_block(GramPlan &p, const void *blk, int width) {
 // Forms the register blocks from the queued products: for each left block,
 // its 16-column tiles in pairs, against all right tiles of its products in
 // groups of bq.  Returns the number of register blocks (may exceed
-// kGramWaves).
+// kGramWaves).  When filling, blocks are split while fewer than kGramWaves
+// (the largest first: a pair of left tiles into two single ones, else its
+// right tiles into halves), then dealt to the waves so that the two waves of
+// each SIMD (waves w and w + 4) carry about equal numbers of tiles: the
+// kernel is bound by the busiest SIMD's MFMA work.
 int form_blocks(GramPlan &p, long ldg, bool fill, int bq) {
-    int nw = 0;
+    struct Blk {
+        int lb, x0, nl, nq;
+        int q[kGramBQ], y[kGramBQ];  // product and right tile of each column of tiles
+    };
+    Blk blks[2 * kGramWaves];
+    int nb = 0, nw = 0;
     bool done[kGramBlocks] = {};
     for (int q0 = 0; q0 < p.nprod; q0++) {
         const int lb = p.pl[q0];
@@ -349,46 +371,109 @@ int form_blocks(GramPlan &p, long ldg, bool fill, int bq) {
                 nqt++;
             }
         }
-        const int lw = p.bw[lb], lt = (lw + 15) / 16;
+        const int lt = (p.bw[lb] + 15) / 16;
         for (int x0 = 0; x0 < lt; x0 += kGramBL) {
             const int nl = lt - x0 < kGramBL ? lt - x0 : kGramBL;
             for (int j0 = 0; j0 < nqt; j0 += bq) {
                 const int nq = nqt - j0 < bq ? nqt - j0 : bq;
-                if (fill && nw < kGramWaves) {
-                    const int w = nw;
-                    p.wnl[w] = (unsigned char)nl;
-                    p.wnq[w] = (unsigned char)nq;
-                    for (int i = 0; i < kGramBL; i++)
-                        p.wl[w][i] = (short)(p.bc0[lb] + 16 * (x0 + (i < nl ? i : 0)));
-                    for (int j = 0; j < kGramBQ; j++) {
-                        const int jj = j0 + (j < nq ? j : 0);
-                        p.wq[w][j] = (short)(p.bc0[p.pq[qtp[jj]]] + 16 * qty[jj]);
+                if (fill && nb < kGramWaves) {
+                    Blk &b = blks[nb++];
+                    b.lb = lb;
+                    b.x0 = x0;
+                    b.nl = nl;
+                    b.nq = nq;
+                    for (int j = 0; j < nq; j++) {
+                        b.q[j] = qtp[j0 + j];
+                        b.y[j] = qty[j0 + j];
                     }
-                    for (int i = 0; i < kGramBL; i++)
-                        for (int j = 0; j < kGramBQ; j++) {
-                            const int s = (w * kGramBL + i) * kGramBQ + j;
-                            p.dst[s] = p.mdst[s] = nullptr;
-                            p.rows[s] = p.cols[s] = 0;
-                            if (i >= nl || j >= nq) continue;
-                            const int q = qtp[j0 + j], x = x0 + i, y = qty[j0 + j];
-                            const int qw = p.bw[p.pq[q]];
-                            const bool sym = p.pq[q] == lb;
-                            if (sym && y < x) continue;  // stored as the transpose of (y, x)
-                            p.rows[s] = (unsigned char)(lw - 16 * x < 16 ? lw - 16 * x : 16);
-                            p.cols[s] = (unsigned char)(qw - 16 * y < 16 ? qw - 16 * y : 16);
-                            if (p.pt[q]) {  // only the transpose is stored
-                                p.mdst[s] = p.pg[q] + 16 * y + (size_t)16 * x * ldg;
-                                continue;
-                            }
-                            p.dst[s] = p.pg[q] + 16 * x + (size_t)16 * y * ldg;
-                            if (sym && x < y) p.mdst[s] = p.pg[q] + 16 * y + (size_t)16 * x * ldg;
-                        }
                 }
                 nw++;
             }
         }
     }
-    if (fill) p.nwb = nw;
+    if (!fill) return nw;
+    BX_REQUIRE(nw <= kGramWaves, "form_blocks: %d register blocks", nw);
+    // split the largest blocks into the spare waves
+    while (nb < kGramWaves) {
+        int big = -1;
+        for (int i = 0; i < nb; i++)
+            if (blks[i].nl * blks[i].nq > 1 &&
+                (big < 0 || blks[i].nl * blks[i].nq > blks[big].nl * blks[big].nq))
+                big = i;
+        if (big < 0) break;
+        Blk &a = blks[big], &b = blks[nb++];
+        b = a;
+        if (a.nl == 2) {
+            a.nl = b.nl = 1;
+            b.x0 = a.x0 + 1;
+        } else {
+            const int h = (a.nq + 1) / 2;
+            for (int j = h; j < a.nq; j++) {
+                b.q[j - h] = a.q[j];
+                b.y[j - h] = a.y[j];
+            }
+            b.nq = a.nq - h;
+            a.nq = h;
+        }
+    }
+    // largest first, each onto the SIMD with the least work and a free wave
+    int order[kGramWaves], load[4] = {}, used[4] = {}, wave_of[kGramWaves];
+    for (int i = 0; i < nb; i++) order[i] = i;
+    for (int i = 1; i < nb; i++)
+        for (int j = i; j > 0 && blks[order[j]].nl * blks[order[j]].nq >
+                                     blks[order[j - 1]].nl * blks[order[j - 1]].nq; j--)
+            std::swap(order[j], order[j - 1]);
+    for (int t = 0; t < nb; t++) {
+        int best = -1;
+        for (int m = 0; m < 4; m++)
+            if (used[m] < kGramWaves / 4 && (best < 0 || load[m] < load[best])) best = m;
+        const Blk &b = blks[order[t]];
+        wave_of[order[t]] = best + 4 * used[best];
+        used[best]++;
+        load[best] += b.nl * b.nq;
+    }
+    // waves without a block sit idle (nl = nq = 0)
+    int wmax = 0;
+    for (int i = 0; i < nb; i++) wmax = std::max(wmax, wave_of[i] + 1);
+    for (int w = 0; w < wmax; w++) {
+        p.wnl[w] = p.wnq[w] = 0;
+        for (int i = 0; i < kGramBL; i++) p.wl[w][i] = 0;
+        for (int j = 0; j < kGramBQ; j++) p.wq[w][j] = 0;
+        for (int t = 0; t < kSlots; t++) {
+            const int s = w * kSlots + t;
+            p.dst[s] = p.mdst[s] = nullptr;
+            p.rows[s] = p.cols[s] = 0;
+        }
+    }
+    for (int bi = 0; bi < nb; bi++) {
+        const Blk &b = blks[bi];
+        const int w = wave_of[bi], lb = b.lb, lw = p.bw[lb];
+        p.wnl[w] = (unsigned char)b.nl;
+        p.wnq[w] = (unsigned char)b.nq;
+        for (int i = 0; i < kGramBL; i++)
+            p.wl[w][i] = (short)(p.bc0[lb] + 16 * (b.x0 + (i < b.nl ? i : 0)));
+        for (int j = 0; j < kGramBQ; j++) {
+            const int jj = j < b.nq ? j : 0;
+            p.wq[w][j] = (short)(p.bc0[p.pq[b.q[jj]]] + 16 * b.y[jj]);
+        }
+        for (int i = 0; i < b.nl; i++)
+            for (int j = 0; j < b.nq; j++) {
+                const int s = (w * kGramBL + i) * kGramBQ + j;
+                const int q = b.q[j], x = b.x0 + i, y = b.y[j];
+                const int qw = p.bw[p.pq[q]];
+                const bool sym = p.pq[q] == lb;
+                if (sym && y < x) continue;  // stored as the transpose of (y, x)
+                p.rows[s] = (unsigned char)(lw - 16 * x < 16 ? lw - 16 * x : 16);
+                p.cols[s] = (unsigned char)(qw - 16 * y < 16 ? qw - 16 * y : 16);
+                if (p.pt[q]) {  // only the transpose is stored
+                    p.mdst[s] = p.pg[q] + 16 * y + (size_t)16 * x * ldg;
+                    continue;
+                }
+                p.dst[s] = p.pg[q] + 16 * x + (size_t)16 * y * ldg;
+                if (sym && x < y) p.mdst[s] = p.pg[q] + 16 * y + (size_t)16 * x * ldg;
+            }
+    }
+    p.nwb = wmax;
     return nw;
 }
 
