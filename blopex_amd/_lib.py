@@ -28,7 +28,7 @@ class BlasLapack(ctypes.Structure):
 
 class Settings(ctypes.Structure):
     _fields_ = [("mode", c_int), ("switch_factor", c_double), ("stall_iterations", c_int),
-                ("refine_iterations", c_int), ("profile", c_int)]
+                ("refine_iterations", c_int), ("profile", c_int), ("refresh_interval", c_int)]
 
 
 class Stats(ctypes.Structure):

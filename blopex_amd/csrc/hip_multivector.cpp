@@ -19,10 +19,11 @@ namespace {
 hip_LobpcgSettings default_settings() {
     hip_LobpcgSettings s;
     s.mode = HIPLOBPCG_FP64;
-    s.switch_factor = 4.0;
+    s.switch_factor = 1.5;
     s.stall_iterations = 50;
     s.refine_iterations = 0;
     s.profile = 0;
+    s.refresh_interval = 25;
     return s;
 }
 

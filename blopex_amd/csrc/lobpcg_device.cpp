@@ -14,10 +14,13 @@
 // copying back (the reference's W/copy pairs).
 //
 // Precision: FP64 runs everything in FP64.  MIXED iterates with FP32 blocks
-// (FP64 Gram accumulation, FP64 small dense) until the FP32 residuals stop
-// improving or fall below switch_factor * atol (or an FP32 Rayleigh-Ritz
-// step breaks down), then restarts from the last FP32 X in FP64 (re-orthonormalisation and Rayleigh-Ritz in FP64) and
-// finishes there, so the final residuals and Ritz values are FP64 ones.
+// (FP64 Gram accumulation, FP64 small dense; AX and BX recomputed from X
+// every refresh_interval iterations) until the FP32 residuals stop improving
+// or fall below switch_factor * atol (or an FP32 Rayleigh-Ritz step breaks
+// down), then restarts from the last FP32 X in FP64 (re-orthonormalisation
+// and Rayleigh-Ritz in FP64) and finishes there, so the final residuals and
+// Ritz values are FP64 ones.  refine_iterations adds FP64 iterations with
+// every column active after convergence.
 #include <cfloat>
 #include <chrono>
 #include <cmath>
@@ -141,6 +144,7 @@ struct HostOut {
 class Phase {
   public:
     int fp32_exit_ = 0;  // why the FP32 phase ended (hip_LobpcgStats.fp32_exit)
+    int refine_done_ = 0;  // refinement iterations run (hip_LobpcgStats.refine_iterations)
     Phase(int prec, long n, int k, int ny, Op opA, void *Ad, Op opB, void *Bd, Op opT, void *Td,
           const hip_Multi_Vector *Yuser, lobpcg_Tolerance tol, int verb, HostOut out, KTimer *tm)
         : prec_(prec), n_(n), k_(k), ny_(ny), opA_(opA), Ad_(Ad), opB_(opB), Bd_(Bd), opT_(opT),
@@ -424,6 +428,11 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
     std::vector<int> mask(k, 1), act, prev_act(k);
     for (int i = 0; i < k; i++) prev_act[i] = i;
     int local_it = 0;
+    // FP64: refine_iterations more iterations with every column active once
+    // all have converged (the first of them recomputes every residual and
+    // starts without P)
+    int refine_left = fp32_phase ? 0 : set.refine_iterations;
+    bool refining = false;
     std::vector<double> gm_hist;  // geometric mean of the k residual norms, per FP32 iteration
     int it = *it_io + 1;
     const int explicit_x = prec_ == HIPMV_FP32 ? 1 : 0;
@@ -433,7 +442,7 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
             mask[i] = out_.res[i] > fabs(out_.lam[i]) * tol_.relative + tol_.absolute + DBL_EPSILON;
             if (mask[i]) act.push_back(i);
         }
-        const int nr = (int)act.size();
+        int nr = (int)act.size();
         if (fp32_phase) {
             // leave FP32 on convergence, near the tolerance, or on a stall
             // (LOBPCG residuals are not monotone: early on they rise for tens
@@ -456,6 +465,22 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
                 break;
             }
         }
+        if (refining || (nr < 1 && flag == 0 && refine_left > 0)) {
+            if (refine_left == 0) break;
+            act.resize(k);
+            for (int i = 0; i < k; i++) act[i] = i;
+            if (!refining) {
+                refining = true;
+                tm_->begin(kOther);
+                residuals(noB_ ? X_ : BX_, colmap_identity(k), lam_);
+                tm_->end();
+                prev_act = act;
+                local_it = 0;
+            }
+            refine_left--;
+            refine_done_++;
+            nr = k;
+        }
         if (nr < 1) break;
         // R, P, AP and BP are laid out by prev_act (the update writes P' only
         // for the columns active when it ran); compact them when columns
@@ -475,6 +500,16 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
                 if (!noB_) gather(BP_, src);
             }
             tm_->end();
+        }
+        // FP32: AX (and BX) follow X by the update recurrence, whose rounding
+        // accumulates; recomputing them from X every refresh_interval
+        // iterations keeps the computed residuals those of X ("residual
+        // replacement"), so FP32 gets close enough to the tolerance that the
+        // FP64 phase needs only a few iterations
+        if (fp32_phase && set.refresh_interval > 0 && local_it > 0 &&
+            local_it % set.refresh_interval == 0) {
+            apply(opA_, Ad_, X_, AX_, k);
+            if (!noB_) apply(opB_, Bd_, X_, BX_, k);
         }
         if (!noT_ && !fuseT_) {
             apply(opT_, Td_, R_, W_, nr);
@@ -651,6 +686,7 @@ int device_lobpcg(mv_MultiVectorPtr X, void *Ad, Op opA, void *Bd, Op opB, void 
         bool dummy;
         flag = ph.run(x->data, &it, maxit, false, set, &dummy);
         x->stats.iterations_fp64 = it - it0;
+        x->stats.refine_iterations = ph.refine_done_;
     }
     *iters = it;
     if (verb == 1) {

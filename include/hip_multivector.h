@@ -43,7 +43,7 @@ extern "C" {
 typedef struct hip_LobpcgSettings {
     int mode;                 /* HIPLOBPCG_FP64 (default) or HIPLOBPCG_MIXED */
     double switch_factor;     /* MIXED: leave FP32 once every active residual is below
-                                 switch_factor * atol, or FP32 has stalled (default 4) */
+                                 switch_factor * atol, or FP32 has stalled (default 1.5) */
     int stall_iterations;     /* MIXED: window W; FP32 has stalled when the smallest geometric
                                  mean of the k residual norms over the last W iterations is
                                  not 2% below that of the W before (default 50) */
@@ -51,6 +51,9 @@ typedef struct hip_LobpcgSettings {
                                  (default 0) */
     int profile;              /* 1: time kernel classes with HIP events (adds event records
                                  to the stream; default 0) */
+    int refresh_interval;     /* MIXED: FP32 iterations between recomputations of AX (and BX)
+                                 from X, which keep the FP32 residuals those of X; 0: never
+                                 (default 25) */
 } hip_LobpcgSettings;
 
 typedef struct hip_LobpcgStats {

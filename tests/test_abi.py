@@ -54,6 +54,7 @@ def test_struct_layouts():
     # hip_Multi_Vector: data, size, ld, 2 ints, ptr, 2 ints, settings(32), stats(80)
     assert L.HipMultiVector.settings.offset == 48
     assert ctypes.sizeof(L.Settings) == 32 and ctypes.sizeof(L.Stats) == 80
+    assert L.Settings.refresh_interval.offset == 28
     assert ctypes.sizeof(L.HipMultiVector) == 48 + 32 + 80
     # hip_CSRMatrix: 6 words, ndiag + 16 offsets (68 bytes, padded to 72), diag_ld, 2 pointers,
     # grid[3] (padded to 16)

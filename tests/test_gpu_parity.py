@@ -375,13 +375,37 @@ def test_device_x0_from_seed_matches_host_x0():
     np.testing.assert_array_equal(r1.lam_hist, r2.lam_hist)
 
 
-def test_mixed_precision_small():
-    say("mixed precision cfg1-like (16^3, k=8, jacobi)")
+@pytest.mark.parametrize("refresh", [25, 7, 0])
+def test_mixed_precision_small(refresh):
+    """MIXED mode (FP32 phase with AX recomputed every `refresh` iterations,
+    0: never) meets the tolerance with FP64 residuals and the oracle's
+    eigenvalues."""
+    say(f"mixed precision cfg1-like (16^3, k=8, jacobi), refresh {refresh}")
     A = pb.laplacian_7pt(16, 16, 16, sigma=1.0, seed=5)
     X0 = po.drand48_block(A.n, 8, 1)
     o = po.solve(A=A, X0=X0, tdiag=1.0 / A.diagonal(), maxit=400)
-    r = bl.lobpcg(X0, A, T="jacobi", maxit=400, mode="mixed")
+    r = bl.lobpcg(X0, A, T="jacobi", maxit=400, mode="mixed", refresh_interval=refresh)
     assert r.flag == 0 and o.flag == 0
     assert r.stats["iterations_fp32"] > 0
     assert np.all(r.res <= 1e-6 + 1e-15)
     assert relerr(r.lam, o.lam) < 1e-8
+
+
+def test_refinement_iterations():
+    """refine_iterations: after convergence, that many more FP64 iterations
+    with every column active; the residuals fall below those at convergence
+    and the eigenvalues approach the closed form (12^3 Dirichlet Laplacian)."""
+    say("FP64 refinement iterations (12^3, k=6)")
+    A = pb.laplacian_7pt(12, 12, 12)
+    X0 = po.drand48_block(A.n, 6, 1)
+    h = 1.0 / 13.0
+    one = [2.0 - 2.0 * np.cos(i * np.pi * h) for i in range(1, 13)]
+    exact = np.sort(np.add.outer(np.add.outer(one, one), one).ravel())[:6]
+    r0 = bl.lobpcg(X0, A, maxit=300)
+    r1 = bl.lobpcg(X0, A, maxit=300, refine_iterations=8)
+    assert r0.flag == 0 and r1.flag == 0
+    assert r1.stats["refine_iterations"] == 8 and r0.stats["refine_iterations"] == 0
+    assert r1.iterations == r0.iterations + 8
+    assert np.max(r1.res) < 0.5 * np.max(r0.res)  # no preconditioner: a slow rate
+    e0, e1 = relerr(r0.lam, exact), relerr(r1.lam, exact)
+    assert e1 < e0
