@@ -39,6 +39,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 COUNTERS = os.path.join(ROOT, "profiles", "spmm_roofline_128cube_k32_counters.txt")
+MFMA_SUMMARY = os.path.join(ROOT, "profiles", "lobpcg_mixed_128cube_k32_mfma.txt")
+GRAM_KERNEL = "k_gram<float, 6, 3, 64>"  # the Gram plan of the FP32 iterations (up to 192 columns)
 METRIC = ("time-to-residual 1e-6, k=32, 128³ Laplacian: mixed vs FP64 on GPU; SpMM GB/s")
 HBM_PEAK_GBS = 8000.0
 
@@ -300,6 +302,14 @@ def main():
                                        "frac": ach / HBM_PEAK_GBS}
     out["spmm_fp64"] = {"achieved": algo64 / (ms64 * 1e-3) / 1e9, "avg_launch_us": ms64 * 1e3,
                         "algorithmic_bytes": algo64}
+    # MFMA rate of the FP32 Gram kernel in the mixed solve, from the committed
+    # rocprofv3 summary (MOPS counters x 512 over the kernel-trace durations)
+    from blopex_amd.profsum import read_mfma
+    gm = read_mfma(MFMA_SUMMARY, GRAM_KERNEL) if (nside, k) == (128, 32) else None
+    if gm:
+        out["gram_mfma"] = {"kernel": GRAM_KERNEL, "bound": "mfma", "achieved": gm[0],
+                            "peak": gm[1], "unit": "TFLOP/s", "frac": gm[0] / gm[1],
+                            "source": os.path.relpath(MFMA_SUMMARY, ROOT)}
 
     if not args.no_cpu:
         per_it, sample = cpu_baseline(A, nside, k)

@@ -2,6 +2,7 @@
 
     python3 -m blopex_amd.profsum stats  <trace-dir> [--title T] [--command C]...
     python3 -m blopex_amd.profsum counters <fetch-dir> <write-dir> [--title T] [--command C]...
+    python3 -m blopex_amd.profsum mfma <counter-dir> <trace-dir> [--title T] [--command C]...
 
 `stats` reads `*_kernel_stats.csv` (rocprofv3 --kernel-trace --stats) and
 prints the per-kernel table.  `counters` reads the `*_counter_collection.csv`
@@ -96,6 +97,57 @@ def counters(fetch_dir: str, write_dir: str) -> tuple[list[str], dict[str, float
     return out, traffic
 
 
+MFMA_COUNTERS = ("SQ_INSTS_VALU_MFMA_MOPS_F32", "SQ_INSTS_VALU_MFMA_MOPS_F64")
+PEAK_TFLOPS = {"F32": 157.3, "F64": 78.6}   # MI355X dense MFMA
+
+
+def mfma(counter_dir: str, trace_dir: str) -> tuple[list[str], dict[str, tuple[float, float]]]:
+    """MFMA flops per dispatch (MOPS x 512, whole GPU) per kernel from one
+    counter pass, and achieved TFLOP/s against the kernel-trace durations of
+    the same program (FP32 or FP64 peak, whichever the kernel mostly runs)."""
+    path = _one(os.path.join(counter_dir, "**", "*counter_collection.csv"))
+    per: dict[str, dict[str, list[float]]] = collections.defaultdict(
+        lambda: collections.defaultdict(list))
+    for r in csv.DictReader(open(path)):
+        if r["Counter_Name"] in MFMA_COUNTERS:
+            per[short_name(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    tpath = _one(os.path.join(trace_dir, "**", "*kernel_stats.csv"))
+    avg_ns = {short_name(r["Name"]): float(r["AverageNs"]) for r in csv.DictReader(open(tpath))}
+    mean = lambda v: sum(v) / len(v) if v else 0.0
+    out = [f"{'kernel':<40} {'disp':>5} {'GFLOP/disp':>11} {'avg us':>8} {'TFLOP/s':>8} "
+           f"{'peak':>6} {'frac':>6}"]
+    res = {}
+    for name, c in sorted(per.items()):
+        f32 = mean(c["SQ_INSTS_VALU_MFMA_MOPS_F32"]) * 512
+        f64 = mean(c["SQ_INSTS_VALU_MFMA_MOPS_F64"]) * 512
+        flops = f32 + f64
+        if flops == 0 or name not in avg_ns:
+            continue
+        kind = "F64" if f64 > f32 else "F32"
+        tf = flops / avg_ns[name] / 1e3
+        if tf < 0.05:
+            continue  # a few MFMA tiles in a latency-bound kernel
+        res[name] = (tf, PEAK_TFLOPS[kind])
+        out.append(f"{name[:40]:<40} {len(c['SQ_INSTS_VALU_MFMA_MOPS_F32']):>5} {flops / 1e9:>11.3f} "
+                   f"{avg_ns[name] / 1e3:>8.1f} {tf:>8.1f} {PEAK_TFLOPS[kind]:>6.1f} "
+                   f"{tf / PEAK_TFLOPS[kind]:>6.3f}")
+    out.append("")
+    out += [f"mfma {name} {tf:.2f} {peak:.1f}" for name, (tf, peak) in res.items()]
+    return out, res
+
+
+def read_mfma(path: str, kernel: str) -> tuple[float, float] | None:
+    """The `mfma <kernel> <TFLOP/s> <peak>` figures of a committed summary, or None."""
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("mfma "):
+            head, tf, peak = line[5:].rsplit(" ", 2)
+            if head == kernel:
+                return float(tf), float(peak)
+    return None
+
+
 def read_traffic(path: str, kernel: str) -> float | None:
     """The `traffic <kernel> <bytes>` figure of a committed counter summary, or None."""
     if not os.path.exists(path):
@@ -108,7 +160,7 @@ def read_traffic(path: str, kernel: str) -> float | None:
 
 def main(argv=None) -> None:
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["stats", "counters"])
+    ap.add_argument("what", choices=["stats", "counters", "mfma"])
     ap.add_argument("dirs", nargs="+")
     ap.add_argument("--title", default="")
     ap.add_argument("--command", action="append", default=[])
@@ -121,6 +173,10 @@ def main(argv=None) -> None:
         head.append("")
     if a.what == "stats":
         body = stats(a.dirs[0])
+    elif a.what == "mfma":
+        if len(a.dirs) != 2:
+            raise SystemExit("mfma needs <counter-dir> <trace-dir>")
+        body, _ = mfma(a.dirs[0], a.dirs[1])
     else:
         if len(a.dirs) != 2:
             raise SystemExit("counters needs <fetch-dir> <write-dir>")
