@@ -143,6 +143,8 @@ def main():
     ap.add_argument("--maxit", type=int, default=6000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-spread", action="store_true",
+                    help="skip the five extra mixed solves with other refresh intervals")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the SpMM roofline launches and a device copy of known size "
                          "(the command the committed rocprofv3 summaries and counters use)")
@@ -176,10 +178,10 @@ def main():
                                    ctypes.c_void_p]
     xdata = lambda b: L.HipMultiVector.from_address(ctypes.addressof(b.ptr.contents)).data
 
-    def solve(mode, profile=False):
+    def solve(mode, profile=False, **settings):
         hip.hipMemcpyAsync(xdata(X), xdata(X0), nbytes, 3, lib.blopex_amd_stream())
         return solve_device(dA, X, T=dT, tol=1e-6, maxit=args.maxit, mode=mode, profile=profile,
-                            histories=False)
+                            histories=False, **settings)
 
     def log(msg):
         print(f"# {msg}", file=sys.stderr, flush=True)
@@ -243,6 +245,28 @@ def main():
                      "iterations_fp32": rm.stats["iterations_fp32"],
                      "iterations_fp64": rm.stats["iterations_fp64"],
                      "max_residual": float(np.max(rm.res))}}
+
+    if not args.no_spread:
+        # The iteration count moves with rounding (a cluster of six eigenvalues
+        # at the top of the block converges last): the same solve with other
+        # residual-replacement intervals perturbs the trajectory and shows the
+        # spread of the count at an unchanged cost per iteration.
+        sp = []
+        for ri in (15, 25, 30, 35, 40):
+            t0 = time.perf_counter()
+            rs = solve("mixed", refresh_interval=ri)
+            lib.blopex_amd_synchronize()
+            sp.append((ri, rs.iterations, time.perf_counter() - t0))
+            log(f"spread: refresh {ri}: iterations {rs.iterations} {sp[-1][2]:.3f}s")
+        import inspect
+        ri0 = inspect.signature(solve_device).parameters["refresh_interval"].default
+        sp.append((ri0, rm.iterations, float(np.mean(times))))
+        its_all = [v[1] for v in sp]
+        secs_all = [v[2] for v in sp]
+        out["mixed_spread"] = {"refresh_intervals": [v[0] for v in sp], "iterations": its_all,
+                               "seconds": secs_all, "mean_seconds": float(np.mean(secs_all)),
+                               "mean_iterations": float(np.mean(its_all)),
+                               "ms_per_iteration": 1e3 * float(np.sum(secs_all)) / float(np.sum(its_all))}
 
     if not args.no_fp64:
         t0 = time.perf_counter()

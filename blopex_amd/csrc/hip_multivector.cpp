@@ -23,7 +23,7 @@ hip_LobpcgSettings default_settings() {
     s.stall_iterations = 50;
     s.refine_iterations = 0;
     s.profile = 0;
-    s.refresh_interval = 25;
+    s.refresh_interval = 20;
     return s;
 }
 

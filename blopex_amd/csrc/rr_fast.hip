@@ -15,7 +15,8 @@
 //   C = Q T Q' (Householder with C in registers, two barriers per column),
 //   eigenvalues of T by multisection on the three-term Sturm recurrence,
 //   eigenvectors by inverse iteration (stored L D L' pivots), re-orthogonalised
-//   inside eigenvalue clusters (relative gap 1e-3, as LAPACK dstein),
+//   inside eigenvalue clusters (relative gap 1e-3, as LAPACK dstein; 1e-6
+//   for the explicit-X pencils of FP32 iterates),
 //   back-transformed through Q and W; coef = S Y.
 // Failure codes follow the reference: R not positive definite -> r_info and
 // abort (lobpcg.c:1031-1040), P not positive definite -> P dropped for this
@@ -886,7 +887,10 @@ __global__ void __launch_bounds__(kT) k_rr_tri(RR2Args r) {
     // first removed more than three quarters of |z|^2 (twice is enough),
     // and the last pass also normalises.
     {
-        const double ortol = 1e-3 * tnorm;
+        // With explicit X blocks (FP32 iterates) the next Gram measures X'BX
+        // anyway, so the Ritz vectors need orthogonality well below FP32
+        // rounding only: eps |T| / gap <= 2e-10 for the clusters below.
+        const double ortol = (r.explicit_x ? 1e-6 : 1e-3) * tnorm;
         for (int c = 1; c < k; c++) {
             int c0 = c;
             while (c0 > 0 && lamk[c0] - lamk[c0 - 1] <= ortol) c0--;

@@ -100,7 +100,7 @@ def solve_device(A: DeviceCSR, X: DeviceBlock, B=None, T=None, Y: DeviceBlock | 
                  tol: float = 1e-6, rtol: float = 1e-50, maxit: int = 200, verbosity: int = 0,
                  mode: str = "fp64", switch_factor: float = 1.5, stall_iterations: int = 50,
                  refine_iterations: int = 0, histories: bool = True, profile: bool = False,
-                 refresh_interval: int = 25) -> Result:
+                 refresh_interval: int = 20) -> Result:
     """lobpcg_solve_double on device-resident operands; X is overwritten."""
     lib = L.load()
     k = X.k

@@ -53,7 +53,7 @@ typedef struct hip_LobpcgSettings {
                                  to the stream; default 0) */
     int refresh_interval;     /* MIXED: FP32 iterations between recomputations of AX (and BX)
                                  from X, which keep the FP32 residuals those of X; 0: never
-                                 (default 25) */
+                                 (default 20) */
 } hip_LobpcgSettings;
 
 typedef struct hip_LobpcgStats {
