@@ -434,6 +434,12 @@ void diagdot_cols(int prec, long n, const void *x, long ldx, const ColMap &xm, c
     BX_HIP(hipGetLastError());
 }
 
+void row_sums(const double *part, int nparts, int ncols, double *out) {
+    if (ncols == 0) return;
+    hipLaunchKernelGGL(k_rowsum, dim3(ncols), dim3(64), 0, stream(), part, nparts, out);
+    BX_HIP(hipGetLastError());
+}
+
 void vecmat_cols(int prec, long n, const void *x, long ldx, const ColMap &xm, const double *c,
                  long ldc, void *y, long ldy, const ColMap &ym, bool accumulate) {
     if (ym.n == 0 || n == 0) return;

@@ -579,13 +579,24 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
         }
         std::vector<int> tail_pos(k, -1);  // P' kept for the active columns only
         for (int t = 0; t < nr; t++) tail_pos[act[t]] = t;
-        update(g, ng, coef_, k + nr + preq, k, tail_pos.data());
+        if (noB_ && prec_ == HIPMV_FP32 && k <= 32) {
+            // FP32, B = I: the residual of the new block is formed as it is
+            // updated (R = T (X' lam - AX') and its norms, for the active columns)
+            const float *t32 = fuseT_ ? ((const hip_DiagMatrix *)Td_)->d32 : nullptr;
+            tm_->begin(kUpdate);
+            block_update_resid(ld_, g, coef_, ld3_, k + nr + preq, k, st_, tail_pos.data(), nr,
+                               lam_new_, t32, (float *)R_, rn2_);
+            tm_->end();
+        } else {
+            update(g, ng, coef_, k + nr + preq, k, tail_pos.data());
+            BXp = noB_ ? X_ : BX_;
+            ColMap am = colmap_identity(nr);
+            for (int t = 0; t < nr; t++) am.idx[t] = act[t];
+            tm_->begin(kOther);
+            residuals(BXp, am, lam_new_);
+            tm_->end();
+        }
         BXp = noB_ ? X_ : BX_;
-        ColMap am = colmap_identity(nr);
-        for (int t = 0; t < nr; t++) am.idx[t] = act[t];
-        tm_->begin(kOther);
-        residuals(BXp, am, lam_new_);
-        tm_->end();
         read_tail(tail, &hs);
         if (hs.abort && fp32_phase) {
             // an FP32 breakdown ends the FP32 phase; X is still the last update

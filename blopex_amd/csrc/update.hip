@@ -15,6 +15,8 @@
 // so C comes in as the A operand (from LDS) and the block rows as B.
 // Rows a wave owns are read completely before they are written, so every
 // output may alias the input of the same group it replaces.
+#include "dense.h"
+#include "dpp.h"
 #include "kernels.h"
 #include "mfma.h"
 
@@ -156,6 +158,115 @@ void launch_update(const UpdKArgs &a, int ngroups) {
     BX_HIP(hipGetLastError());
 }
 
+// The FP32 block update of the X and AX groups (B = I) with the residual of
+// the new block folded in: one wave computes AX' and then X' for its 64 rows
+// (both groups' accumulators live at once), stores them and P', AP' as
+// k_update does, and forms R = T (X' diag(lam) - AX') for the columns that
+// keep a tail position, with the squared norms of X' lam - AX' summed per
+// workgroup (FP64) into part[tail column][workgroup].  Saves the residual
+// pass's reads of X' and AX'.
+struct UpdRArgs {
+    UpdKArgs u;         // g[0] = X group, g[1] = AX group
+    const double *lam;  // new Ritz values, by output column
+    const float *tdiag; // diagonal preconditioner or null
+    float *r;           // residual block (tail columns), leading dimension u.ld
+    double *part;       // [tail column][nparts]
+    int nparts;         // workgroups in the grid
+};
+
+__global__ void __launch_bounds__(kUT) k_update_resid(UpdRArgs a) {
+    constexpr int NOT = 2;
+    extern __shared__ unsigned char smem_raw[];
+    float *Cl = reinterpret_cast<float *>(smem_raw);
+    constexpr int WT = NOT * 16;
+    const UpdKArgs &u = a.u;
+    if (u.st && u.st->abort) return;
+    for (int e = threadIdx.x; e < u.crows * WT; e += kUT) {
+        const int r = e / WT, c = e % WT;
+        Cl[e] = (c < u.w) ? (float)u.C[r + (size_t)c * u.ldc] : 0.f;
+    }
+    __syncthreads();
+    // tail positions and Ritz values of the output columns, read where used
+    __shared__ int tposl[WT];
+    __shared__ float laml[WT];
+    for (int c = threadIdx.x; c < WT; c += kUT) {
+        tposl[c] = c < u.w ? (int)u.tail_pos[c] : -1;
+        laml[c] = c < u.w ? (float)a.lam[c] : 0.f;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const long nchunks = u.ld / 64;
+    int pfull[NOT][4], ptail[NOT][4];
+    double ss[NOT][4];
+#pragma unroll
+    for (int o = 0; o < NOT; o++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int c = o * 16 + Mfma<float>::drow(lk, r);
+            pfull[o][r] = c < u.w ? c : -1;
+            ptail[o][r] = tposl[c];
+            ss[o][r] = 0.0;
+        }
+    const UpdGroup &gx = u.g[0], &ga = u.g[1];
+    for (long ch = (long)blockIdx.x * 4 + wave; ch < nchunks; ch += (long)gridDim.x * 4) {
+        const long r0 = ch * 64 + 4 * li;
+        typename Mfma<float>::acc ax[NOT][4], xx[NOT][4];
+#pragma unroll
+        for (int o = 0; o < NOT; o++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) ax[o][t] = xx[o][t] = typename Mfma<float>::acc{};
+        // AX' (with AP' on the way), then X' (with P')
+        if (ga.in[1]) upd_block<float, NOT>((const float *)ga.in[1], ga.h[1], ga.crow[1], Cl, u.ld, r0, li, lk, ax);
+        if (ga.in[2]) upd_block<float, NOT>((const float *)ga.in[2], ga.h[2], ga.crow[2], Cl, u.ld, r0, li, lk, ax);
+        if (ga.out_tail) upd_store<float, NOT>((float *)ga.out_tail, u.ld, r0, ptail, ax);
+        upd_block<float, NOT>((const float *)ga.in[0], ga.h[0], ga.crow[0], Cl, u.ld, r0, li, lk, ax);
+        upd_store<float, NOT>((float *)ga.out_full, u.ld, r0, pfull, ax);
+        if (gx.in[1]) upd_block<float, NOT>((const float *)gx.in[1], gx.h[1], gx.crow[1], Cl, u.ld, r0, li, lk, xx);
+        if (gx.in[2]) upd_block<float, NOT>((const float *)gx.in[2], gx.h[2], gx.crow[2], Cl, u.ld, r0, li, lk, xx);
+        if (gx.out_tail) upd_store<float, NOT>((float *)gx.out_tail, u.ld, r0, ptail, xx);
+        upd_block<float, NOT>((const float *)gx.in[0], gx.h[0], gx.crow[0], Cl, u.ld, r0, li, lk, xx);
+        upd_store<float, NOT>((float *)gx.out_full, u.ld, r0, pfull, xx);
+        // R = T (X' lam - AX') for the tail columns, as the residual kernel forms it
+        float td[4] = {1.f, 1.f, 1.f, 1.f};
+        if (a.tdiag) load4(a.tdiag + r0, td);
+#pragma unroll
+        for (int o = 0; o < NOT; o++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int q = ptail[o][r];
+                if (q < 0) continue;
+                const float lc = laml[o * 16 + Mfma<float>::drow(lk, r)];
+                float v[4];
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    v[t] = lc * xx[o][t][r] - ax[o][t][r];
+                    ss[o][r] = fma((double)v[t], (double)v[t], ss[o][r]);
+                }
+                if (a.tdiag)
+                    store4(a.r + (size_t)q * u.ld + r0, td[0] * v[0], td[1] * v[1], td[2] * v[2], td[3] * v[3]);
+                else
+                    store4(a.r + (size_t)q * u.ld + r0, v[0], v[1], v[2], v[3]);
+            }
+    }
+    // norms: sum over the 16 lanes that share the output columns, then over
+    // the four waves in order; one partial per workgroup
+    __shared__ double wsum[4][WT];
+#pragma unroll
+    for (int o = 0; o < NOT; o++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const double t = group_sum16(ss[o][r]);
+            if (li == 0) wsum[wave][o * 16 + Mfma<float>::drow(lk, r)] = t;
+        }
+    __syncthreads();
+    for (int c = threadIdx.x; c < WT; c += kUT) {
+        const int q = tposl[c];
+        if (q >= 0)
+            a.part[(size_t)q * a.nparts + blockIdx.x] = (wsum[0][c] + wsum[1][c]) + (wsum[2][c] + wsum[3][c]);
+    }
+}
+
 // D = A B with A[i][0] = i (other k-slots 0), B[0][j] = 1: D[i][j] = i;
 // then A[i][0] = 1, B[0][j] = j: D[i][j] = j.
 template <typename T>
@@ -183,6 +294,65 @@ void mfma_layout_probe(int prec, int *rows, int *cols) {
     download_sync(rows, d, 256 * sizeof(int));
     download_sync(cols, d + 256, 256 * sizeof(int));
     BX_HIP(hipFree(d));
+}
+
+void block_update_resid(long ld, const UpdGroup *groups, const double *C, int ldc, int crows,
+                        int w, const DevStatus *st, const int *tail_pos, int ntail,
+                        const double *lam, const float *tdiag, float *r, double *rn2) {
+    BX_REQUIRE(ld % 64 == 0, "block_update_resid: leading dimension %ld not a multiple of 64", ld);
+    BX_REQUIRE(w >= 1 && w <= 32, "block_update_resid: %d output columns (max 32)", w);
+    UpdRArgs a;
+    for (int i = 0; i < 2; i++) {
+        a.u.g[i] = groups[i];
+        BX_REQUIRE(groups[i].in[0] && groups[i].out_full, "block_update_resid: group %d", i);
+        for (int b = 0; b < 3; b++) {
+            if (!a.u.g[i].in[b]) continue;
+            BX_REQUIRE(a.u.g[i].crow[b] >= 0 && a.u.g[i].crow[b] + a.u.g[i].h[b] <= crows,
+                       "block_update_resid: coefficient rows %d..%d outside %d", a.u.g[i].crow[b],
+                       a.u.g[i].crow[b] + a.u.g[i].h[b], crows);
+        }
+    }
+    a.u.C = C;
+    a.u.ldc = ldc;
+    a.u.crows = crows;
+    a.u.w = w;
+    a.u.ld = ld;
+    a.u.st = st;
+    for (int c = 0; c < 64; c++) {
+        const int q = c < w ? tail_pos[c] : -1;
+        BX_REQUIRE(q >= -1 && q < ntail, "block_update_resid: tail position %d of column %d", q, c);
+        a.u.tail_pos[c] = (signed char)q;
+    }
+    a.lam = lam;
+    a.tdiag = tdiag;
+    a.r = r;
+    const size_t lds = (size_t)crows * 32 * sizeof(float);
+    BX_REQUIRE(crows <= kMaxRR, "block_update_resid: %d coefficient rows", crows);
+    static bool attr = false;
+    static int per_cu = 0;
+    if (!attr) {
+        // the coefficients (at most kMaxRR rows of 32 floats); static LDS comes on top
+        BX_HIP(hipFuncSetAttribute((const void *)k_update_resid,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)((size_t)kMaxRR * 32 * sizeof(float))));
+        attr = true;
+    }
+    if (per_cu == 0) {
+        // the resident workgroups for the largest pencil (LDS never limits it)
+        BX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_update_resid,
+                                                            kUT, (size_t)kMaxRR * 32 * sizeof(float)));
+        if (per_cu < 1) per_cu = 1;
+    }
+    const long nchunks = ld / 64;
+    long blocks = (nchunks + 3) / 4;
+    const long cap = (long)num_cus() * per_cu;
+    if (blocks > cap) blocks = cap;
+    if (blocks < 1) blocks = 1;
+    a.nparts = (int)blocks;
+    a.part = (double *)scratch(1, (size_t)a.nparts * ntail * sizeof(double));
+    hipLaunchKernelGGL(k_update_resid, dim3((unsigned)blocks), dim3(kUT), lds, stream(), a);
+    BX_HIP(hipGetLastError());
+    row_sums(a.part, a.nparts, ntail, rn2);
 }
 
 void block_update(int prec, long ld, const UpdGroup *groups, int ngroups, const double *C, int ldc,

@@ -391,6 +391,22 @@ def test_mixed_precision_small(refresh):
     assert relerr(r.lam, o.lam) < 1e-8
 
 
+@pytest.mark.parametrize("B", [None, "diag"])
+def test_mixed_precision_without_preconditioner(B):
+    """MIXED mode without T: with B = I the FP32 update forms the residual
+    itself (no T to apply); with a diagonal B the separate residual pass runs."""
+    say(f"mixed precision, no preconditioner, B={B}")
+    A = pb.laplacian_7pt(10, 10, 10, sigma=1.0, seed=3)
+    X0 = po.drand48_block(A.n, 5, 1)
+    Bd = None if B is None else 0.5 + np.random.default_rng(7).random(A.n)
+    o = po.solve(A=A, X0=X0, bdiag=Bd, maxit=800)
+    r = bl.lobpcg(X0, A, B=Bd, maxit=800, mode="mixed")
+    assert r.flag == 0 and o.flag == 0
+    assert r.stats["iterations_fp32"] > 0
+    assert np.all(r.res <= 1e-6 + 1e-15)
+    assert relerr(r.lam, o.lam) < 1e-8
+
+
 def test_refinement_iterations():
     """refine_iterations: after convergence, that many more FP64 iterations
     with every column active; the residuals fall below those at convergence
