@@ -28,7 +28,8 @@ class BlasLapack(ctypes.Structure):
 
 class Settings(ctypes.Structure):
     _fields_ = [("mode", c_int), ("switch_factor", c_double), ("stall_iterations", c_int),
-                ("refine_iterations", c_int), ("profile", c_int), ("refresh_interval", c_int)]
+                ("refine_iterations", c_int), ("profile", c_int), ("refresh_interval", c_int),
+                ("final_rr", c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -36,7 +37,7 @@ class Stats(ctypes.Structure):
                 ("refine_iterations", c_int), ("exit_flag", c_int), ("seconds_small_dense", c_double),
                 ("seconds_spmm", c_double), ("seconds_gram", c_double), ("seconds_update", c_double),
                 ("seconds_other", c_double), ("spmm_bytes", c_double), ("spmm_calls", c_int),
-                ("fp32_exit", c_int)]
+                ("fp32_exit", c_int), ("final_rr", c_int)]
 
 
 class HipMultiVector(ctypes.Structure):

@@ -24,6 +24,7 @@ hip_LobpcgSettings default_settings() {
     s.refine_iterations = 0;
     s.profile = 0;
     s.refresh_interval = 20;
+    s.final_rr = 0;
     return s;
 }
 

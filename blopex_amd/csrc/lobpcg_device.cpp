@@ -67,6 +67,11 @@ hip_Multi_Vector view(void *p, long n, int w, int prec) {
 // of launches, resolved after the stream has been synchronised.
 enum KClass { kSpmm, kGram, kUpdate, kDense, kOther, kNumClasses };
 
+// residual level below which FP64 iterations measure X'AX and X'BX.
+// lobpcg.m switches at eps^0.6 (4e-10); here the implicit blocks already broke
+// a 10^3 Jacobi solve down near 1e-10, so the switch comes at eps^0.5 (1.5e-8)
+const double kExplicitGramRes = sqrt(DBL_EPSILON);
+
 class KTimer {
   public:
     explicit KTimer(bool on) : on_(on) {}
@@ -145,6 +150,7 @@ class Phase {
   public:
     int fp32_exit_ = 0;  // why the FP32 phase ended (hip_LobpcgStats.fp32_exit)
     int refine_done_ = 0;  // refinement iterations run (hip_LobpcgStats.refine_iterations)
+    int final_rr_done_ = 0;  // 1 once the final Rayleigh-Ritz ran
     Phase(int prec, long n, int k, int ny, Op opA, void *Ad, Op opB, void *Bd, Op opT, void *Td,
           const hip_Multi_Vector *Yuser, lobpcg_Tolerance tol, int verb, HostOut out, KTimer *tm)
         : prec_(prec), n_(n), k_(k), ny_(ny), opA_(opA), Ad_(Ad), opB_(opB), Bd_(Bd), opT_(opT),
@@ -435,7 +441,6 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
     bool refining = false;
     std::vector<double> gm_hist;  // geometric mean of the k residual norms, per FP32 iteration
     int it = *it_io + 1;
-    const int explicit_x = prec_ == HIPMV_FP32 ? 1 : 0;
     for (; flag == 0 && it <= maxit; it++) {
         act.clear();
         for (int i = 0; i < k; i++) {
@@ -520,6 +525,12 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
         if (!noB_) apply(opB_, Bd_, R_, BR_, nr);
         apply(opA_, Ad_, R_, AR_, nr);
         const int preq = local_it > 0 ? nr : 0;
+        // explicit X'AX, X'BX: always for FP32 iterates; in FP64 once every
+        // residual is below kExplicitGramRes, where the implicit blocks'
+        // rounding would dominate (lobpcg.m:692-696).  With them, a pencil that is not
+        // positive definite is solved again without P (the steepest-descent
+        // restart of lobpcg.m:845-860)
+        const int explicit_x = prec_ == HIPMV_FP32 || maxres() <= kExplicitGramRes ? 1 : 0;
         void *BRp = noB_ ? R_ : BR_, *BPp = noB_ ? P_ : BP_;
         BXp = noB_ ? X_ : BX_;
         {
@@ -637,6 +648,50 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
         local_it++;
     }
     *it_io = it - 1;
+    // ---- final Rayleigh-Ritz on X (lobpcg.m:922-974): AX and BX from X, the
+    // explicit, symmetrised X'AX and X'BX, X <- X C, residuals of every column
+    if (set.final_rr && !*switched && flag == 0) {
+        if (!noB_) apply(opB_, Bd_, X_, BX_, k);
+        BXp = noB_ ? X_ : BX_;
+        apply(opA_, Ad_, X_, AX_, k);
+        {
+            GramBatch gb = gram_batch();
+            gb.add(X_, k, AX_, k, GA_);
+            gb.add(X_, k, BXp, k, GB_);
+            run_gram(gb);
+        }
+        tm_->begin(kDense);
+        rr_solve(GA_, GB_, ld3_, k, 0, 0, 1, lam_, lam_new_, coef_, ld3_, st_);
+        tm_->end();
+        UpdGroup g[3] = {};
+        int ng = 0;
+        for (void *b : {X_, AX_, noB_ ? nullptr : BX_}) {
+            if (!b) continue;
+            g[ng].in[0] = b;
+            g[ng].h[0] = k;
+            g[ng].out_full = b;
+            ng++;
+        }
+        update(g, ng, coef_, k, k);
+        tm_->begin(kOther);
+        residuals(BXp, colmap_identity(k), lam_new_);
+        tm_->end();
+        read_tail(tail, &hs);
+        if (hs.abort) {
+            flag = hs.gevp_info ? hs.gevp_info : 1;
+            if (verb_) fprintf(stderr, "Error in LOBPCG:\nfinal Rayleigh-Ritz failed\n");
+        } else {
+            std::swap(lam_, lam_new_);
+            const double *lam_h = tail.data() + (lam_ - tail_);
+            const double *rn_h = tail.data() + 2 * k;
+            for (int i = 0; i < k; i++) {
+                out_.lam[i] = lam_h[i];
+                out_.res[i] = sqrt(fabs(rn_h[i]));
+            }
+            store_hist(*it_io);  // replaces the last column, as lobpcg.m does
+            final_rr_done_ = 1;
+        }
+    }
     // ---- copy the block back ----
     convert_block(n_, k, X_, ld_, prec_, Xuser, n_, HIPMV_FP64);
     BX_HIP(hipStreamSynchronize(stream()));
@@ -698,6 +753,7 @@ int device_lobpcg(mv_MultiVectorPtr X, void *Ad, Op opA, void *Bd, Op opB, void 
         flag = ph.run(x->data, &it, maxit, false, set, &dummy);
         x->stats.iterations_fp64 = it - it0;
         x->stats.refine_iterations = ph.refine_done_;
+        x->stats.final_rr = ph.final_rr_done_;
     }
     *iters = it;
     if (verb == 1) {

@@ -100,12 +100,13 @@ def solve_device(A: DeviceCSR, X: DeviceBlock, B=None, T=None, Y: DeviceBlock | 
                  tol: float = 1e-6, rtol: float = 1e-50, maxit: int = 200, verbosity: int = 0,
                  mode: str = "fp64", switch_factor: float = 1.5, stall_iterations: int = 50,
                  refine_iterations: int = 0, histories: bool = True, profile: bool = False,
-                 refresh_interval: int = 20) -> Result:
+                 refresh_interval: int = 20, final_rr: bool = False) -> Result:
     """lobpcg_solve_double on device-resident operands; X is overwritten."""
     lib = L.load()
     k = X.k
     s = L.Settings(L.HIPLOBPCG_MIXED if mode == "mixed" else L.HIPLOBPCG_FP64, switch_factor,
-                   stall_iterations, refine_iterations, int(profile), refresh_interval)
+                   stall_iterations, refine_iterations, int(profile), refresh_interval,
+                   int(final_rr))
     lib.hip_Multi_VectorSetSettings(X.ptr, ctypes.byref(s))
     ii = interpreter()
     xx = lib.mv_MultiVectorWrap(ctypes.byref(ii), ctypes.cast(X.ptr, ctypes.c_void_p), 0)

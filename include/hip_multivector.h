@@ -54,6 +54,10 @@ typedef struct hip_LobpcgSettings {
     int refresh_interval;     /* MIXED: FP32 iterations between recomputations of AX (and BX)
                                  from X, which keep the FP32 residuals those of X; 0: never
                                  (default 20) */
+    int final_rr;             /* 1: after the last iteration, recompute AX (and BX) from X and
+                                 do a Rayleigh-Ritz step on X with explicit X'AX, X'BX, so
+                                 that X is B-orthonormal and the residuals are those of the
+                                 returned X (lobpcg.m:922-974; default 0) */
 } hip_LobpcgSettings;
 
 typedef struct hip_LobpcgStats {
@@ -75,6 +79,7 @@ typedef struct hip_LobpcgStats {
     int fp32_exit;              /* MIXED: why the FP32 phase ended: 0 converged or within
                                    switch_factor * atol, 1 stall, 2 breakdown (R or pencil
                                    not positive definite in FP32) */
+    int final_rr;               /* 1 when the final Rayleigh-Ritz step ran */
 } hip_LobpcgStats;
 
 typedef struct hip_Multi_Vector {

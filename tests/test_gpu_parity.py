@@ -407,6 +407,46 @@ def test_mixed_precision_without_preconditioner(B):
     assert relerr(r.lam, o.lam) < 1e-8
 
 
+@pytest.mark.parametrize("mode,B", [("fp64", None), ("mixed", "diag")])
+def test_final_rayleigh_ritz(mode, B):
+    """final_rr: after the last iteration X is B-orthonormal to FP64 accuracy
+    and the returned residual norms are those of the returned X and lambda
+    (lobpcg.m's post-processing); iterations and eigenvalues are unchanged."""
+    say(f"final Rayleigh-Ritz ({mode}, B={B})")
+    A = pb.laplacian_7pt(10, 9, 8, sigma=1.0, seed=11)
+    X0 = po.drand48_block(A.n, 5, 1)
+    Bd = None if B is None else 0.5 + np.random.default_rng(2).random(A.n)
+    r0 = bl.lobpcg(X0, A, B=Bd, T="jacobi", maxit=400, mode=mode)
+    r1 = bl.lobpcg(X0, A, B=Bd, T="jacobi", maxit=400, mode=mode, final_rr=True)
+    assert r0.flag == 0 and r1.flag == 0
+    assert r1.stats["final_rr"] == 1 and r0.stats["final_rr"] == 0
+    assert r1.iterations == r0.iterations
+    assert relerr(r1.lam, r0.lam) < 1e-10
+    S = A.to_scipy()
+    BX = r1.X if Bd is None else Bd[:, None] * r1.X
+    assert np.max(np.abs(r1.X.T @ BX - np.eye(5))) < 1e-13
+    res = np.linalg.norm(S @ r1.X - BX * r1.lam, axis=0)
+    assert np.max(np.abs(res - r1.res)) < 1e-12 * np.max(np.abs(r1.lam)) + 1e-14
+    assert np.all(r1.res <= 1e-6 + 1e-15)
+
+
+@pytest.mark.parametrize("tol", [1e-11, 1e-12])
+def test_tight_tolerance_explicit_gram(tol):
+    """Below sqrt(eps) the FP64 iterations measure X'AX and X'BX instead of
+    assuming them: tolerances of 1e-11 and 1e-12 are reached.  (With the
+    implicit blocks throughout, as lobpcg.c has them, this solve breaks down
+    near 1e-10; the reference itself stalls there.)"""
+    say(f"FP64 tolerance {tol} (explicit Gram near convergence)")
+    A = pb.laplacian_7pt(10, 10, 10, sigma=1.0, seed=4)
+    X0 = po.drand48_block(A.n, 4, 1)
+    r = bl.lobpcg(X0, A, T="jacobi", tol=tol, maxit=600)
+    assert r.flag == 0
+    assert np.all(r.res <= tol)
+    S = A.to_scipy()
+    res = np.linalg.norm(S @ r.X - r.X * r.lam, axis=0)
+    assert np.max(res) < 2 * tol
+
+
 def test_refinement_iterations():
     """refine_iterations: after convergence, that many more FP64 iterations
     with every column active; the residuals fall below those at convergence
