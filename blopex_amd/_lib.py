@@ -132,6 +132,12 @@ _SIGS = {
     "blopex_amd_kernel_mfma_probe": (c_int, [c_int, c_void_p, c_void_p]),
     "blopex_amd_kernel_rr": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p,
                                      c_int, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    "hip_IJReadCSR": (c_int, [ctypes.c_char_p, P(c_long), P(c_long), P(P(c_int)), P(P(c_int)),
+                              P(P(c_double))]),
+    "hip_ParVectorsRead": (c_int, [ctypes.c_char_p, P(c_long), P(c_int), P(P(c_double))]),
+    "hip_IJFree": (None, [c_void_p]),
+    "hip_CSRMatrixReadIJ": (P(CSRMatrix), [ctypes.c_char_p]),
+    "hip_Multi_VectorReadParVectors": (P(HipMultiVector), [ctypes.c_char_p]),
     "utilities_FortranMatrixUpperInv": (None, [P(FortranMatrix)]),
     "utilities_FortranMatrixSymmetrize": (None, [P(FortranMatrix)]),
 }
