@@ -391,6 +391,22 @@ def test_mixed_precision_small(refresh):
     assert relerr(r.lam, o.lam) < 1e-8
 
 
+@pytest.mark.parametrize("k", [1, 3, 17, 32])
+def test_mixed_precision_block_sizes(k):
+    """MIXED mode from one column to the fused update's full 32: the FP32
+    update forms R for the active columns of every width."""
+    say(f"mixed precision, k={k}")
+    A = pb.laplacian_7pt(10, 10, 10, sigma=1.0, seed=9)
+    X0 = po.drand48_block(A.n, k, 1)
+    o = po.solve(A=A, X0=X0, tdiag=1.0 / A.diagonal(), maxit=500)
+    r = bl.lobpcg(X0, A, T="jacobi", maxit=500, mode="mixed")
+    assert r.flag == 0 and o.flag == 0
+    assert r.stats["iterations_fp32"] > 0
+    assert np.all(r.res <= 1e-6 + 1e-15)
+    # the same eigenpairs as the FP64 oracle (no cluster straddles these block edges)
+    assert relerr(r.lam, o.lam) < 1e-8
+
+
 @pytest.mark.parametrize("B", [None, "diag"])
 def test_mixed_precision_without_preconditioner(B):
     """MIXED mode without T: with B = I the FP32 update forms the residual
