@@ -31,6 +31,11 @@ def cases():
     A = pb.dense_to_csr(np.diag(np.arange(1.0, n + 1)))
     yield "testdiag", dict(A=A, X0=po.drand48_block(n, 3, 1, which="ref"), Y=np.eye(n, 3),
                            tdiag=1.0 / np.arange(1.0, n + 1), atol=1e-4, maxit=40)
+    # config 3: config 1's A, B = diag(uniform[0.5, 1.5]) (seed 7), k = 64, no T; capped at 120
+    # iterations (the reference runs on and breaks down at iteration 158)
+    A = pb.laplacian_7pt(20, 20, 20)
+    yield "cfg3", dict(A=A, X0=po.drand48_block(A.n, 64, 1, which="ref"),
+                       bdiag=pb.uniform(7, A.n, 0.5, 1.5), maxit=120)
     # generalized + Jacobi on a small variable-coefficient grid
     A = pb.laplacian_7pt(12, 12, 12, sigma=1.0, seed=3)
     yield "gen12", dict(A=A, X0=po.drand48_block(A.n, 8, 1, which="ref"),
