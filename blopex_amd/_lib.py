@@ -40,6 +40,15 @@ class Stats(ctypes.Structure):
                 ("fp32_exit", c_int), ("final_rr", c_int)]
 
 
+class AdjointSettings(ctypes.Structure):
+    _fields_ = [("mode", c_int), ("tol", c_double), ("maxit", c_int)]
+
+
+class AdjointStats(ctypes.Structure):
+    _fields_ = [("iterations", c_int), ("corrections", c_int), ("residual", c_double),
+                ("seconds", c_double)]
+
+
 class HipMultiVector(ctypes.Structure):
     _fields_ = [("data", c_void_p), ("size", c_long), ("ld", c_long), ("num_vectors", c_int),
                 ("num_active_vectors", c_int), ("active_indices", ctypes.POINTER(c_int)),
@@ -138,6 +147,10 @@ _SIGS = {
     "hip_IJFree": (None, [c_void_p]),
     "hip_CSRMatrixReadIJ": (P(CSRMatrix), [ctypes.c_char_p]),
     "hip_Multi_VectorReadParVectors": (P(HipMultiVector), [ctypes.c_char_p]),
+    "hip_AdjointSettingsDefault": (None, [P(AdjointSettings)]),
+    "hip_LobpcgAdjoint": (c_int, [P(CSRMatrix), P(DiagMatrix), P(DiagMatrix), P(HipMultiVector),
+                                  c_void_p, c_void_p, P(HipMultiVector), c_void_p, c_void_p,
+                                  P(HipMultiVector), P(AdjointSettings), P(AdjointStats)]),
     "utilities_FortranMatrixUpperInv": (None, [P(FortranMatrix)]),
     "utilities_FortranMatrixSymmetrize": (None, [P(FortranMatrix)]),
 }

@@ -129,6 +129,18 @@ class GramBatch {
     GramPlan p_;
 };
 
+// ---- eigenpair adjoint ----
+// y(:, c) = a[j] x(:, c) + b[j] y(:, c) for c = cm.idx[j]; a, b device FP64 (cm.n)
+void axpby_cols(int prec, long n, const double *a, const void *x, long ldx, const double *b,
+                void *y, long ldy, const ColMap &cm);
+// gval[e] = sum_i gl[i] x_i[r] x_i[c] - (w_i[r] x_i[c] + x_i[r] w_i[c]) / 2 over the
+// stored entries e = (r, c) of A; x, w n x k FP64 (leading dim ld), gl device
+void sampled_outer(const hip_CSRMatrix *A, const double *x, const double *w, long ld, int k,
+                   const double *gl, double *gval);
+// gb[r] = sum_i x_i[r] (a[i] x_i[r] + lam[i] w_i[r]); a, lam device
+void diag_outer(long n, const double *x, const double *w, long ld, int k, const double *a,
+                const double *lam, double *gb);
+
 // kernel spmm_csr uses for ncols contiguous columns with leading dimension
 // ld: 0 CSR, 1 diagonal copy, 2 z-marching stencil kernel
 int spmm_kernel(int prec, const hip_CSRMatrix *A, long ld, int ncols);

@@ -69,7 +69,9 @@ enum KClass { kSpmm, kGram, kUpdate, kDense, kOther, kNumClasses };
 
 // residual level below which FP64 iterations measure X'AX and X'BX.
 // lobpcg.m switches at eps^0.6 (4e-10); here the implicit blocks already broke
-// a 10^3 Jacobi solve down near 1e-10, so the switch comes at eps^0.5 (1.5e-8)
+// a 10^3 Jacobi solve down near 1e-10, so the switch comes at eps^0.5 (1.5e-8).
+// Active columns only: at the usual tolerances (above the switch) columns
+// lock before they reach it, and the iterations stay those of lobpcg.c
 const double kExplicitGramRes = sqrt(DBL_EPSILON);
 
 class KTimer {
@@ -525,12 +527,16 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
         if (!noB_) apply(opB_, Bd_, R_, BR_, nr);
         apply(opA_, Ad_, R_, AR_, nr);
         const int preq = local_it > 0 ? nr : 0;
-        // explicit X'AX, X'BX: always for FP32 iterates; in FP64 once every
-        // residual is below kExplicitGramRes, where the implicit blocks'
-        // rounding would dominate (lobpcg.m:692-696).  With them, a pencil that is not
-        // positive definite is solved again without P (the steepest-descent
-        // restart of lobpcg.m:845-860)
-        const int explicit_x = prec_ == HIPMV_FP32 || maxres() <= kExplicitGramRes ? 1 : 0;
+        // explicit X'AX, X'BX: always for FP32 iterates; in FP64 once an
+        // active column's residual is below kExplicitGramRes, where the
+        // implicit blocks' rounding would dominate (lobpcg.m:692-696; the
+        // columns that converge first are the ones whose R block loses
+        // accuracy, so the smallest active residual decides).  With them, a
+        // pencil that is not positive definite is solved again without P (the
+        // steepest-descent restart of lobpcg.m:845-860)
+        double min_active = HUGE_VAL;
+        for (int t = 0; t < nr; t++) min_active = fmin(min_active, out_.res[act[t]]);
+        const int explicit_x = prec_ == HIPMV_FP32 || min_active <= kExplicitGramRes ? 1 : 0;
         void *BRp = noB_ ? R_ : BR_, *BPp = noB_ ? P_ : BP_;
         BXp = noB_ ? X_ : BX_;
         {

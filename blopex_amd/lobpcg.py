@@ -182,3 +182,79 @@ def lobpcg(X0: np.ndarray | None, A: CSR, B: np.ndarray | CSR | None = None, T=N
     finally:
         for o in reversed(res):
             o.close()
+
+
+@dataclass
+class AdjointResult:
+    gval: np.ndarray            # dL/d(stored values of A), CSR order
+    gb: np.ndarray | None       # dL/d(diagonal of B)
+    W: np.ndarray | None        # the adjoint vectors w_i
+    flag: int                   # 0 converged, 1 CG short of tol
+    stats: dict = field(default_factory=dict)
+
+
+def adjoint(A: CSR, X: np.ndarray, lam: np.ndarray, glam: np.ndarray | None = None,
+            GX: np.ndarray | None = None, B: np.ndarray | None = None, T=None,
+            mode: str = "fp64", tol: float = 1e-10, maxit: int = 2000, want_gb: bool = False,
+            want_W: bool = False) -> AdjointResult:
+    """Gradients of a loss L(lambda, X) of the eigenpairs lobpcg returned, with
+    respect to the stored values of A (and the diagonal of B), through
+    hip_LobpcgAdjoint.  glam = dL/dlambda (k), GX = dL/dX (n x k); B: None or
+    a diagonal; T as for lobpcg (preconditions the projected solves)."""
+    lib = L.load()
+    n, k = X.shape
+    if B is not None and isinstance(B, CSR):
+        raise ValueError("the adjoint takes B = I or a diagonal B")
+    res = []
+    try:
+        dA = DeviceCSR(A)
+        res.append(dA)
+        dB = None
+        if B is not None:
+            dB = DeviceDiag(np.asarray(B))
+            res.append(dB)
+        dT = None
+        if isinstance(T, str):
+            if T != "jacobi":
+                raise ValueError(f"unknown preconditioner {T!r}")
+            dT = DeviceDiag(jacobi_of=dA)
+            res.append(dT)
+        elif T is not None:
+            dT = DeviceDiag(np.asarray(T))
+            res.append(dT)
+        dX = DeviceBlock.from_numpy(X)
+        res.append(dX)
+        dG = None
+        if GX is not None:
+            dG = DeviceBlock.from_numpy(GX)
+            res.append(dG)
+        gv = DeviceBlock(A.nnz, 1)
+        res.append(gv)
+        gb = None
+        if want_gb:
+            gb = DeviceBlock(n, 1)
+            res.append(gb)
+        W = None
+        if want_W:
+            W = DeviceBlock(n, k)
+            res.append(W)
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        gl = None if glam is None else np.ascontiguousarray(glam, dtype=np.float64)
+        s = L.AdjointSettings()
+        lib.hip_AdjointSettingsDefault(ctypes.byref(s))
+        s.mode = L.HIPLOBPCG_MIXED if mode == "mixed" else L.HIPLOBPCG_FP64
+        s.tol, s.maxit = tol, maxit
+        st = L.AdjointStats()
+        flag = lib.hip_LobpcgAdjoint(
+            dA.ptr, dB.ptr if dB else None, dT.ptr if dT else None, dX.ptr, lam.ctypes.data,
+            None if gl is None else gl.ctypes.data, dG.ptr if dG else None,
+            gv.ptr.contents.data, gb.ptr.contents.data if gb else None,
+            W.ptr if W else None, ctypes.byref(s), ctypes.byref(st))
+        if flag < 0:
+            raise RuntimeError(L.last_error())
+        return AdjointResult(gv.to_numpy()[:, 0], None if gb is None else gb.to_numpy()[:, 0],
+                             None if W is None else W.to_numpy(), flag,
+                             {f: getattr(st, f) for f, _ in L.AdjointStats._fields_})
+    finally:
+        for o in reversed(res):
+            o.close()

@@ -61,6 +61,20 @@ def test_struct_layouts():
     # grid[3] (padded to 16)
     assert L.CSRMatrix.diag_ld.offset == 48 + 72 and L.CSRMatrix.grid.offset == 48 + 72 + 24
     assert ctypes.sizeof(L.CSRMatrix) == 48 + 72 + 24 + 16
+    # hip_AdjointSettings {int, double, int}, hip_AdjointStats {int, int, double, double}
+    assert ctypes.sizeof(L.AdjointSettings) == 24 and L.AdjointSettings.maxit.offset == 16
+    assert ctypes.sizeof(L.AdjointStats) == 24 and L.AdjointStats.seconds.offset == 16
+
+
+def test_adjoint_defaults_and_argument_checks(lib):
+    """Host-only paths of hip_LobpcgAdjoint: the defaults, and missing
+    required arguments rejected before any device work."""
+    s = L.AdjointSettings()
+    lib.hip_AdjointSettingsDefault(ctypes.byref(s))
+    assert (s.mode, s.tol, s.maxit) == (L.HIPLOBPCG_FP64, 1e-10, 2000)
+    rc = lib.hip_LobpcgAdjoint(None, None, None, None, None, None, None, None, None, None, None,
+                               None)
+    assert rc == -1 and "required" in L.last_error()
 
 
 def test_fortran_matrix_host_routines(lib):
