@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "libblopex_amd.so")
 c_int, c_long, c_double, c_void_p = ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_void_p
 OP = ctypes.CFUNCTYPE(None, c_void_p, c_void_p, c_void_p)
 
-HIPMV_FP64, HIPMV_FP32 = 0, 1
+HIPMV_FP64, HIPMV_FP32, HIPMV_C64 = 0, 1, 2
 HIPLOBPCG_FP64, HIPLOBPCG_MIXED = 0, 1
 
 
@@ -135,6 +135,16 @@ _SIGS = {
                                      P(c_int), c_void_p, c_void_p, c_int, c_void_p, c_void_p,
                                      c_int]),
     "blopex_amd_default_interpreters": (None, [P(LobpcgInterp), P(FortranInterp)]),
+    "blopex_amd_complex_interpreters": (None, [P(LobpcgInterp), P(FortranInterp)]),
+    "zutilities_FortranMatrixMultiply": (None, [P(FortranMatrix), c_int, P(FortranMatrix), c_int,
+                                                P(FortranMatrix)]),
+    "zutilities_FortranMatrixSymmetrize": (None, [P(FortranMatrix)]),
+    "zutilities_FortranMatrixTransposeSquare": (None, [P(FortranMatrix)]),
+    "zutilities_FortranMatrixUpperInv": (None, [P(FortranMatrix)]),
+    "zutilities_FortranMatrixCopy": (None, [P(FortranMatrix), c_int, P(FortranMatrix)]),
+    "zutilities_FortranMatrixIndexCopy": (None, [c_void_p, P(FortranMatrix), c_int,
+                                                 P(FortranMatrix)]),
+    "zutilities_FortranMatrixAbs": (c_double, [P(FortranMatrix), c_long, c_long]),
     "blopex_amd_kernel_update": (c_int, [c_int, c_long, c_int, c_int, c_int, c_void_p, c_void_p,
                                          c_void_p, c_void_p, c_void_p, c_void_p]),
     "blopex_amd_kernel_gram": (c_int, [c_int, c_long, c_int, c_int, c_void_p, c_void_p, c_void_p]),
@@ -157,7 +167,8 @@ _SIGS = {
 
 # every function include/*.h declares (checked against the built library by the CPU tests)
 HEADER_FUNCTIONS = sorted(_SIGS) + [
-    "HipCSRMatMultiVec", "HipDiagMatMultiVec", "mv_MultiVectorCreateFromSampleVector",
+    "HipCSRMatMultiVec", "HipDiagMatMultiVec", "HipDenseMatMultiVec",
+    "mv_MultiVectorCreateFromSampleVector",
     "mv_MultiVectorPrint",
 ]
 

@@ -50,7 +50,9 @@ void upload_async(void *dst, const void *src, size_t bytes);
 // Device->host copy on the stream, then waits for it.
 void download_sync(void *dst, const void *src, size_t bytes);
 
-inline size_t elem_size(int precision) { return precision == HIPMV_FP32 ? 4 : 8; }
+inline size_t elem_size(int precision) {
+    return precision == HIPMV_FP32 ? 4 : precision == HIPMV_C64 ? 16 : 8;
+}
 
 // Device-side status of one solve; kernels after a fatal failure are no-ops.
 struct DevStatus {

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "common.h"
+#include "fortran_matrix.h"
 #include "kernels.h"
 
 using namespace bx;
@@ -30,7 +31,7 @@ hip_LobpcgSettings default_settings() {
 
 hip_Multi_Vector *alloc_header(long size, BlopexInt k, int precision) {
     BX_REQUIRE(size >= 0 && k >= 0, "hip_Multi_VectorCreate: bad shape %ld x %d", size, (int)k);
-    BX_REQUIRE(precision == HIPMV_FP64 || precision == HIPMV_FP32,
+    BX_REQUIRE(precision == HIPMV_FP64 || precision == HIPMV_FP32 || precision == HIPMV_C64,
                "hip_Multi_VectorCreate: bad precision %d", precision);
     hip_Multi_Vector *x = (hip_Multi_Vector *)calloc(1, sizeof(hip_Multi_Vector));
     x->size = size;
@@ -49,6 +50,21 @@ inline hip_Multi_Vector *H(void *p) { return (hip_Multi_Vector *)p; }
 void same_shape(const hip_Multi_Vector *x, const hip_Multi_Vector *y, const char *op) {
     BX_REQUIRE(x->size == y->size, "%s: row counts differ (%ld vs %ld)", op, x->size, y->size);
     BX_REQUIRE(x->precision == y->precision, "%s: precisions differ", op);
+}
+
+// The block as the FP64 kernels see it: a komplex column is 2 * size doubles
+// (copies, clears, real axpy and the drand48 fill act on those directly).
+struct View {
+    int prec;
+    long n, ld;
+};
+View view(const hip_Multi_Vector *x) {
+    if (x->precision == HIPMV_C64) return {HIPMV_FP64, 2 * x->size, 2 * x->ld};
+    return {x->precision, x->size, x->ld};
+}
+bool cplx(const hip_Multi_Vector *x) { return x->precision == HIPMV_C64; }
+void real_only(const hip_Multi_Vector *x, const char *op) {
+    BX_REQUIRE(x->precision != HIPMV_C64, "%s: complex blocks take HipDenseMatMultiVec", op);
 }
 
 // active positions of a caller mask over n entries (NULL: all)
@@ -92,29 +108,29 @@ void hip_Multi_VectorDestroy(hip_Multi_Vector *x) {
 }
 
 int hip_Multi_VectorUpload(hip_Multi_Vector *x, const double *host) {
-    size_t n = (size_t)x->size * x->num_vectors;
+    const View v = view(x);
+    size_t n = (size_t)v.n * x->num_vectors;
     if (!n) return 0;
-    if (x->precision == HIPMV_FP64 && x->ld == x->size) {
+    if (v.prec == HIPMV_FP64 && v.ld == v.n) {
         BX_HIP(hipMemcpyAsync(x->data, host, n * 8, hipMemcpyHostToDevice, stream()));
     } else {
         void *tmp = scratch(4, n * 8);
         BX_HIP(hipMemcpyAsync(tmp, host, n * 8, hipMemcpyHostToDevice, stream()));
-        convert_block(x->size, x->num_vectors, tmp, x->size, HIPMV_FP64, x->data, x->ld,
-                      x->precision);
+        convert_block(v.n, x->num_vectors, tmp, v.n, HIPMV_FP64, x->data, v.ld, v.prec);
     }
     BX_HIP(hipStreamSynchronize(stream()));
     return 0;
 }
 
 int hip_Multi_VectorDownload(const hip_Multi_Vector *x, double *host) {
-    size_t n = (size_t)x->size * x->num_vectors;
+    const View v = view(x);
+    size_t n = (size_t)v.n * x->num_vectors;
     if (!n) return 0;
-    if (x->precision == HIPMV_FP64 && x->ld == x->size) {
+    if (v.prec == HIPMV_FP64 && v.ld == v.n) {
         BX_HIP(hipMemcpyAsync(host, x->data, n * 8, hipMemcpyDeviceToHost, stream()));
     } else {
         void *tmp = scratch(4, n * 8);
-        convert_block(x->size, x->num_vectors, x->data, x->ld, x->precision, tmp, x->size,
-                      HIPMV_FP64);
+        convert_block(v.n, x->num_vectors, x->data, v.ld, v.prec, tmp, v.n, HIPMV_FP64);
         BX_HIP(hipMemcpyAsync(host, tmp, n * 8, hipMemcpyDeviceToHost, stream()));
     }
     BX_HIP(hipStreamSynchronize(stream()));
@@ -122,7 +138,10 @@ int hip_Multi_VectorDownload(const hip_Multi_Vector *x, double *host) {
 }
 
 int hip_Multi_VectorSetRandomValues(hip_Multi_Vector *x, BlopexInt seed) {
-    drand48_cols(x->precision, x->size, (int)seed, x->data, x->ld, colmap_of(x));
+    // komplex: real and imaginary parts take consecutive values of the stream
+    // (blopex_serial_complex/multivector/multi_vector.c:166-190)
+    const View v = view(x);
+    drand48_cols(v.prec, v.n, (int)seed, x->data, v.ld, colmap_of(x));
     return 0;
 }
 
@@ -148,9 +167,9 @@ static void *hipmv_CopyCreate(void *src_, BlopexInt copyValues) {
     hip_Multi_Vector *src = H(src_);
     hip_Multi_Vector *dst = hip_Multi_VectorCreate(src->size, src->num_vectors, src->precision);
     dst->settings = src->settings;
+    const View vs = view(src), vd = view(dst);
     if (copyValues && src->size && src->num_vectors)
-        convert_block(src->size, src->num_vectors, src->data, src->ld, src->precision, dst->data,
-                      dst->ld, dst->precision);
+        convert_block(vs.n, src->num_vectors, src->data, vs.ld, vs.prec, dst->data, vd.ld, vd.prec);
     return dst;
 }
 
@@ -170,12 +189,14 @@ static void hipmv_SetMask(void *x_, BlopexInt *mask) {
 static void hipmv_Copy(void *x_, void *y_) {
     hip_Multi_Vector *x = H(x_), *y = H(y_);
     same_shape(x, y, "CopyMultiVector");
-    copy_cols(x->precision, x->size, x->data, x->ld, colmap_of(x), y->data, y->ld, colmap_of(y));
+    const View vx = view(x), vy = view(y);
+    copy_cols(vx.prec, vx.n, x->data, vx.ld, colmap_of(x), y->data, vy.ld, colmap_of(y));
 }
 
 static void hipmv_Clear(void *x_) {
     hip_Multi_Vector *x = H(x_);
-    clear_cols(x->precision, x->size, x->data, x->ld, colmap_of(x));
+    const View v = view(x);
+    clear_cols(v.prec, v.n, x->data, v.ld, colmap_of(x));
 }
 
 static void hipmv_SetRandom(void *x, BlopexInt seed) { hip_Multi_VectorSetRandomValues(H(x), seed); }
@@ -187,6 +208,15 @@ static void hipmv_InnerProd(void *x_, void *y_, BlopexInt gh, BlopexInt h, Blope
                "MultiInnerProd: %d x %d result for %d x %d active columns", (int)h, (int)w,
                (int)x->num_active_vectors, (int)y->num_active_vectors);
     if (h == 0 || w == 0) return;
+    if (cplx(x)) {
+        komplex *g = (komplex *)scratch(3, (size_t)h * w * sizeof(komplex));
+        zgram_cols(x->size, x->data, x->ld, colmap_of(x), y->data, y->ld, colmap_of(y), g, h);
+        std::vector<komplex> tmp((size_t)h * w);
+        download_sync(tmp.data(), g, tmp.size() * sizeof(komplex));
+        for (BlopexInt j = 0; j < w; j++)
+            memcpy((komplex *)v + (size_t)j * gh, tmp.data() + (size_t)j * h, sizeof(komplex) * h);
+        return;
+    }
     double *g = (double *)scratch(3, (size_t)h * w * sizeof(double));
     gram_cols(x->precision, x->size, x->data, x->ld, colmap_of(x), y->data, y->ld,
               colmap_of(y), g, h);
@@ -205,6 +235,14 @@ static void hipmv_InnerProdDiag(void *x_, void *y_, BlopexInt *mask, BlopexInt n
                "MultiInnerProdDiag: mask selects %d entries for %d active columns", (int)pos.size(),
                (int)x->num_active_vectors);
     if (pos.empty()) return;
+    if (cplx(x)) {
+        komplex *d = (komplex *)scratch(3, pos.size() * sizeof(komplex));
+        zdiagdot_cols(x->size, x->data, x->ld, colmap_of(x), y->data, y->ld, colmap_of(y), d);
+        std::vector<komplex> tmp(pos.size());
+        download_sync(tmp.data(), d, tmp.size() * sizeof(komplex));
+        for (size_t i = 0; i < pos.size(); i++) ((komplex *)diag)[pos[i]] = tmp[i];
+        return;
+    }
     double *d = (double *)scratch(3, pos.size() * sizeof(double));
     diagdot_cols(x->precision, x->size, x->data, x->ld, colmap_of(x), y->data, y->ld,
                  colmap_of(y), d);
@@ -222,6 +260,17 @@ static void vecmat_common(void *x_, BlopexInt gh, BlopexInt h, BlopexInt w, void
                "MultiVecMat: %d x %d matrix for %d -> %d active columns", (int)h, (int)w,
                (int)x->num_active_vectors, (int)y->num_active_vectors);
     BX_REQUIRE(x != y, "MultiVecMat: x and y must differ");
+    if (cplx(x)) {
+        std::vector<komplex> c((size_t)h * w);
+        for (BlopexInt j = 0; j < w; j++)
+            memcpy(c.data() + (size_t)j * h, (const komplex *)v + (size_t)j * gh,
+                   sizeof(komplex) * h);
+        void *dc = scratch(2, c.size() * sizeof(komplex));
+        upload_async(dc, c.data(), c.size() * sizeof(komplex));
+        zvecmat_cols(x->size, x->data, x->ld, colmap_of(x), dc, h, y->data, y->ld, colmap_of(y),
+                     acc);
+        return;
+    }
     std::vector<double> c((size_t)h * w);
     for (BlopexInt j = 0; j < w; j++)
         memcpy(c.data() + (size_t)j * h, (const double *)v + (size_t)j * gh, sizeof(double) * h);
@@ -246,6 +295,15 @@ static void hipmv_VecMatDiag(void *x_, BlopexInt *mask, BlopexInt n, void *diag,
     BX_REQUIRE((int)pos.size() == x->num_active_vectors,
                "MultiVecMatDiag: mask selects %d entries for %d active columns", (int)pos.size(),
                (int)x->num_active_vectors);
+    if (cplx(x)) {
+        if (pos.empty()) return;
+        std::vector<komplex> coef(pos.size());
+        for (size_t i = 0; i < pos.size(); i++) coef[i] = ((const komplex *)diag)[pos[i]];
+        void *dc = scratch(2, coef.size() * sizeof(komplex));
+        upload_async(dc, coef.data(), coef.size() * sizeof(komplex));
+        zscale_cols(x->size, dc, x->data, x->ld, colmap_of(x), y->data, y->ld, colmap_of(y));
+        return;
+    }
     std::vector<double> coef(pos.size());
     for (size_t i = 0; i < pos.size(); i++) coef[i] = ((const double *)diag)[pos[i]];
     scale_cols(x->precision, x->size, coef.data(), x->data, x->ld, colmap_of(x), y->data,
@@ -255,13 +313,14 @@ static void hipmv_VecMatDiag(void *x_, BlopexInt *mask, BlopexInt n, void *diag,
 static void hipmv_Axpy(double alpha, void *x_, void *y_) {
     hip_Multi_Vector *x = H(x_), *y = H(y_);
     same_shape(x, y, "MultiAxpy");
-    axpy_cols(x->precision, x->size, alpha, x->data, x->ld, colmap_of(x), y->data, y->ld,
-              colmap_of(y));
+    const View vx = view(x), vy = view(y);
+    axpy_cols(vx.prec, vx.n, alpha, x->data, vx.ld, colmap_of(x), y->data, vy.ld, colmap_of(y));
 }
 
 static void hipmv_Print(void *x_, char *tag, BlopexInt limit) {
     hip_Multi_Vector *x = H(x_);
-    std::vector<double> host((size_t)x->size * x->num_vectors);
+    const int c = cplx(x) ? 2 : 1;
+    std::vector<double> host((size_t)x->size * x->num_vectors * c);
     hip_Multi_VectorDownload(x, host.data());
     printf("======= %s =========\n", tag ? tag : "");
     printf("size %ld\nowns data %d\nnum vectors %d\nnum active vectors  %d\n", x->size,
@@ -274,9 +333,13 @@ static void hipmv_Print(void *x_, char *tag, BlopexInt limit) {
     }
     for (int i = 0; i < cols; i++) printf("index %d active %d\n", i, (int)x->active_indices[i]);
     for (int i = 0; i < cols; i++)
-        for (long r = 0; r < rows; r++)
-            printf("%ld %d  %22.16e  \n", r, i,
-                   host[(size_t)x->active_indices[i] * x->size + r]);
+        for (long r = 0; r < rows; r++) {
+            const size_t e = ((size_t)x->active_indices[i] * x->size + r) * c;
+            if (c == 2)
+                printf("%ld %d  %22.16e  %22.16e \n", r, i, host[e], host[e + 1]);
+            else
+                printf("%ld %d  %22.16e  \n", r, i, host[e]);
+        }
 }
 
 static void hipmv_Eval(void (*f)(void *, void *, void *), void *par, void *x_, void *y_) {
@@ -439,6 +502,7 @@ void HipCSRMatMultiVec(void *A_, void *X_, void *Y_) {
     BX_REQUIRE(A && X && Y, "HipCSRMatMultiVec: NULL argument");
     same_shape(X, Y, "HipCSRMatMultiVec");
     BX_REQUIRE(X->size == A->n, "HipCSRMatMultiVec: block has %ld rows, matrix %ld", X->size, A->n);
+    real_only(X, "HipCSRMatMultiVec");
     spmm_csr(X->precision, A, X->data, X->ld, colmap_of(X), Y->data, Y->ld, colmap_of(Y));
 }
 
@@ -487,7 +551,20 @@ void HipDiagMatMultiVec(void *D_, void *X_, void *Y_) {
     BX_REQUIRE(D && X && Y, "HipDiagMatMultiVec: NULL argument");
     same_shape(X, Y, "HipDiagMatMultiVec");
     BX_REQUIRE(X->size == D->n, "HipDiagMatMultiVec: block has %ld rows, diagonal %ld", X->size, D->n);
+    real_only(X, "HipDiagMatMultiVec");
     diag_apply(X->precision, D, X->data, X->ld, colmap_of(X), Y->data, Y->ld, colmap_of(Y));
+}
+
+void HipDenseMatMultiVec(void *A_, void *X_, void *Y_) {
+    const hip_Multi_Vector *A = H(A_), *X = H(X_), *Y = H(Y_);
+    BX_REQUIRE(A->precision == HIPMV_C64 && X->precision == HIPMV_C64 &&
+                   Y->precision == HIPMV_C64,
+               "HipDenseMatMultiVec: A, X and Y must be complex blocks");
+    BX_REQUIRE(A->size == A->num_vectors && A->size == X->size && X->size == Y->size,
+               "HipDenseMatMultiVec: A is %ld x %d, X has %ld rows, Y %ld", A->size,
+               (int)A->num_vectors, X->size, Y->size);
+    zdense_apply(A->size, A->data, A->ld, X->data, X->ld, colmap_of(X), Y->data, Y->ld,
+                 colmap_of(Y));
 }
 
 }  // extern "C"

@@ -129,6 +129,23 @@ class GramBatch {
     GramPlan p_;
 };
 
+// ---- complex (komplex) blocks: column-major n x k, interleaved FP64 pairs ----
+// g (device komplex, h x w, leading dim ldg) = x(:, xm)^H y(:, ym)
+void zgram_cols(long n, const void *x, long ldx, const ColMap &xm, const void *y, long ldy,
+                const ColMap &ym, void *g, long ldg);
+// d (device komplex, xm.n) = diag(x(:, xm)^H y(:, ym))
+void zdiagdot_cols(long n, const void *x, long ldx, const ColMap &xm, const void *y, long ldy,
+                   const ColMap &ym, void *d);
+// y(:, ym) = x(:, xm) c (+ y); c device komplex xm.n x ym.n, leading dim ldc
+void zvecmat_cols(long n, const void *x, long ldx, const ColMap &xm, const void *c, long ldc,
+                  void *y, long ldy, const ColMap &ym, bool accumulate);
+// y(:, ym[j]) = d[j] x(:, xm[j]), d device komplex
+void zscale_cols(long n, const void *d, const void *x, long ldx, const ColMap &xm, void *y,
+                 long ldy, const ColMap &ym);
+// y(:, ym) = A x(:, xm), A dense komplex n x n
+void zdense_apply(long n, const void *A, long lda, const void *x, long ldx, const ColMap &xm,
+                  void *y, long ldy, const ColMap &ym);
+
 // ---- eigenpair adjoint ----
 // y(:, c) = a[j] x(:, c) + b[j] y(:, c) for c = cm.idx[j]; a, b device FP64 (cm.n)
 void axpby_cols(int prec, long n, const double *a, const void *x, long ldx, const double *b,

@@ -1,7 +1,8 @@
 // lobpcg_solve_double / lobpcg_solve_complex entry points
-// (krylov/lobpcg.c:332-498): a block wrapped with the HIP interpreter runs
-// the device-resident solver, any other backend the generic algorithm with
-// the FortranMatrix routines of this library and the caller's LAPACK.
+// (krylov/lobpcg.c:332-498): a real block wrapped with the HIP interpreter
+// runs the device-resident solver; complex blocks and other backends run the
+// generic algorithm with the FortranMatrix routines of this library and the
+// caller's LAPACK.
 #include <cstdio>
 
 #include "common.h"
@@ -10,6 +11,8 @@
 extern "C" int blopex_amd_is_hip_interpreter(const mv_InterfaceInterpreter *i);
 extern "C" mv_InterfaceInterpreter *blopex_amd_mv_interpreter(mv_MultiVectorPtr x);
 extern "C" void blopex_amd_default_interpreters(lobpcg_Interpreter *lp,
+                                                utilities_FortranInterpreter *u);
+extern "C" void blopex_amd_complex_interpreters(lobpcg_Interpreter *lp,
                                                 utilities_FortranInterpreter *u);
 
 namespace bx {
@@ -28,9 +31,16 @@ int lobpcg_solve_double(mv_MultiVectorPtr X, void *Ad, void (*opA)(void *, void 
                         lobpcg_BLASLAPACKFunctions blap, lobpcg_Tolerance tol, int maxit, int verb,
                         int *iters, double *lam, double *lamh, BlopexInt lamh_gh, double *res,
                         double *resh, BlopexInt resh_gh) {
-    if (blopex_amd_is_hip_interpreter(blopex_amd_mv_interpreter(X)))
+    if (blopex_amd_is_hip_interpreter(blopex_amd_mv_interpreter(X))) {
+        if (((hip_Multi_Vector *)mv_MultiVectorGetData(X))->precision == HIPMV_C64) {
+            bx::set_error("lobpcg_solve_double: complex block (use lobpcg_solve_complex)");
+            if (verb) fprintf(stderr, "Error in LOBPCG:\n%s\n", blopex_amd_last_error());
+            *iters = 0;
+            return WRONG_BLOCK_SIZE;
+        }
         return bx::device_lobpcg(X, Ad, opA, Bd, opB, Td, opT, Y, tol, maxit, verb, iters, lam,
                                  lamh, lamh_gh, res, resh, resh_gh);
+    }
     if (!blap.dpotrf || !blap.dsygv) {
         bx::set_error("lobpcg_solve_double: dpotrf/dsygv are required for a host backend");
         if (verb) fprintf(stderr, "Error in LOBPCG:\n%s\n", blopex_amd_last_error());
@@ -44,15 +54,33 @@ int lobpcg_solve_double(mv_MultiVectorPtr X, void *Ad, void (*opA)(void *, void 
                         lamh_gh, res, resh, resh_gh, &lp, &u);
 }
 
-int lobpcg_solve_complex(mv_MultiVectorPtr, void *, void (*)(void *, void *, void *), void *,
-                         void (*)(void *, void *, void *), void *, void (*)(void *, void *, void *),
-                         mv_MultiVectorPtr, lobpcg_BLASLAPACKFunctions, lobpcg_Tolerance, int,
-                         int verb, int *iters, komplex *, komplex *, BlopexInt, double *, double *,
-                         BlopexInt) {
-    bx::set_error("lobpcg_solve_complex: complex Hermitian problems are not supported");
-    if (verb) fprintf(stderr, "Error in LOBPCG:\ncomplex Hermitian problems are not supported\n");
-    if (iters) *iters = 0;
-    return WRONG_BLOCK_SIZE;
+int lobpcg_solve_complex(mv_MultiVectorPtr X, void *Ad, void (*opA)(void *, void *, void *),
+                         void *Bd, void (*opB)(void *, void *, void *), void *Td,
+                         void (*opT)(void *, void *, void *), mv_MultiVectorPtr Y,
+                         lobpcg_BLASLAPACKFunctions blap, lobpcg_Tolerance tol, int maxit,
+                         int verb, int *iters, komplex *lam, komplex *lamh, BlopexInt lamh_gh,
+                         double *res, double *resh, BlopexInt resh_gh) {
+    // complex Hermitian problems run the generic algorithm: block work through
+    // X's interpreter (HIPMV_C64 blocks for the device), small dense work
+    // through the complex FortranMatrix routines and the caller's zpotrf/zhegv
+    if (blopex_amd_is_hip_interpreter(blopex_amd_mv_interpreter(X)) &&
+        ((hip_Multi_Vector *)mv_MultiVectorGetData(X))->precision != HIPMV_C64) {
+        bx::set_error("lobpcg_solve_complex: the block is not complex (HIPMV_C64)");
+        if (verb) fprintf(stderr, "Error in LOBPCG:\n%s\n", blopex_amd_last_error());
+        if (iters) *iters = 0;
+        return WRONG_BLOCK_SIZE;
+    }
+    if (!blap.zpotrf || !blap.zhegv) {
+        bx::set_error("lobpcg_solve_complex: zpotrf/zhegv are required");
+        if (verb) fprintf(stderr, "Error in LOBPCG:\n%s\n", blopex_amd_last_error());
+        if (iters) *iters = 0;
+        return REQUESTED_ACCURACY_NOT_ACHIEVED;
+    }
+    lobpcg_Interpreter lp;
+    utilities_FortranInterpreter u;
+    blopex_amd_complex_interpreters(&lp, &u);
+    return lobpcg_solve(X, Ad, opA, Bd, opB, Td, opT, Y, blap, tol, maxit, verb, iters, lam, lamh,
+                        lamh_gh, res, resh, resh_gh, &lp, &u);
 }
 
 }  // extern "C"

@@ -1,0 +1,181 @@
+// Complex (komplex: interleaved FP64 real, imaginary) block kernels for the
+// hip_Multi_Vector slots that lobpcg_solve_complex drives: the Gram product
+// X^H Y and its diagonal, block times small complex matrix, column scaling by
+// complex diagonals, and a dense complex operator (the reference's
+// MatMultiVec over a dense multivector, blopex_serial_complex/multivector/
+// multi_vector.c:567-610).  Copies, clears, real axpy and the drand48 fill
+// treat a complex column as 2n doubles and use the FP64 kernels.
+// Reductions are two-stage with a fixed summation order, as in blas.hip.
+#include "kernels.h"
+
+namespace bx {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
+}
+// conj(a) * b
+__device__ __forceinline__ double2 cmulc(double2 a, double2 b) {
+    return make_double2(fma(a.x, b.x, a.y * b.y), fma(a.x, b.y, -a.y * b.x));
+}
+
+unsigned zrow_grid(long n) {
+    size_t blocks = ((size_t)n + kThreads - 1) / kThreads;
+    size_t cap = (size_t)num_cus() * 4;
+    return (unsigned)(blocks < 1 ? 1 : (blocks > cap ? cap : blocks));
+}
+
+// part[(pair) * nparts + blockIdx.x] = sum over this block's rows of conj(x_i) y_j,
+// pair = blockIdx.y: (i, j) = (pair % h, pair / h), or (pair, pair) when diag
+__global__ void __launch_bounds__(kThreads) k_zgram(long n, const double2 *x, long ldx, ColMap xm,
+                                                    const double2 *y, long ldy, ColMap ym, int h,
+                                                    int diag, double2 *part) {
+    __shared__ double2 red[kThreads];
+    const int pr = blockIdx.y;
+    const int i = diag ? pr : pr % h, j = diag ? pr : pr / h;
+    const double2 *xc = x + (size_t)xm.idx[i] * ldx;
+    const double2 *yc = y + (size_t)ym.idx[j] * ldy;
+    double2 s = make_double2(0.0, 0.0);
+    for (long r = (long)blockIdx.x * kThreads + threadIdx.x; r < n; r += (long)gridDim.x * kThreads) {
+        const double2 p = cmulc(xc[r], yc[r]);
+        s.x += p.x;
+        s.y += p.y;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            red[threadIdx.x].x += red[threadIdx.x + w].x;
+            red[threadIdx.x].y += red[threadIdx.x + w].y;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(size_t)pr * gridDim.x + blockIdx.x] = red[0];
+}
+
+// g[(pair % h) + (pair / h) * ldg] (or g[pair] when diag) = sum of the pair's partials
+__global__ void k_zgram_sum(const double2 *part, int nparts, int npairs, int h, int diag, long ldg,
+                            double2 *g) {
+    const int pr = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pr >= npairs) return;
+    double2 s = make_double2(0.0, 0.0);
+    for (int p = 0; p < nparts; p++) {
+        s.x += part[(size_t)pr * nparts + p].x;
+        s.y += part[(size_t)pr * nparts + p].y;
+    }
+    if (diag)
+        g[pr] = s;
+    else
+        g[(pr % h) + (size_t)(pr / h) * ldg] = s;
+}
+
+// y(:, ym[j]) = sum_i c[i + j ldc] x(:, xm[i]) (+ y when accumulate)
+__global__ void __launch_bounds__(kThreads) k_zvecmat(long n, const double2 *x, long ldx,
+                                                      ColMap xm, const double2 *c, long ldc,
+                                                      double2 *y, long ldy, ColMap ym,
+                                                      int accumulate) {
+    const int j = blockIdx.y;
+    double2 *yc = y + (size_t)ym.idx[j] * ldy;
+    const double2 *cj = c + (size_t)j * ldc;
+    for (long r = (long)blockIdx.x * kThreads + threadIdx.x; r < n; r += (long)gridDim.x * kThreads) {
+        double2 s = accumulate ? yc[r] : make_double2(0.0, 0.0);
+        for (int i = 0; i < xm.n; i++) {
+            const double2 p = cmul(cj[i], x[(size_t)xm.idx[i] * ldx + r]);
+            s.x += p.x;
+            s.y += p.y;
+        }
+        yc[r] = s;
+    }
+}
+
+// y(:, ym[j]) = d[j] x(:, xm[j])
+__global__ void __launch_bounds__(kThreads) k_zscale(long n, const double2 *d, const double2 *x,
+                                                     long ldx, ColMap xm, double2 *y, long ldy,
+                                                     ColMap ym) {
+    const int j = blockIdx.y;
+    const double2 a = d[j];
+    const double2 *xc = x + (size_t)xm.idx[j] * ldx;
+    double2 *yc = y + (size_t)ym.idx[j] * ldy;
+    for (long r = (long)blockIdx.x * kThreads + threadIdx.x; r < n; r += (long)gridDim.x * kThreads)
+        yc[r] = cmul(a, xc[r]);
+}
+
+// y(:, ym[j]) = A x(:, xm[j]), A dense n x n (leading dim lda)
+__global__ void __launch_bounds__(kThreads) k_zdense(long n, const double2 *A, long lda,
+                                                     const double2 *x, long ldx, ColMap xm,
+                                                     double2 *y, long ldy, ColMap ym) {
+    const int j = blockIdx.y;
+    const double2 *xc = x + (size_t)xm.idx[j] * ldx;
+    double2 *yc = y + (size_t)ym.idx[j] * ldy;
+    for (long r = (long)blockIdx.x * kThreads + threadIdx.x; r < n; r += (long)gridDim.x * kThreads) {
+        double2 s = make_double2(0.0, 0.0);
+        for (long i = 0; i < n; i++) {
+            const double2 p = cmul(A[(size_t)i * lda + r], xc[i]);
+            s.x += p.x;
+            s.y += p.y;
+        }
+        yc[r] = s;
+    }
+}
+
+void zgram_launch(long n, const void *x, long ldx, const ColMap &xm, const void *y, long ldy,
+                  const ColMap &ym, int diag, void *g, long ldg) {
+    const int h = xm.n, npairs = diag ? xm.n : xm.n * ym.n;
+    if (npairs == 0) return;
+    unsigned nparts = zrow_grid(n);
+    // at most ~64K workgroups in all
+    while (nparts > 1 && (size_t)nparts * npairs > 65536) nparts = (nparts + 1) / 2;
+    double2 *part = (double2 *)scratch(0, (size_t)nparts * npairs * sizeof(double2));
+    hipLaunchKernelGGL(k_zgram, dim3(nparts, npairs), dim3(kThreads), 0, stream(), n,
+                       (const double2 *)x, ldx, xm, (const double2 *)y, ldy, ym, h, diag, part);
+    BX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_zgram_sum, dim3((npairs + 255) / 256), dim3(256), 0, stream(), part,
+                       (int)nparts, npairs, h, diag, ldg, (double2 *)g);
+    BX_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void zgram_cols(long n, const void *x, long ldx, const ColMap &xm, const void *y, long ldy,
+                const ColMap &ym, void *g, long ldg) {
+    zgram_launch(n, x, ldx, xm, y, ldy, ym, 0, g, ldg);
+}
+
+void zdiagdot_cols(long n, const void *x, long ldx, const ColMap &xm, const void *y, long ldy,
+                   const ColMap &ym, void *d) {
+    BX_REQUIRE(xm.n == ym.n, "zdiagdot: active column counts differ (%d vs %d)", xm.n, ym.n);
+    zgram_launch(n, x, ldx, xm, y, ldy, ym, 1, d, 1);
+}
+
+void zvecmat_cols(long n, const void *x, long ldx, const ColMap &xm, const void *c, long ldc,
+                  void *y, long ldy, const ColMap &ym, bool accumulate) {
+    if (ym.n == 0 || n == 0) return;
+    hipLaunchKernelGGL(k_zvecmat, dim3(zrow_grid(n), ym.n), dim3(kThreads), 0, stream(), n,
+                       (const double2 *)x, ldx, xm, (const double2 *)c, ldc, (double2 *)y, ldy, ym,
+                       (int)accumulate);
+    BX_HIP(hipGetLastError());
+}
+
+void zscale_cols(long n, const void *d, const void *x, long ldx, const ColMap &xm, void *y,
+                 long ldy, const ColMap &ym) {
+    BX_REQUIRE(xm.n == ym.n, "zscale: active column counts differ (%d vs %d)", xm.n, ym.n);
+    if (xm.n == 0 || n == 0) return;
+    hipLaunchKernelGGL(k_zscale, dim3(zrow_grid(n), xm.n), dim3(kThreads), 0, stream(), n,
+                       (const double2 *)d, (const double2 *)x, ldx, xm, (double2 *)y, ldy, ym);
+    BX_HIP(hipGetLastError());
+}
+
+void zdense_apply(long n, const void *A, long lda, const void *x, long ldx, const ColMap &xm,
+                  void *y, long ldy, const ColMap &ym) {
+    BX_REQUIRE(xm.n == ym.n, "dense operator: active column counts differ (%d vs %d)", xm.n, ym.n);
+    if (xm.n == 0 || n == 0) return;
+    hipLaunchKernelGGL(k_zdense, dim3(zrow_grid(n), xm.n), dim3(kThreads), 0, stream(), n,
+                       (const double2 *)A, lda, (const double2 *)x, ldx, xm, (double2 *)y, ldy,
+                       ym);
+    BX_HIP(hipGetLastError());
+}
+
+}  // namespace bx

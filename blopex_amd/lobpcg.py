@@ -70,14 +70,19 @@ class DeviceBlock:
         self.ptr = self.lib.hip_Multi_VectorCreate(n, k, precision)
 
     @classmethod
-    def from_numpy(cls, X: np.ndarray, precision: int = L.HIPMV_FP64) -> "DeviceBlock":
+    def from_numpy(cls, X: np.ndarray, precision: int | None = None) -> "DeviceBlock":
+        """Upload X; complex arrays make HIPMV_C64 blocks."""
+        if precision is None:
+            precision = L.HIPMV_C64 if np.iscomplexobj(X) else L.HIPMV_FP64
         b = cls(X.shape[0], X.shape[1], precision)
-        Xf = np.asfortranarray(X, dtype=np.float64)
+        dt = np.complex128 if precision == L.HIPMV_C64 else np.float64
+        Xf = np.asfortranarray(X, dtype=dt)
         b.lib.hip_Multi_VectorUpload(b.ptr, Xf.ctypes.data)
         return b
 
     def to_numpy(self) -> np.ndarray:
-        out = np.zeros((self.n, self.k), order="F")
+        cplx = self.ptr.contents.precision == L.HIPMV_C64
+        out = np.zeros((self.n, self.k), dtype=np.complex128 if cplx else np.float64, order="F")
         self.lib.hip_Multi_VectorDownload(self.ptr, out.ctypes.data)
         return out
 
@@ -257,4 +262,54 @@ def adjoint(A: CSR, X: np.ndarray, lam: np.ndarray, glam: np.ndarray | None = No
                              {f: getattr(st, f) for f, _ in L.AdjointStats._fields_})
     finally:
         for o in reversed(res):
+            o.close()
+
+
+def _zlapack() -> tuple[int, int]:
+    """zpotrf / zhegv of scipy's LAPACK, as function pointers (the caller's
+    LAPACK that lobpcg_BLASLAPACKFunctions carries)."""
+    from scipy.linalg import cython_lapack
+
+    get_name = ctypes.pythonapi.PyCapsule_GetName
+    get_name.restype = ctypes.c_char_p
+    get_name.argtypes = [ctypes.py_object]
+    get_ptr = ctypes.pythonapi.PyCapsule_GetPointer
+    get_ptr.restype = ctypes.c_void_p
+    get_ptr.argtypes = [ctypes.py_object, ctypes.c_char_p]
+    capi = cython_lapack.__pyx_capi__
+    return tuple(get_ptr(capi[f], get_name(capi[f])) for f in ("zpotrf", "zhegv"))
+
+
+def lobpcg_complex(X0: np.ndarray, A: np.ndarray, tol: float = 1e-6, maxit: int = 1000,
+                   verbosity: int = 0, rtol: float = 1e-50) -> Result:
+    """Smallest eigenpairs of a dense complex Hermitian A through
+    lobpcg_solve_complex on HIPMV_C64 blocks (the reference's complex
+    driver, blopex_serial_complex/driver/serial_driver.c, on the device)."""
+    lib = L.load()
+    n, k = X0.shape
+    res_objs = []
+    try:
+        dA = DeviceBlock.from_numpy(np.asarray(A, dtype=np.complex128))
+        res_objs.append(dA)
+        X = DeviceBlock.from_numpy(np.asarray(X0, dtype=np.complex128))
+        res_objs.append(X)
+        ii = interpreter()
+        xx = lib.mv_MultiVectorWrap(ctypes.byref(ii), ctypes.cast(X.ptr, ctypes.c_void_p), 0)
+        zp, zh = _zlapack()
+        blap = L.BlasLapack(None, None, zp, zh)
+        lam = np.zeros(k, dtype=np.complex128)
+        lh = np.zeros((k, maxit + 1), dtype=np.complex128, order="F")
+        res = np.zeros(k)
+        rh = np.zeros((k, maxit + 1), order="F")
+        it = ctypes.c_int(0)
+        flag = lib.lobpcg_solve_complex(
+            xx, ctypes.cast(dA.ptr, ctypes.c_void_p), L.symbol_address("HipDenseMatMultiVec"),
+            None, None, None, None, None, blap, L.Tolerance(tol, rtol), maxit, verbosity,
+            ctypes.byref(it), lam.ctypes.data, lh.ctypes.data, k, res.ctypes.data,
+            rh.ctypes.data, k)
+        lib.mv_MultiVectorDestroy(xx)
+        m = it.value
+        return Result(X.to_numpy(), lam, flag, m, lh[:, : m + 1], rh[:, : m + 1], res)
+    finally:
+        for o in reversed(res_objs):
             o.close()

@@ -32,6 +32,8 @@
 
 #define HIPMV_FP64 0
 #define HIPMV_FP32 1
+#define HIPMV_C64 2 /* komplex entries (interleaved FP64 real, imaginary): blocks of
+                       lobpcg_solve_complex, which runs the generic algorithm */
 
 #define HIPLOBPCG_FP64 0  /* every block operation in FP64 */
 #define HIPLOBPCG_MIXED 1 /* FP32 blocks, FP64 Gram accumulation and small dense, FP64 finish */
@@ -167,6 +169,12 @@ hip_DiagMatrix *hip_DiagMatrixCreateJacobi(const hip_CSRMatrix *A);
 void hip_DiagMatrixDestroy(hip_DiagMatrix *D);
 /* operatorB / operatorT callback: Y(active) = diag(d) * X(active) */
 void HipDiagMatMultiVec(void *D, void *X, void *Y);
+
+/* operatorA callback for a dense complex matrix: A is a hip_Multi_Vector of
+   n x n HIPMV_C64 entries (all columns), Y(active) = A * X(active) -- the
+   reference's MatMultiVec over a dense multivector (blopex_serial_complex/
+   pcg_multi/pcg_multi.c, multi_vector.c:567-610) */
+void HipDenseMatMultiVec(void *A, void *X, void *Y);
 
 #ifdef __cplusplus
 }

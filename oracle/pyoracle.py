@@ -47,6 +47,42 @@ def lapack_pointers() -> tuple[int, int]:
     return _capsule_ptr(capi["dpotrf"]), _capsule_ptr(capi["dsygv"])
 
 
+def zlapack_pointers() -> tuple[int, int]:
+    from scipy.linalg import cython_lapack
+
+    capi = cython_lapack.__pyx_capi__
+    return _capsule_ptr(capi["zpotrf"]), _capsule_ptr(capi["zhegv"])
+
+
+REF_COMPLEX_SO = os.path.join(HERE, "_ref", "libblopex_ref_complex.so")
+
+
+def ref_solve_complex(A: np.ndarray, X0: np.ndarray, atol: float = 1e-6, rtol: float = 1e-50,
+                      maxit: int = 1000) -> "SolveResult":
+    """The reference's lobpcg_solve_complex (compiled in oracle/_ref) on a
+    dense complex A, as its complex serial driver runs it."""
+    lib = ctypes.CDLL(REF_COMPLEX_SO)
+    fn = lib.ref_lobpcg_solve_complex
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                   ctypes.c_double, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                   ctypes.c_void_p]
+    n, k = X0.shape
+    Af = np.asfortranarray(A, dtype=np.complex128)
+    X = np.asfortranarray(X0, dtype=np.complex128).copy(order="F")
+    lam = np.zeros(k, dtype=np.complex128)
+    lamh = np.zeros((k, maxit + 1), dtype=np.complex128, order="F")
+    res = np.zeros(k)
+    resh = np.zeros((k, maxit + 1), order="F")
+    it = ctypes.c_int(0)
+    zp, zh = zlapack_pointers()
+    flag = fn(n, k, Af.ctypes.data, X.ctypes.data, atol, rtol, maxit, zp, zh, ctypes.byref(it),
+              lam.ctypes.data, lamh.ctypes.data, res.ctypes.data, resh.ctypes.data)
+    m = it.value
+    return SolveResult(flag, m, X, lam, res, lamh[:, : m + 1], resh[:, : m + 1])
+
+
 _SOLVE_ARGS = [
     ctypes.c_long, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,

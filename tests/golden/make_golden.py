@@ -3,7 +3,9 @@
 Runs the reference's own lobpcg_solve_double (oracle/_ref/libblopex_ref.so,
 built by oracle/Makefile from /root/reference) on small seeded cases and
 stores inputs' seeds and outputs (eigenvalues, iteration counts, eigenvalue
-and residual histories, final block) as plain arrays.  Run from the repo root:
+and residual histories, final block) as plain arrays; and its
+lobpcg_solve_complex (oracle/_ref/libblopex_ref_complex.so) on the complex
+driver's own data files, stored with the outputs as the inputs A and X0.  Run from the repo root:
 
     python tests/golden/make_golden.py
 """
@@ -42,8 +44,45 @@ def cases():
                         bdiag=pb.uniform(5, A.n, 0.5, 1.5), tdiag=1.0 / A.diagonal(), maxit=300)
 
 
+REF = "/root/reference"
+
+
+def load_complex_driver_file(path: str) -> np.ndarray:
+    """A complex matrix file of the reference's complex driver (first line
+    "rows cols", then "real imag" per entry, column by column), with the
+    entries rounded to single precision as serial_Multi_VectorLoad reads
+    them (fscanf "%e" into float, blopex_serial_complex/multivector/
+    multi_vector.c:668-700)."""
+    with open(path) as f:
+        n, m = map(int, f.readline().split())
+        v = np.loadtxt(f)[: n * m]
+    f32 = lambda a: a.astype(np.float32).astype(np.float64)  # noqa: E731
+    return (f32(v[:, 0]) + 1j * f32(v[:, 1])).reshape((m, n)).T.copy(order="F")
+
+
+def complex_cases():
+    # the complex driver's own inputs: A (Hermitian) and X0 from
+    # blopex_serial_complex/driver/test{1,2}{a,x}.txt, tol 1e-6/1e-50, maxit 1000
+    # (serial_driver.c:100-140)
+    d = os.path.join(REF, "blopex_serial_complex", "driver")
+    for t in ("test1", "test2"):
+        yield "z" + t, load_complex_driver_file(os.path.join(d, t + "a.txt")), \
+            load_complex_driver_file(os.path.join(d, t + "x.txt"))
+
+
 def main():
     out = {}
+    for name, A, X0 in complex_cases():
+        r = po.ref_solve_complex(A, X0, maxit=1000)
+        out[name + "_A"] = A
+        out[name + "_X0"] = X0
+        out[name + "_flag"] = np.array(r.flag)
+        out[name + "_iters"] = np.array(r.iterations)
+        out[name + "_lam"] = r.lam.real.copy()
+        out[name + "_res"] = r.res
+        out[name + "_lam_hist"] = r.lam_hist.real.copy()
+        out[name + "_res_hist"] = r.res_hist
+        print(name, r.flag, r.iterations, r.lam.real)
     for name, kw in cases():
         r = po.solve(which="ref", **kw)
         out[name + "_flag"] = np.array(r.flag)
