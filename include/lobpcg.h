@@ -71,8 +71,12 @@ int lobpcg_solve_double(mv_MultiVectorPtr blockVectorX, void *operatorAData,
                         BlopexInt lambdaHistory_gh, double *residualNorms_values,
                         double *residualNormsHistory_values, BlopexInt residualNormsHistory_gh);
 
-/* Complex Hermitian problems are not supported: returns WRONG_BLOCK_SIZE
-   after reporting the error (see blopex_amd_last_error). */
+/* Complex Hermitian problems (lobpcg.c:417-498): the generic algorithm with
+   the complex FortranMatrix routines (zutilities_*) and blap_fn.zpotrf /
+   zhegv (required: -1 without them).  lambda and its history are komplex
+   with zero imaginary parts.  Blocks wrapped with the HIP interpreter must
+   be HIPMV_C64 (WRONG_BLOCK_SIZE otherwise); HipDenseMatMultiVec is their
+   dense operator. */
 int lobpcg_solve_complex(mv_MultiVectorPtr blockVectorX, void *operatorAData,
                          void (*operatorA)(void *, void *, void *), void *operatorBData,
                          void (*operatorB)(void *, void *, void *), void *operatorTData,
