@@ -49,6 +49,21 @@ class AdjointStats(ctypes.Structure):
                 ("seconds", c_double)]
 
 
+class BatchProblem(ctypes.Structure):
+    _fields_ = [("n", c_long), ("nnz", c_long), ("rowptr", c_void_p), ("colind", c_void_p),
+                ("val", c_void_p), ("k", c_int)]
+
+
+class BatchSettings(ctypes.Structure):
+    _fields_ = [("tol", c_double), ("maxit", c_int), ("jacobi", c_int), ("seed", c_int),
+                ("graph", c_int)]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("seconds", c_double), ("iterations", c_int), ("graph_captures", c_int),
+                ("graph_replays", c_int), ("total_rows", c_long)]
+
+
 class HipMultiVector(ctypes.Structure):
     _fields_ = [("data", c_void_p), ("size", c_long), ("ld", c_long), ("num_vectors", c_int),
                 ("num_active_vectors", c_int), ("active_indices", ctypes.POINTER(c_int)),
@@ -157,6 +172,9 @@ _SIGS = {
     "hip_IJFree": (None, [c_void_p]),
     "hip_CSRMatrixReadIJ": (P(CSRMatrix), [ctypes.c_char_p]),
     "hip_Multi_VectorReadParVectors": (P(HipMultiVector), [ctypes.c_char_p]),
+    "hip_BatchSettingsDefault": (None, [P(BatchSettings)]),
+    "hip_LobpcgSolveBatched": (c_int, [c_int, P(BatchProblem), P(BatchSettings), c_void_p,
+                                       c_void_p, c_void_p, c_void_p, c_void_p, P(BatchStats)]),
     "hip_AdjointSettingsDefault": (None, [P(AdjointSettings)]),
     "hip_LobpcgAdjoint": (c_int, [P(CSRMatrix), P(DiagMatrix), P(DiagMatrix), P(HipMultiVector),
                                   c_void_p, c_void_p, P(HipMultiVector), c_void_p, c_void_p,

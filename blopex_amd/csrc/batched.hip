@@ -1,0 +1,752 @@
+// hip_LobpcgSolveBatched (include/lobpcg_batched.h): LOBPCG on many
+// independent problems at once.
+//
+// Layout: the problems are stacked row-wise into one block-diagonal CSR
+// matrix of N = sum n_p rows, and every block (X, AX, R, AR, P, AP) is one
+// N x K FP64 array (K = the largest k, column-major, leading dimension N);
+// problem p owns rows off_p .. off_p + n_p and columns 0 .. k_p.  Rows are
+// cut into chunks of at most kChunk rows that never straddle two problems;
+// every grouped kernel is launched over the chunks of the unconverged
+// problems, so one launch advances the whole batch:
+//   k_bresid     R = T (AX - X diag(lambda)) and per-chunk partial norms
+//   (CSR SpMM)   AR = A R over the stacked matrix (spmm_csr)
+//   k_bgram      per-chunk partial Gram blocks [X R P]' [AX AR AP] and
+//                [X R P]' [X R P] (upper triangles, the problem's columns only)
+//   k_bgram_sum  per-problem fixed-order sums of the partials
+//   k_brr        one workgroup per problem: the Rayleigh-Ritz pencil in LDS,
+//                diagonal scaling, Cholesky, reduction to standard form,
+//                cyclic Jacobi, back-substitution (FP64)
+//   k_bupd       X' = [X R P] C_X, P' = [R P] C_RP and the same for AX, AP
+// The host reads the residual norms once per iteration; while the set of
+// unconverged problems stays the same, the launches of an iteration are
+// replayed from a captured hipGraph.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "kernels.h"
+#include "lobpcg_batched.h"
+
+namespace bx {
+namespace {
+
+constexpr int kK = HIP_BATCH_MAX_K;   // largest block width
+constexpr int kM = 3 * kK;            // largest pencil
+constexpr int kT = 256;               // threads per workgroup
+constexpr int kChunk = 2048;          // rows per chunk
+constexpr int kSub = 64;              // rows staged per Gram step
+constexpr int kPairs = kM * (kM + 1) / 2;
+constexpr int kPairsPerThread = (kPairs + kT - 1) / kT;
+
+struct Chunks {
+    const int *prob;   // problem of each active chunk
+    const long *r0;    // first (global) row
+    const int *nr;     // rows
+};
+
+struct Probs {
+    const long *off;
+    const int *n;
+    const int *k;
+};
+
+__device__ __forceinline__ double blk_get(const double *b, long ld, long row, int col) {
+    return b[(size_t)col * ld + row];
+}
+
+// ---- residual, Jacobi, per-chunk squared norms ----
+__global__ void __launch_bounds__(kT) k_bresid(Chunks ch, Probs pr, long ld, const double *X,
+                                               const double *AX, const double *lam,
+                                               const double *dinv, double *R, double *part) {
+    __shared__ double red[kT];
+    const int c = blockIdx.x;
+    const int p = ch.prob[c];
+    const int kp = pr.k[p];
+    const long r0 = ch.r0[c];
+    const int nr = ch.nr[c];
+    for (int j = 0; j < kK; j++) {
+        double acc = 0.0;
+        const double l = j < kp ? lam[p * kK + j] : 0.0;
+        for (int t = threadIdx.x; t < nr; t += kT) {
+            const long r = r0 + t;
+            double v = 0.0;
+            if (j < kp) {
+                v = blk_get(AX, ld, r, j) - l * blk_get(X, ld, r, j);
+                acc = fma(v, v, acc);
+                if (dinv) v *= dinv[r];
+            }
+            R[(size_t)j * ld + r] = v;
+        }
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int w = kT / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) part[(size_t)c * kK + j] = red[0];
+        __syncthreads();
+    }
+}
+
+// res2[p K + j] = sum of problem p's chunk partials (fixed order)
+__global__ void k_bres_sum(const int *act_prob, const int *pc_begin, const int *pc_count,
+                           const double *part, double *res2) {
+    const int p = act_prob[blockIdx.x];
+    const int j = threadIdx.x;
+    if (j >= kK) return;
+    double s = 0.0;
+    for (int c = pc_begin[p], e = pc_begin[p] + pc_count[p]; c < e; c++) s += part[(size_t)c * kK + j];
+    res2[p * kK + j] = s;
+}
+
+// ---- grouped Gram blocks ----
+// pencil index q = b * kp + col (block b of {X, R, P}); partials of the upper
+// triangles of GA = L' A L and GB = L' L over the chunk's rows
+__global__ void __launch_bounds__(kT) k_bgram(Chunks ch, Probs pr, long ld, int nb, const double *X,
+                                              const double *R, const double *P, const double *AX,
+                                              const double *AR, const double *AP, double *part) {
+    __shared__ double Ls[kSub][kM + 1];
+    __shared__ double As[kSub][kM + 1];
+    const int c = blockIdx.x;
+    const int p = ch.prob[c];
+    const int kp = pr.k[p];
+    const int m = nb * kp;
+    const int npairs = m * (m + 1) / 2;
+    const long r0 = ch.r0[c];
+    const int nr = ch.nr[c];
+    const double *Lb[3] = {X, R, P};
+    const double *Ab[3] = {AX, AR, AP};
+    int pi[kPairsPerThread], pj[kPairsPerThread];
+    double ga[kPairsPerThread], gb[kPairsPerThread];
+#pragma unroll
+    for (int s = 0; s < kPairsPerThread; s++) {
+        const int o = threadIdx.x + s * kT;
+        int j = (int)((sqrt(8.0 * o + 1.0) - 1.0) * 0.5);
+        while ((j + 1) * (j + 2) / 2 <= o) j++;
+        while (j * (j + 1) / 2 > o) j--;
+        pi[s] = o - j * (j + 1) / 2;
+        pj[s] = j;
+        ga[s] = gb[s] = 0.0;
+    }
+    for (int s0 = 0; s0 < nr; s0 += kSub) {
+        const int rows = min(kSub, nr - s0);
+        __syncthreads();
+        for (int e = threadIdx.x; e < kSub * m; e += kT) {
+            const int rr = e % kSub, q = e / kSub;
+            const int b = q / kp, col = q % kp;
+            double lv = 0.0, av = 0.0;
+            if (rr < rows) {
+                const long r = r0 + s0 + rr;
+                lv = blk_get(Lb[b], ld, r, col);
+                av = blk_get(Ab[b], ld, r, col);
+            }
+            Ls[rr][q] = lv;
+            As[rr][q] = av;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < kPairsPerThread; s++) {
+            const int o = threadIdx.x + s * kT;
+            if (o < npairs) {
+                const int i = pi[s], j = pj[s];
+                double a = ga[s], g = gb[s];
+                for (int rr = 0; rr < rows; rr++) {
+                    const double li = Ls[rr][i];
+                    a = fma(li, As[rr][j], a);
+                    g = fma(li, Ls[rr][j], g);
+                }
+                ga[s] = a;
+                gb[s] = g;
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < kPairsPerThread; s++) {
+        const int o = threadIdx.x + s * kT;
+        if (o < npairs) {
+            part[((size_t)c * 2 + 0) * kPairs + o] = ga[s];
+            part[((size_t)c * 2 + 1) * kPairs + o] = gb[s];
+        }
+    }
+}
+
+// full symmetric GA, GB (kM x kM, pencil indexing) of each active problem
+__global__ void __launch_bounds__(kT) k_bgram_sum(const int *act_prob, const int *pc_begin,
+                                                  const int *pc_count, Probs pr, int nb,
+                                                  const double *part, double *GA, double *GB) {
+    const int p = act_prob[blockIdx.x];
+    const int m = nb * pr.k[p];
+    const int npairs = m * (m + 1) / 2;
+    double *ga = GA + (size_t)p * kM * kM, *gb = GB + (size_t)p * kM * kM;
+    for (int o = threadIdx.x; o < npairs; o += kT) {
+        int j = (int)((sqrt(8.0 * o + 1.0) - 1.0) * 0.5);
+        while ((j + 1) * (j + 2) / 2 <= o) j++;
+        while (j * (j + 1) / 2 > o) j--;
+        const int i = o - j * (j + 1) / 2;
+        double a = 0.0, g = 0.0;
+        for (int c = pc_begin[p], e = pc_begin[p] + pc_count[p]; c < e; c++) {
+            a += part[((size_t)c * 2 + 0) * kPairs + o];
+            g += part[((size_t)c * 2 + 1) * kPairs + o];
+        }
+        ga[i + j * kM] = a;
+        ga[j + i * kM] = a;
+        gb[i + j * kM] = g;
+        gb[j + i * kM] = g;
+    }
+}
+
+// ---- batched Rayleigh-Ritz, one workgroup per problem ----
+// Solves GA v = theta GB v on the leading m = nb kp rows / columns; writes
+// the kp smallest theta to lam and the GB-orthonormal v to C (pencil rows,
+// kp columns, leading dim kM), nbe[p] = blocks used.  A GB that is not
+// positive definite is retried without P (nb = 3 -> 2); if that fails too,
+// status[p] = 1 and X is left as it is (C = [I; 0]).
+__global__ void __launch_bounds__(kT) k_brr(const int *act_prob, Probs pr, int nb, const double *GA,
+                                            const double *GB, double *lam, double *C, int *nbe,
+                                            int *status) {
+    __shared__ double A[kM][kM + 1], B[kM][kM + 1], V[kM][kM + 1];
+    __shared__ double sc[kM], ev[kM], cs[kM / 2], sn[kM / 2];
+    __shared__ int order[kM], top[kM / 2], bot[kM / 2];
+    __shared__ int fail, mm;
+    const int p = act_prob[blockIdx.x];
+    const int kp = pr.k[p];
+    const int tid = threadIdx.x;
+    const double *ga = GA + (size_t)p * kM * kM, *gb = GB + (size_t)p * kM * kM;
+    for (int nbt = nb; nbt >= 1; nbt--) {
+        const int m = nbt * kp;
+        // load and scale to a unit diagonal of B
+        for (int e = tid; e < m * m; e += kT) {
+            const int i = e % m, j = e / m;
+            A[i][j] = ga[i + j * kM];
+            B[i][j] = gb[i + j * kM];
+        }
+        if (tid == 0) fail = 0;
+        __syncthreads();
+        if (tid < m) {
+            const double d = B[tid][tid];
+            if (!(d > 0.0)) fail = 1;
+            sc[tid] = d > 0.0 ? 1.0 / sqrt(d) : 0.0;
+        }
+        __syncthreads();
+        for (int e = tid; e < m * m; e += kT) {
+            const int i = e % m, j = e / m;
+            A[i][j] *= sc[i] * sc[j];
+            B[i][j] *= sc[i] * sc[j];
+        }
+        __syncthreads();
+        // Cholesky B = L L' (lower, in place), right-looking
+        for (int j = 0; j < m && !fail; j++) {
+            if (tid == 0) {
+                const double d = B[j][j];
+                if (!(d > 1e-14)) fail = 1;
+                B[j][j] = d > 0.0 ? sqrt(d) : 1.0;
+            }
+            __syncthreads();
+            if (fail) break;
+            for (int i = j + 1 + tid; i < m; i += kT) B[i][j] /= B[j][j];
+            __syncthreads();
+            for (int e = tid; e < (m - j - 1) * (m - j - 1); e += kT) {
+                const int i = j + 1 + e % (m - j - 1), q = j + 1 + e / (m - j - 1);
+                if (q <= i) B[i][q] -= B[i][j] * B[q][j];
+            }
+            __syncthreads();
+        }
+        __syncthreads();
+        if (fail) {
+            __syncthreads();
+            continue;
+        }
+        // A <- L^-1 A L^-T: columns of L^-1 A (thread per column), then rows
+        for (int col = tid; col < m; col += kT)
+            for (int i = 0; i < m; i++) {
+                double s = A[i][col];
+                for (int q = 0; q < i; q++) s -= B[i][q] * A[q][col];
+                A[i][col] = s / B[i][i];
+            }
+        __syncthreads();
+        for (int row = tid; row < m; row += kT)
+            for (int i = 0; i < m; i++) {
+                double s = A[row][i];
+                for (int q = 0; q < i; q++) s -= B[i][q] * A[row][q];
+                A[row][i] = s / B[i][i];
+            }
+        __syncthreads();
+        for (int e = tid; e < m * m; e += kT) {
+            const int i = e % m, j = e / m;
+            if (i < j) {
+                const double s = 0.5 * (A[i][j] + A[j][i]);
+                A[i][j] = s;
+                A[j][i] = s;
+            }
+            V[i][j] = i == j ? 1.0 : 0.0;
+        }
+        // cyclic Jacobi, round-robin pairs (m padded to even with an identity row)
+        const int me = m + (m & 1);
+        if (tid == 0) mm = me;
+        if (m & 1) {
+            for (int e = tid; e < me; e += kT) {
+                A[m][e] = 0.0;
+                A[e][m] = 0.0;
+                V[m][e] = 0.0;
+                V[e][m] = 0.0;
+            }
+        }
+        __syncthreads();
+        if (tid == 0 && (m & 1)) {
+            A[m][m] = 1e300;  // decoupled: never rotates
+            V[m][m] = 1.0;
+        }
+        const int h = me / 2;
+        if (tid < h) {
+            top[tid] = 2 * tid;
+            bot[tid] = 2 * tid + 1;
+        }
+        __syncthreads();
+        for (int sweep = 0; sweep < 30; sweep++) {
+            // convergence: off-diagonal mass against the diagonal
+            __shared__ double offs, dia;
+            if (tid == 0) {
+                double o = 0.0, d = 0.0;
+                for (int i = 0; i < m; i++)
+                    for (int j = 0; j < m; j++)
+                        if (i != j) o += A[i][j] * A[i][j];
+                        else d += A[i][i] * A[i][i];
+                offs = o;
+                dia = d;
+            }
+            __syncthreads();
+            if (offs <= 1e-30 * dia) break;
+            for (int step = 0; step < me - 1; step++) {
+                if (tid < h) {
+                    const int a = min(top[tid], bot[tid]), b = max(top[tid], bot[tid]);
+                    const double apq = A[a][b];
+                    double c = 1.0, s = 0.0;
+                    if (apq != 0.0) {
+                        const double tau = (A[b][b] - A[a][a]) / (2.0 * apq);
+                        const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                        c = 1.0 / sqrt(1.0 + t * t);
+                        s = t * c;
+                    }
+                    cs[tid] = c;
+                    sn[tid] = s;
+                }
+                __syncthreads();
+                // rows a, b of A
+                for (int e = tid; e < h * me; e += kT) {
+                    const int pp = e / me, col = e % me;
+                    const int a = min(top[pp], bot[pp]), b = max(top[pp], bot[pp]);
+                    const double c = cs[pp], s = sn[pp];
+                    const double xa = A[a][col], xb = A[b][col];
+                    A[a][col] = c * xa - s * xb;
+                    A[b][col] = s * xa + c * xb;
+                }
+                __syncthreads();
+                // columns a, b of A and V
+                for (int e = tid; e < h * me; e += kT) {
+                    const int pp = e / me, row = e % me;
+                    const int a = min(top[pp], bot[pp]), b = max(top[pp], bot[pp]);
+                    const double c = cs[pp], s = sn[pp];
+                    const double xa = A[row][a], xb = A[row][b];
+                    A[row][a] = c * xa - s * xb;
+                    A[row][b] = s * xa + c * xb;
+                    const double va = V[row][a], vb = V[row][b];
+                    V[row][a] = c * va - s * vb;
+                    V[row][b] = s * va + c * vb;
+                }
+                __syncthreads();
+                // next round-robin pairing (position 0 of top stays; a single
+                // pair never changes)
+                if (tid == 0 && h > 1) {
+                    const int tl = top[h - 1], b0 = bot[0];
+                    for (int q = h - 1; q > 1; q--) top[q] = top[q - 1];
+                    top[1] = b0;
+                    for (int q = 0; q < h - 1; q++) bot[q] = bot[q + 1];
+                    bot[h - 1] = tl;
+                }
+                __syncthreads();
+            }
+        }
+        // ascending order of the m eigenvalues (selection by thread 0)
+        if (tid == 0) {
+            for (int i = 0; i < m; i++) {
+                ev[i] = A[i][i];
+                order[i] = i;
+            }
+            for (int i = 0; i < kp; i++) {
+                int b = i;
+                for (int j = i + 1; j < m; j++)
+                    if (ev[order[j]] < ev[order[b]]) b = j;
+                const int t = order[i];
+                order[i] = order[b];
+                order[b] = t;
+            }
+        }
+        __syncthreads();
+        // C = S L^-T V(:, order[0..kp)): back substitution per output column
+        double *cp = C + (size_t)p * kM * kK;
+        for (int j = tid; j < kp; j += kT) {
+            const int src = order[j];
+            double y[kM];
+            for (int i = m - 1; i >= 0; i--) {
+                double s = V[i][src];
+                for (int q = i + 1; q < m; q++) s -= B[q][i] * y[q];
+                y[i] = s / B[i][i];
+            }
+            for (int i = 0; i < kM; i++) cp[i + j * kM] = i < m ? y[i] * sc[i] : 0.0;
+            lam[p * kK + j] = ev[src];
+        }
+        if (tid == 0) {
+            nbe[p] = nbt;
+            status[p] = 0;
+        }
+        return;
+    }
+    // no positive definite pencil: keep X (C = [I; 0])
+    double *cp = C + (size_t)p * kM * kK;
+    for (int e = tid; e < kM * kK; e += kT) cp[e] = (e % kM) == (e / kM) ? 1.0 : 0.0;
+    if (tid == 0) {
+        nbe[p] = 1;
+        status[p] = 1;
+    }
+}
+
+// ---- block update, row by row ----
+// X' = sum_q C[q, j] L_q (q over the nbe blocks' columns), P' = the part with
+// q >= kp; the same for the A-side blocks
+__global__ void __launch_bounds__(kT) k_bupd(Chunks ch, Probs pr, long ld, const double *C,
+                                             const int *nbe, double *X, double *R, double *P,
+                                             double *AX, double *AR, double *AP) {
+    __shared__ double cs[kM * kK];
+    const int c = blockIdx.x;
+    const int p = ch.prob[c];
+    const int kp = pr.k[p];
+    const int nb = nbe[p];
+    const int m = nb * kp;
+    const long r0 = ch.r0[c];
+    const int nr = ch.nr[c];
+    for (int e = threadIdx.x; e < kM * kK; e += kT) cs[e] = C[(size_t)p * kM * kK + e];
+    __syncthreads();
+    for (int side = 0; side < 2; side++) {
+        double *b0 = side ? AX : X, *b1 = side ? AR : R, *b2 = side ? AP : P;
+        double *bl[3] = {b0, b1, b2};
+        for (int t = threadIdx.x; t < nr; t += kT) {
+            const long r = r0 + t;
+            double in[kM];
+            for (int q = 0; q < m; q++) in[q] = blk_get(bl[q / kp], ld, r, q % kp);
+            for (int j = 0; j < kp; j++) {
+                double x = 0.0, pp = 0.0;
+                for (int q = 0; q < kp; q++) x = fma(cs[q + j * kM], in[q], x);
+                for (int q = kp; q < m; q++) pp = fma(cs[q + j * kM], in[q], pp);
+                if (nb > 1) b2[(size_t)j * ld + r] = pp;
+                b0[(size_t)j * ld + r] = x + pp;
+            }
+        }
+    }
+}
+
+// ---- X0: drand48 stream, problem by problem, column by column ----
+constexpr uint64_t kM48 = (1ull << 48) - 1, kA48 = 0x5DEECE66Dull, kC48 = 0xBull;
+__device__ uint64_t jump48(uint64_t x, uint64_t steps) {
+    uint64_t a = kA48, c = kC48;
+    while (steps) {
+        if (steps & 1) x = (a * x + c) & kM48;
+        c = (a * c + c) & kM48;
+        a = (a * a) & kM48;
+        steps >>= 1;
+    }
+    return x;
+}
+__global__ void __launch_bounds__(kT) k_bdrand(Chunks ch, Probs pr, long ld, const long *sbase,
+                                               uint64_t x0, double *X) {
+    const int c = blockIdx.x;
+    const int p = ch.prob[c];
+    const int kp = pr.k[p], np = pr.n[p];
+    const long r0 = ch.r0[c], off = pr.off[p];
+    for (int j = 0; j < kp; j++)
+        for (int t = threadIdx.x; t < ch.nr[c]; t += kT) {
+            const long r = r0 + t;
+            const uint64_t pos = (uint64_t)sbase[p] + (uint64_t)j * np + (uint64_t)(r - off);
+            const uint64_t s = jump48(x0, pos + 1);
+            X[(size_t)j * ld + r] = 2.0 * ((double)s * (1.0 / 281474976710656.0)) - 1.0;
+        }
+}
+
+template <typename T>
+T *dev_alloc(size_t count) {
+    T *p = nullptr;
+    BX_HIP(hipMalloc((void **)&p, (count ? count : 1) * sizeof(T)));
+    return p;
+}
+
+}  // namespace
+}  // namespace bx
+
+using namespace bx;
+
+extern "C" {
+
+void hip_BatchSettingsDefault(hip_BatchSettings *s) {
+    if (!s) return;
+    s->tol = 1e-6;
+    s->maxit = 500;
+    s->jacobi = 1;
+    s->seed = 1;
+    s->graph = 1;
+}
+
+int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_BatchSettings *set,
+                           double *lambda, double *resnorm, int *iterations, int *flags,
+                           double *Xout, hip_BatchStats *st) {
+    hip_BatchSettings s;
+    hip_BatchSettingsDefault(&s);
+    if (set) s = *set;
+    if (nprob < 1 || !probs || !lambda || !resnorm || !iterations || !flags) {
+        set_error("hip_LobpcgSolveBatched: need nprob >= 1 and the output arrays");
+        return -1;
+    }
+    long N = 0, nnz = 0;
+    int K = 1;
+    for (int p = 0; p < nprob; p++) {
+        const hip_BatchProblem &q = probs[p];
+        if (q.k < 1 || q.k > kK || q.n < 3L * q.k || !q.rowptr || !q.colind || !q.val ||
+            q.nnz != q.rowptr[q.n]) {
+            set_error("hip_LobpcgSolveBatched: problem %d: need 1 <= k <= %d, n >= 3k and a "
+                      "consistent CSR (n %ld, k %d, nnz %ld)",
+                      p, kK, q.n, q.k, q.nnz);
+            return -1;
+        }
+        N += q.n;
+        nnz += q.nnz;
+        K = std::max(K, q.k);
+    }
+    if (N > 0x7fffffffL || nnz > 0x7fffffffL) {
+        set_error("hip_LobpcgSolveBatched: %ld rows / %ld entries exceed int32 indices", N, nnz);
+        return -1;
+    }
+    // stacked block-diagonal CSR, Jacobi, per-problem data, chunks
+    std::vector<int> rowptr(N + 1), colind(nnz);
+    std::vector<double> val(nnz), dinv(N);
+    std::vector<long> off(nprob), sbase(nprob);
+    std::vector<int> pn(nprob), pk(nprob);
+    std::vector<int> chunk_prob;
+    std::vector<long> chunk_r0;
+    std::vector<int> chunk_nr, chunk_first(nprob), chunk_cnt(nprob);
+    long r = 0, e = 0, sb = 0;
+    for (int p = 0; p < nprob; p++) {
+        const hip_BatchProblem &q = probs[p];
+        off[p] = r;
+        sbase[p] = sb;
+        sb += q.n * q.k;
+        pn[p] = (int)q.n;
+        pk[p] = q.k;
+        for (long i = 0; i < q.n; i++) {
+            rowptr[r + i] = (int)(e + q.rowptr[i]);
+            double d = 0.0;
+            for (int t = q.rowptr[i]; t < q.rowptr[i + 1]; t++) {
+                if (q.colind[t] < 0 || q.colind[t] >= q.n) {
+                    set_error("hip_LobpcgSolveBatched: problem %d row %ld: column %d outside "
+                              "0..%ld",
+                              p, i, q.colind[t], q.n - 1);
+                    return -1;
+                }
+                colind[e + t] = (int)(r + q.colind[t]);
+                val[e + t] = q.val[t];
+                if (q.colind[t] == i) d = q.val[t];
+            }
+            dinv[r + i] = d != 0.0 ? 1.0 / d : 1.0;
+        }
+        chunk_first[p] = (int)chunk_prob.size();
+        for (long c0 = 0; c0 < q.n; c0 += kChunk) {
+            chunk_prob.push_back(p);
+            chunk_r0.push_back(r + c0);
+            chunk_nr.push_back((int)std::min<long>(kChunk, q.n - c0));
+        }
+        chunk_cnt[p] = (int)chunk_prob.size() - chunk_first[p];
+        r += q.n;
+        e += q.nnz;
+    }
+    rowptr[N] = (int)nnz;
+    const auto t0 = std::chrono::steady_clock::now();
+    hip_CSRMatrix *A = hip_CSRMatrixCreate(N, nnz, rowptr.data(), colind.data(), val.data());
+    if (!A) return -1;
+    const int nch = (int)chunk_prob.size();
+    const size_t nk = (size_t)N * kK;
+    double *X = dev_alloc<double>(nk), *AX = dev_alloc<double>(nk), *R = dev_alloc<double>(nk),
+           *AR = dev_alloc<double>(nk), *P = dev_alloc<double>(nk), *AP = dev_alloc<double>(nk);
+    for (double *b : {X, AX, R, AR, P, AP}) BX_HIP(hipMemsetAsync(b, 0, nk * 8, stream()));
+    double *dv = dev_alloc<double>(N);
+    upload_async(dv, dinv.data(), N * 8);
+    long *d_off = dev_alloc<long>(nprob), *d_sbase = dev_alloc<long>(nprob);
+    int *d_n = dev_alloc<int>(nprob), *d_k = dev_alloc<int>(nprob);
+    upload_async(d_off, off.data(), nprob * 8);
+    upload_async(d_sbase, sbase.data(), nprob * 8);
+    upload_async(d_n, pn.data(), nprob * 4);
+    upload_async(d_k, pk.data(), nprob * 4);
+    int *c_prob = dev_alloc<int>(nch), *c_nr = dev_alloc<int>(nch);
+    long *c_r0 = dev_alloc<long>(nch);
+    int *d_actp = dev_alloc<int>(nprob), *d_pcb = dev_alloc<int>(nprob),
+        *d_pcc = dev_alloc<int>(nprob);
+    double *lam = dev_alloc<double>((size_t)nprob * kK), *res2 = dev_alloc<double>((size_t)nprob * kK);
+    double *GA = dev_alloc<double>((size_t)nprob * kM * kM), *GB = dev_alloc<double>((size_t)nprob * kM * kM);
+    double *Cm = dev_alloc<double>((size_t)nprob * kM * kK);
+    int *nbe = dev_alloc<int>(nprob), *status = dev_alloc<int>(nprob);
+    double *part_r = dev_alloc<double>((size_t)nch * kK);
+    double *part_g = dev_alloc<double>((size_t)nch * 2 * kPairs);
+    BX_HIP(hipMemsetAsync(lam, 0, (size_t)nprob * kK * 8, stream()));
+    BX_HIP(hipMemsetAsync(status, 0, nprob * 4, stream()));
+
+    Probs pr{d_off, d_n, d_k};
+    Chunks ch{c_prob, c_r0, c_nr};
+    std::vector<char> active(nprob, 1);
+    std::vector<int> flag_h(nprob, -1), iters_h(nprob, 0);
+    int nact_ch = 0, nact = 0;
+    // active chunk list (chunks of a problem contiguous) and per-problem ranges
+    auto rebuild = [&]() {
+        std::vector<int> cp, cn, actp, pcb(nprob, 0), pcc(nprob, 0);
+        std::vector<long> cr;
+        for (int p = 0; p < nprob; p++) {
+            if (!active[p]) continue;
+            actp.push_back(p);
+            pcb[p] = (int)cp.size();
+            for (int c = chunk_first[p]; c < chunk_first[p] + chunk_cnt[p]; c++) {
+                cp.push_back(chunk_prob[c]);
+                cr.push_back(chunk_r0[c]);
+                cn.push_back(chunk_nr[c]);
+            }
+            pcc[p] = (int)cp.size() - pcb[p];
+        }
+        nact_ch = (int)cp.size();
+        nact = (int)actp.size();
+        if (nact_ch) {
+            upload_async(c_prob, cp.data(), cp.size() * 4);
+            upload_async(c_r0, cr.data(), cr.size() * 8);
+            upload_async(c_nr, cn.data(), cn.size() * 4);
+        }
+        if (nact) upload_async(d_actp, actp.data(), actp.size() * 4);
+        upload_async(d_pcb, pcb.data(), nprob * 4);
+        upload_async(d_pcc, pcc.data(), nprob * 4);
+    };
+    rebuild();
+    const ColMap all = colmap_identity(kK);
+    auto step = [&](int nb, bool with_spmm) {
+        if (with_spmm) spmm_csr(HIPMV_FP64, A, R, N, all, AR, N, all);
+        hipLaunchKernelGGL(k_bgram, dim3(nact_ch), dim3(kT), 0, stream(), ch, pr, N, nb, X, R, P,
+                           AX, AR, AP, part_g);
+        hipLaunchKernelGGL(k_bgram_sum, dim3(nact), dim3(kT), 0, stream(), d_actp, d_pcb, d_pcc,
+                           pr, nb, part_g, GA, GB);
+        hipLaunchKernelGGL(k_brr, dim3(nact), dim3(kT), 0, stream(), d_actp, pr, nb, GA, GB, lam,
+                           Cm, nbe, status);
+        hipLaunchKernelGGL(k_bupd, dim3(nact_ch), dim3(kT), 0, stream(), ch, pr, N, Cm, nbe, X, R,
+                           P, AX, AR, AP);
+        hipLaunchKernelGGL(k_bresid, dim3(nact_ch), dim3(kT), 0, stream(), ch, pr, N, X, AX, lam,
+                           s.jacobi ? dv : nullptr, R, part_r);
+        hipLaunchKernelGGL(k_bres_sum, dim3(nact), dim3(64), 0, stream(), d_actp, d_pcb, d_pcc,
+                           part_r, res2);
+        BX_HIP(hipGetLastError());
+    };
+    // X0, AX, Rayleigh-Ritz in span(X), first residuals
+    const uint64_t x0 = (((uint64_t)(uint32_t)s.seed) << 16) | 0x330Eull;
+    hipLaunchKernelGGL(k_bdrand, dim3(nch), dim3(kT), 0, stream(), ch, pr, N, d_sbase, x0, X);
+    BX_HIP(hipGetLastError());
+    spmm_csr(HIPMV_FP64, A, X, N, all, AX, N, all);
+    step(1, false);
+
+    std::vector<double> res_h((size_t)nprob * kK), lam_h((size_t)nprob * kK);
+    std::vector<int> st_h(nprob);
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int captures = 0, replays = 0, it = 0;
+    bool all_done = false;
+    for (;;) {
+        download_sync(res_h.data(), res2, res_h.size() * 8);
+        download_sync(st_h.data(), status, nprob * 4);
+        bool changed = false;
+        for (int p = 0; p < nprob; p++) {
+            if (!active[p]) continue;
+            bool conv = true;
+            for (int j = 0; j < pk[p]; j++) conv = conv && sqrt(fabs(res_h[p * kK + j])) <= s.tol;
+            if (conv || st_h[p]) {
+                active[p] = 0;
+                changed = true;
+                flag_h[p] = conv ? 0 : 3;
+                iters_h[p] = it;
+            }
+        }
+        all_done = std::none_of(active.begin(), active.end(), [](char a) { return a != 0; });
+        if (all_done || it >= s.maxit) break;
+        if (changed) {
+            rebuild();
+            if (exec) {
+                BX_HIP(hipGraphExecDestroy(exec));
+                BX_HIP(hipGraphDestroy(graph));
+                exec = nullptr;
+                graph = nullptr;
+            }
+        }
+        it++;
+        const int nb = it == 1 ? 2 : 3;
+        if (s.graph && nb == 3) {
+            if (!exec) {
+                BX_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeRelaxed));
+                step(nb, true);
+                BX_HIP(hipStreamEndCapture(stream(), &graph));
+                BX_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+                captures++;
+            } else {
+                replays++;
+            }
+            BX_HIP(hipGraphLaunch(exec, stream()));
+        } else {
+            step(nb, true);
+        }
+    }
+    for (int p = 0; p < nprob; p++)
+        if (active[p]) iters_h[p] = it;
+    if (exec) {
+        BX_HIP(hipGraphExecDestroy(exec));
+        BX_HIP(hipGraphDestroy(graph));
+    }
+    download_sync(lam_h.data(), lam, lam_h.size() * 8);
+    download_sync(res_h.data(), res2, res_h.size() * 8);
+    size_t o = 0, xo = 0;
+    std::vector<double> col;
+    for (int p = 0; p < nprob; p++) {
+        for (int j = 0; j < pk[p]; j++) {
+            lambda[o + j] = lam_h[p * kK + j];
+            resnorm[o + j] = sqrt(fabs(res_h[p * kK + j]));
+        }
+        o += pk[p];
+        iterations[p] = iters_h[p];
+        flags[p] = flag_h[p];
+        if (Xout) {
+            for (int j = 0; j < pk[p]; j++)
+                download_sync(Xout + xo + (size_t)j * pn[p], X + (size_t)j * N + off[p],
+                              (size_t)pn[p] * 8);
+            xo += (size_t)pn[p] * pk[p];
+        }
+    }
+    BX_HIP(hipStreamSynchronize(stream()));
+    const double secs =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (void *b : {(void *)X, (void *)AX, (void *)R, (void *)AR, (void *)P, (void *)AP,
+                    (void *)dv, (void *)d_off, (void *)d_sbase, (void *)d_n, (void *)d_k,
+                    (void *)c_prob, (void *)c_nr, (void *)c_r0, (void *)d_actp, (void *)d_pcb,
+                    (void *)d_pcc, (void *)lam, (void *)res2, (void *)GA, (void *)GB, (void *)Cm,
+                    (void *)nbe, (void *)status, (void *)part_r, (void *)part_g})
+        BX_HIP(hipFree(b));
+    hip_CSRMatrixDestroy(A);
+    if (st) {
+        st->seconds = secs;
+        st->iterations = it;
+        st->graph_captures = captures;
+        st->graph_replays = replays;
+        st->total_rows = N;
+    }
+    bool ok = true;
+    for (int p = 0; p < nprob; p++) ok = ok && flag_h[p] == 0;
+    return ok ? 0 : -1;
+}
+
+}  // extern "C"

@@ -313,3 +313,52 @@ def lobpcg_complex(X0: np.ndarray, A: np.ndarray, tol: float = 1e-6, maxit: int 
     finally:
         for o in reversed(res_objs):
             o.close()
+
+
+@dataclass
+class BatchResult:
+    lam: list          # per problem, k eigenvalues (ascending)
+    res: list          # per problem, k residual norms
+    X: list | None     # per problem, n x k eigenvectors
+    iterations: np.ndarray
+    flags: np.ndarray
+    stats: dict = field(default_factory=dict)
+
+
+def lobpcg_batched(problems, tol: float = 1e-6, maxit: int = 500, jacobi: bool = True,
+                   seed: int = 1, graph: bool = True, want_X: bool = False) -> BatchResult:
+    """Smallest eigenpairs of many independent problems in one device solve
+    (hip_LobpcgSolveBatched).  problems: list of (CSR, k)."""
+    lib = L.load()
+    arr = (L.BatchProblem * len(problems))()
+    keep = []
+    for i, (A, k) in enumerate(problems):
+        keep.append(A)
+        arr[i] = L.BatchProblem(A.n, A.nnz, A.rowptr.ctypes.data, A.colind.ctypes.data,
+                                A.val.ctypes.data, k)
+    s = L.BatchSettings()
+    lib.hip_BatchSettingsDefault(ctypes.byref(s))
+    s.tol, s.maxit, s.jacobi, s.seed, s.graph = tol, maxit, int(jacobi), seed, int(graph)
+    ks = [k for _, k in problems]
+    lam = np.zeros(sum(ks))
+    res = np.zeros(sum(ks))
+    its = np.zeros(len(problems), dtype=np.int32)
+    flags = np.zeros(len(problems), dtype=np.int32)
+    X = np.zeros(sum(A.n * k for A, k in problems)) if want_X else None
+    st = L.BatchStats()
+    rc = lib.hip_LobpcgSolveBatched(len(problems), arr, ctypes.byref(s), lam.ctypes.data,
+                                    res.ctypes.data, its.ctypes.data, flags.ctypes.data,
+                                    None if X is None else X.ctypes.data, ctypes.byref(st))
+    if rc < 0 and np.all(flags == 0):  # an argument error (no problem was run)
+        raise RuntimeError(L.last_error())
+    lam_l, res_l, X_l = [], [], [] if want_X else None
+    o = xo = 0
+    for A, k in problems:
+        lam_l.append(lam[o:o + k].copy())
+        res_l.append(res[o:o + k].copy())
+        if want_X:
+            X_l.append(X[xo:xo + A.n * k].reshape((k, A.n)).T.copy())
+            xo += A.n * k
+        o += k
+    return BatchResult(lam_l, res_l, X_l, its, flags,
+                       {f: getattr(st, f) for f, _ in L.BatchStats._fields_})

@@ -147,6 +147,23 @@ def laplacian_5pt(nx: int, ny: int, sigma: float = 0.0, seed: int = 0, stream: i
     return diffusion_stencil((nx, ny), c)
 
 
+def batched_problems(count: int, smin: int = 32, smax: int = 256, ks=(4, 8, 16),
+                     sigma: float = 0.5, seed: int = 4) -> list[tuple[CSR, int]]:
+    """The batched workload: count independent 2D 5-point diffusion problems,
+    side s_i uniform in [smin, smax], cell coefficients lognormal(0, sigma)
+    (problem i draws from stream i), block size k_i uniform over ks.  Sides
+    and block sizes come from the counter-based generator (streams 500, 501)."""
+    idx = np.arange(count, dtype=np.uint64)
+    us = _uniform01(seed, 500, idx)
+    uk = _uniform01(seed, 501, idx)
+    out = []
+    for i in range(count):
+        sd = smin + int(us[i] * (smax - smin + 1))
+        k = ks[int(uk[i] * len(ks))]
+        out.append((laplacian_5pt(sd, sd, sigma=sigma, seed=seed, stream=1 + i), k))
+    return out
+
+
 def dirichlet_eigenvalues(dims: tuple[int, ...], m: int) -> np.ndarray:
     """Smallest m eigenvalues of the constant-coefficient Dirichlet stencil:
     sum over axes of 4 sin^2(pi j / (2 (N + 1))), j = 1..N

@@ -1,0 +1,65 @@
+/* Many independent eigenproblems in one solve: every iteration advances all
+ * unconverged problems together, one grouped launch per step (residual and
+ * Jacobi preconditioner, SpMM, Gram blocks, Rayleigh-Ritz, block update), and
+ * the launches of an iteration are replayed from a captured hipGraph.
+ *
+ * No counterpart in the reference, which solves one problem per call
+ * (lobpcg_solve, krylov/lobpcg.c:502-1388).  Per problem the iteration is
+ * LOBPCG's: Rayleigh-Ritz on [X, T R, P] (lobpcg.c:1085-1190) with the
+ * initial Rayleigh-Ritz in span(X) (lobpcg.c:842-905).  Columns are not
+ * locked one by one: a problem keeps all k columns in its basis until every
+ * residual norm ||A x_i - lambda_i x_i|| is below the tolerance, then it
+ * leaves the batch (the convergence test of lobpcg_checkResiduals,
+ * lobpcg.c:265-294, applied to the whole block). */
+#ifndef BLOPEX_AMD_LOBPCG_BATCHED_H
+#define BLOPEX_AMD_LOBPCG_BATCHED_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HIP_BATCH_MAX_K 16
+
+typedef struct hip_BatchProblem {
+    long n;             /* rows */
+    long nnz;
+    const int *rowptr;  /* host CSR, symmetric positive definite A */
+    const int *colind;
+    const double *val;
+    int k;              /* eigenpairs wanted, 1..HIP_BATCH_MAX_K */
+} hip_BatchProblem;
+
+typedef struct hip_BatchSettings {
+    double tol;         /* absolute residual tolerance (default 1e-6) */
+    int maxit;          /* iterations (default 500) */
+    int jacobi;         /* 1: T = diag(A)^-1 (default 1) */
+    int seed;           /* X0: drand48 after srand48(seed), problem by problem, column by
+                           column (default 1) */
+    int graph;          /* 1: replay each iteration's launches from a hipGraph (default 1) */
+} hip_BatchSettings;
+
+typedef struct hip_BatchStats {
+    double seconds;     /* wall time of the solve (device-resident inputs excluded) */
+    int iterations;     /* batch iterations (the slowest problem's count) */
+    int graph_captures; /* iterations captured (the active set changed) */
+    int graph_replays;  /* iterations replayed from a captured graph */
+    long total_rows;    /* sum of n */
+} hip_BatchStats;
+
+void hip_BatchSettingsDefault(hip_BatchSettings *s);
+
+/* Solves all nprob problems.  Outputs (host): lambda and resnorm, sum of k
+   doubles (problem by problem, ascending lambda); iterations and flags,
+   nprob ints (flag 0 converged, -1 not within maxit, 3 Rayleigh-Ritz
+   breakdown); X (NULL: not returned) sum of n*k doubles, each problem's n x k
+   block column-major.  Returns 0 when every problem converged, -1 otherwise
+   (blopex_amd_last_error for argument errors). */
+int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs,
+                           const hip_BatchSettings *s, double *lambda, double *resnorm,
+                           int *iterations, int *flags, double *X, hip_BatchStats *st);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
