@@ -17,9 +17,10 @@
 //                diagonal scaling, Cholesky, reduction to standard form,
 //                cyclic Jacobi, back-substitution (FP64)
 //   k_bupd       X' = [X R P] C_X, P' = [R P] C_RP and the same for AX, AP
-// The host reads the residual norms once per iteration; while the set of
-// unconverged problems stays the same, the launches of an iteration are
-// replayed from a captured hipGraph.
+// The host reads the residual norms once per iteration.  The launches of an
+// iteration are captured once into a hipGraph and replayed: their grids cover
+// every chunk and problem, and workgroups past the active counts (device
+// memory, rewritten when problems converge) exit at once.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -44,6 +45,8 @@ struct Chunks {
     const int *prob;   // problem of each active chunk
     const long *r0;    // first (global) row
     const int *nr;     // rows
+    const int *count;  // active chunks (device): workgroups past it exit, so a captured
+                       // launch stays valid while the batch shrinks
 };
 
 struct Probs {
@@ -62,6 +65,7 @@ __global__ void __launch_bounds__(kT) k_bresid(Chunks ch, Probs pr, long ld, con
                                                const double *dinv, double *R, double *part) {
     __shared__ double red[kT];
     const int c = blockIdx.x;
+    if (c >= *ch.count) return;
     const int p = ch.prob[c];
     const int kp = pr.k[p];
     const long r0 = ch.r0[c];
@@ -91,8 +95,9 @@ __global__ void __launch_bounds__(kT) k_bresid(Chunks ch, Probs pr, long ld, con
 }
 
 // res2[p K + j] = sum of problem p's chunk partials (fixed order)
-__global__ void k_bres_sum(const int *act_prob, const int *pc_begin, const int *pc_count,
-                           const double *part, double *res2) {
+__global__ void k_bres_sum(const int *act_prob, const int *nact, const int *pc_begin,
+                           const int *pc_count, const double *part, double *res2) {
+    if ((int)blockIdx.x >= *nact) return;
     const int p = act_prob[blockIdx.x];
     const int j = threadIdx.x;
     if (j >= kK) return;
@@ -110,6 +115,7 @@ __global__ void __launch_bounds__(kT) k_bgram(Chunks ch, Probs pr, long ld, int 
     __shared__ double Ls[kSub][kM + 1];
     __shared__ double As[kSub][kM + 1];
     const int c = blockIdx.x;
+    if (c >= *ch.count) return;
     const int p = ch.prob[c];
     const int kp = pr.k[p];
     const int m = nb * kp;
@@ -173,9 +179,11 @@ __global__ void __launch_bounds__(kT) k_bgram(Chunks ch, Probs pr, long ld, int 
 }
 
 // full symmetric GA, GB (kM x kM, pencil indexing) of each active problem
-__global__ void __launch_bounds__(kT) k_bgram_sum(const int *act_prob, const int *pc_begin,
-                                                  const int *pc_count, Probs pr, int nb,
-                                                  const double *part, double *GA, double *GB) {
+__global__ void __launch_bounds__(kT) k_bgram_sum(const int *act_prob, const int *nact,
+                                                  const int *pc_begin, const int *pc_count,
+                                                  Probs pr, int nb, const double *part,
+                                                  double *GA, double *GB) {
+    if ((int)blockIdx.x >= *nact) return;
     const int p = act_prob[blockIdx.x];
     const int m = nb * pr.k[p];
     const int npairs = m * (m + 1) / 2;
@@ -197,15 +205,17 @@ __global__ void __launch_bounds__(kT) k_bgram_sum(const int *act_prob, const int
     }
 }
 
-// ---- batched Rayleigh-Ritz, one workgroup per problem ----
+// ---- batched Rayleigh-Ritz, one workgroup (a single wave) per problem ----
+constexpr int kRT = 64;
 // Solves GA v = theta GB v on the leading m = nb kp rows / columns; writes
 // the kp smallest theta to lam and the GB-orthonormal v to C (pencil rows,
 // kp columns, leading dim kM), nbe[p] = blocks used.  A GB that is not
 // positive definite is retried without P (nb = 3 -> 2); if that fails too,
 // status[p] = 1 and X is left as it is (C = [I; 0]).
-__global__ void __launch_bounds__(kT) k_brr(const int *act_prob, Probs pr, int nb, const double *GA,
-                                            const double *GB, double *lam, double *C, int *nbe,
-                                            int *status) {
+__global__ void __launch_bounds__(kRT) k_brr(const int *act_prob, const int *nact, Probs pr, int nb,
+                                             const double *GA, const double *GB, double *lam,
+                                             double *C, int *nbe, int *status) {
+    if ((int)blockIdx.x >= *nact) return;
     __shared__ double A[kM][kM + 1], B[kM][kM + 1], V[kM][kM + 1];
     __shared__ double sc[kM], ev[kM], cs[kM / 2], sn[kM / 2];
     __shared__ int order[kM], top[kM / 2], bot[kM / 2];
@@ -217,7 +227,7 @@ __global__ void __launch_bounds__(kT) k_brr(const int *act_prob, Probs pr, int n
     for (int nbt = nb; nbt >= 1; nbt--) {
         const int m = nbt * kp;
         // load and scale to a unit diagonal of B
-        for (int e = tid; e < m * m; e += kT) {
+        for (int e = tid; e < m * m; e += kRT) {
             const int i = e % m, j = e / m;
             A[i][j] = ga[i + j * kM];
             B[i][j] = gb[i + j * kM];
@@ -230,7 +240,7 @@ __global__ void __launch_bounds__(kT) k_brr(const int *act_prob, Probs pr, int n
             sc[tid] = d > 0.0 ? 1.0 / sqrt(d) : 0.0;
         }
         __syncthreads();
-        for (int e = tid; e < m * m; e += kT) {
+        for (int e = tid; e < m * m; e += kRT) {
             const int i = e % m, j = e / m;
             A[i][j] *= sc[i] * sc[j];
             B[i][j] *= sc[i] * sc[j];
@@ -245,9 +255,9 @@ __global__ void __launch_bounds__(kT) k_brr(const int *act_prob, Probs pr, int n
             }
             __syncthreads();
             if (fail) break;
-            for (int i = j + 1 + tid; i < m; i += kT) B[i][j] /= B[j][j];
+            for (int i = j + 1 + tid; i < m; i += kRT) B[i][j] /= B[j][j];
             __syncthreads();
-            for (int e = tid; e < (m - j - 1) * (m - j - 1); e += kT) {
+            for (int e = tid; e < (m - j - 1) * (m - j - 1); e += kRT) {
                 const int i = j + 1 + e % (m - j - 1), q = j + 1 + e / (m - j - 1);
                 if (q <= i) B[i][q] -= B[i][j] * B[q][j];
             }
@@ -259,21 +269,21 @@ __global__ void __launch_bounds__(kT) k_brr(const int *act_prob, Probs pr, int n
             continue;
         }
         // A <- L^-1 A L^-T: columns of L^-1 A (thread per column), then rows
-        for (int col = tid; col < m; col += kT)
+        for (int col = tid; col < m; col += kRT)
             for (int i = 0; i < m; i++) {
                 double s = A[i][col];
                 for (int q = 0; q < i; q++) s -= B[i][q] * A[q][col];
                 A[i][col] = s / B[i][i];
             }
         __syncthreads();
-        for (int row = tid; row < m; row += kT)
+        for (int row = tid; row < m; row += kRT)
             for (int i = 0; i < m; i++) {
                 double s = A[row][i];
                 for (int q = 0; q < i; q++) s -= B[i][q] * A[row][q];
                 A[row][i] = s / B[i][i];
             }
         __syncthreads();
-        for (int e = tid; e < m * m; e += kT) {
+        for (int e = tid; e < m * m; e += kRT) {
             const int i = e % m, j = e / m;
             if (i < j) {
                 const double s = 0.5 * (A[i][j] + A[j][i]);
@@ -286,7 +296,7 @@ __global__ void __launch_bounds__(kT) k_brr(const int *act_prob, Probs pr, int n
         const int me = m + (m & 1);
         if (tid == 0) mm = me;
         if (m & 1) {
-            for (int e = tid; e < me; e += kT) {
+            for (int e = tid; e < me; e += kRT) {
                 A[m][e] = 0.0;
                 A[e][m] = 0.0;
                 V[m][e] = 0.0;
@@ -304,37 +314,31 @@ __global__ void __launch_bounds__(kT) k_brr(const int *act_prob, Probs pr, int n
             bot[tid] = 2 * tid + 1;
         }
         __syncthreads();
-        for (int sweep = 0; sweep < 30; sweep++) {
-            // convergence: off-diagonal mass against the diagonal
-            __shared__ double offs, dia;
-            if (tid == 0) {
-                double o = 0.0, d = 0.0;
-                for (int i = 0; i < m; i++)
-                    for (int j = 0; j < m; j++)
-                        if (i != j) o += A[i][j] * A[i][j];
-                        else d += A[i][i] * A[i][i];
-                offs = o;
-                dia = d;
-            }
+        // sweeps until one makes no rotation: a pair rotates while
+        // |a_pq| > 1e-15 sqrt(|a_pp a_qq|) (the classical threshold; below it
+        // the rotation would change the eigenvalues by less than rounding)
+        __shared__ int rotated;
+        for (int sweep = 0; sweep < 40; sweep++) {
+            if (tid == 0) rotated = 0;
             __syncthreads();
-            if (offs <= 1e-30 * dia) break;
             for (int step = 0; step < me - 1; step++) {
                 if (tid < h) {
                     const int a = min(top[tid], bot[tid]), b = max(top[tid], bot[tid]);
                     const double apq = A[a][b];
                     double c = 1.0, s = 0.0;
-                    if (apq != 0.0) {
+                    if (fabs(apq) > 1e-15 * sqrt(fabs(A[a][a] * A[b][b])) && apq != 0.0) {
                         const double tau = (A[b][b] - A[a][a]) / (2.0 * apq);
                         const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
                         c = 1.0 / sqrt(1.0 + t * t);
                         s = t * c;
+                        rotated = 1;
                     }
                     cs[tid] = c;
                     sn[tid] = s;
                 }
                 __syncthreads();
                 // rows a, b of A
-                for (int e = tid; e < h * me; e += kT) {
+                for (int e = tid; e < h * me; e += kRT) {
                     const int pp = e / me, col = e % me;
                     const int a = min(top[pp], bot[pp]), b = max(top[pp], bot[pp]);
                     const double c = cs[pp], s = sn[pp];
@@ -344,7 +348,7 @@ __global__ void __launch_bounds__(kT) k_brr(const int *act_prob, Probs pr, int n
                 }
                 __syncthreads();
                 // columns a, b of A and V
-                for (int e = tid; e < h * me; e += kT) {
+                for (int e = tid; e < h * me; e += kRT) {
                     const int pp = e / me, row = e % me;
                     const int a = min(top[pp], bot[pp]), b = max(top[pp], bot[pp]);
                     const double c = cs[pp], s = sn[pp];
@@ -367,6 +371,8 @@ __global__ void __launch_bounds__(kT) k_brr(const int *act_prob, Probs pr, int n
                 }
                 __syncthreads();
             }
+            if (!rotated) break;
+            __syncthreads();
         }
         // ascending order of the m eigenvalues (selection by thread 0)
         if (tid == 0) {
@@ -386,7 +392,7 @@ __global__ void __launch_bounds__(kT) k_brr(const int *act_prob, Probs pr, int n
         __syncthreads();
         // C = S L^-T V(:, order[0..kp)): back substitution per output column
         double *cp = C + (size_t)p * kM * kK;
-        for (int j = tid; j < kp; j += kT) {
+        for (int j = tid; j < kp; j += kRT) {
             const int src = order[j];
             double y[kM];
             for (int i = m - 1; i >= 0; i--) {
@@ -405,7 +411,7 @@ __global__ void __launch_bounds__(kT) k_brr(const int *act_prob, Probs pr, int n
     }
     // no positive definite pencil: keep X (C = [I; 0])
     double *cp = C + (size_t)p * kM * kK;
-    for (int e = tid; e < kM * kK; e += kT) cp[e] = (e % kM) == (e / kM) ? 1.0 : 0.0;
+    for (int e = tid; e < kM * kK; e += kRT) cp[e] = (e % kM) == (e / kM) ? 1.0 : 0.0;
     if (tid == 0) {
         nbe[p] = 1;
         status[p] = 1;
@@ -414,34 +420,51 @@ __global__ void __launch_bounds__(kT) k_brr(const int *act_prob, Probs pr, int n
 
 // ---- block update, row by row ----
 // X' = sum_q C[q, j] L_q (q over the nbe blocks' columns), P' = the part with
-// q >= kp; the same for the A-side blocks
+// q >= kp; the same for the A-side blocks.  The output columns accumulate in
+// registers; C is read from LDS (the same address across the wave).
 __global__ void __launch_bounds__(kT) k_bupd(Chunks ch, Probs pr, long ld, const double *C,
                                              const int *nbe, double *X, double *R, double *P,
                                              double *AX, double *AR, double *AP) {
     __shared__ double cs[kM * kK];
     const int c = blockIdx.x;
+    if (c >= *ch.count) return;
     const int p = ch.prob[c];
     const int kp = pr.k[p];
     const int nb = nbe[p];
-    const int m = nb * kp;
     const long r0 = ch.r0[c];
     const int nr = ch.nr[c];
     for (int e = threadIdx.x; e < kM * kK; e += kT) cs[e] = C[(size_t)p * kM * kK + e];
     __syncthreads();
     for (int side = 0; side < 2; side++) {
         double *b0 = side ? AX : X, *b1 = side ? AR : R, *b2 = side ? AP : P;
-        double *bl[3] = {b0, b1, b2};
+        const double *bl[3] = {b0, b1, b2};
         for (int t = threadIdx.x; t < nr; t += kT) {
             const long r = r0 + t;
-            double in[kM];
-            for (int q = 0; q < m; q++) in[q] = blk_get(bl[q / kp], ld, r, q % kp);
-            for (int j = 0; j < kp; j++) {
-                double x = 0.0, pp = 0.0;
-                for (int q = 0; q < kp; q++) x = fma(cs[q + j * kM], in[q], x);
-                for (int q = kp; q < m; q++) pp = fma(cs[q + j * kM], in[q], pp);
-                if (nb > 1) b2[(size_t)j * ld + r] = pp;
-                b0[(size_t)j * ld + r] = x + pp;
+            double ax[kK], ap[kK];
+#pragma unroll
+            for (int j = 0; j < kK; j++) ax[j] = ap[j] = 0.0;
+            for (int b = 0; b < nb; b++) {
+                const double *src = bl[b];
+                for (int col = 0; col < kp; col++) {
+                    const int q = b * kp + col;
+                    const double v = src[(size_t)col * ld + r];
+                    if (b == 0) {
+#pragma unroll
+                        for (int j = 0; j < kK; j++)
+                            if (j < kp) ax[j] = fma(cs[q + j * kM], v, ax[j]);
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < kK; j++)
+                            if (j < kp) ap[j] = fma(cs[q + j * kM], v, ap[j]);
+                    }
+                }
             }
+#pragma unroll
+            for (int j = 0; j < kK; j++)
+                if (j < kp) {
+                    if (nb > 1) b2[(size_t)j * ld + r] = ap[j];
+                    b0[(size_t)j * ld + r] = ax[j] + ap[j];
+                }
         }
     }
 }
@@ -587,7 +610,7 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
     int *c_prob = dev_alloc<int>(nch), *c_nr = dev_alloc<int>(nch);
     long *c_r0 = dev_alloc<long>(nch);
     int *d_actp = dev_alloc<int>(nprob), *d_pcb = dev_alloc<int>(nprob),
-        *d_pcc = dev_alloc<int>(nprob);
+        *d_pcc = dev_alloc<int>(nprob), *d_counts = dev_alloc<int>(2);
     double *lam = dev_alloc<double>((size_t)nprob * kK), *res2 = dev_alloc<double>((size_t)nprob * kK);
     double *GA = dev_alloc<double>((size_t)nprob * kM * kM), *GB = dev_alloc<double>((size_t)nprob * kM * kM);
     double *Cm = dev_alloc<double>((size_t)nprob * kM * kK);
@@ -598,7 +621,8 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
     BX_HIP(hipMemsetAsync(status, 0, nprob * 4, stream()));
 
     Probs pr{d_off, d_n, d_k};
-    Chunks ch{c_prob, c_r0, c_nr};
+    Chunks ch{c_prob, c_r0, c_nr, d_counts};
+    const int *d_nact = d_counts + 1;
     std::vector<char> active(nprob, 1);
     std::vector<int> flag_h(nprob, -1), iters_h(nprob, 0);
     int nact_ch = 0, nact = 0;
@@ -627,23 +651,27 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
         if (nact) upload_async(d_actp, actp.data(), actp.size() * 4);
         upload_async(d_pcb, pcb.data(), nprob * 4);
         upload_async(d_pcc, pcc.data(), nprob * 4);
+        const int counts[2] = {nact_ch, nact};
+        upload_async(d_counts, counts, sizeof counts);
     };
     rebuild();
     const ColMap all = colmap_identity(kK);
+    // grids cover every chunk / problem; workgroups past the device counts
+    // exit at once, so one captured graph serves the whole solve
     auto step = [&](int nb, bool with_spmm) {
         if (with_spmm) spmm_csr(HIPMV_FP64, A, R, N, all, AR, N, all);
-        hipLaunchKernelGGL(k_bgram, dim3(nact_ch), dim3(kT), 0, stream(), ch, pr, N, nb, X, R, P,
-                           AX, AR, AP, part_g);
-        hipLaunchKernelGGL(k_bgram_sum, dim3(nact), dim3(kT), 0, stream(), d_actp, d_pcb, d_pcc,
-                           pr, nb, part_g, GA, GB);
-        hipLaunchKernelGGL(k_brr, dim3(nact), dim3(kT), 0, stream(), d_actp, pr, nb, GA, GB, lam,
-                           Cm, nbe, status);
-        hipLaunchKernelGGL(k_bupd, dim3(nact_ch), dim3(kT), 0, stream(), ch, pr, N, Cm, nbe, X, R,
-                           P, AX, AR, AP);
-        hipLaunchKernelGGL(k_bresid, dim3(nact_ch), dim3(kT), 0, stream(), ch, pr, N, X, AX, lam,
+        hipLaunchKernelGGL(k_bgram, dim3(nch), dim3(kT), 0, stream(), ch, pr, N, nb, X, R, P, AX,
+                           AR, AP, part_g);
+        hipLaunchKernelGGL(k_bgram_sum, dim3(nprob), dim3(kT), 0, stream(), d_actp, d_nact, d_pcb,
+                           d_pcc, pr, nb, part_g, GA, GB);
+        hipLaunchKernelGGL(k_brr, dim3(nprob), dim3(kRT), 0, stream(), d_actp, d_nact, pr, nb, GA,
+                           GB, lam, Cm, nbe, status);
+        hipLaunchKernelGGL(k_bupd, dim3(nch), dim3(kT), 0, stream(), ch, pr, N, Cm, nbe, X, R, P,
+                           AX, AR, AP);
+        hipLaunchKernelGGL(k_bresid, dim3(nch), dim3(kT), 0, stream(), ch, pr, N, X, AX, lam,
                            s.jacobi ? dv : nullptr, R, part_r);
-        hipLaunchKernelGGL(k_bres_sum, dim3(nact), dim3(64), 0, stream(), d_actp, d_pcb, d_pcc,
-                           part_r, res2);
+        hipLaunchKernelGGL(k_bres_sum, dim3(nprob), dim3(64), 0, stream(), d_actp, d_nact, d_pcb,
+                           d_pcc, part_r, res2);
         BX_HIP(hipGetLastError());
     };
     // X0, AX, Rayleigh-Ritz in span(X), first residuals
@@ -676,15 +704,7 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
         }
         all_done = std::none_of(active.begin(), active.end(), [](char a) { return a != 0; });
         if (all_done || it >= s.maxit) break;
-        if (changed) {
-            rebuild();
-            if (exec) {
-                BX_HIP(hipGraphExecDestroy(exec));
-                BX_HIP(hipGraphDestroy(graph));
-                exec = nullptr;
-                graph = nullptr;
-            }
-        }
+        if (changed) rebuild();
         it++;
         const int nb = it == 1 ? 2 : 3;
         if (s.graph && nb == 3) {
@@ -737,6 +757,7 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
                     (void *)nbe, (void *)status, (void *)part_r, (void *)part_g})
         BX_HIP(hipFree(b));
     hip_CSRMatrixDestroy(A);
+    BX_HIP(hipFree(d_counts));
     if (st) {
         st->seconds = secs;
         st->iterations = it;

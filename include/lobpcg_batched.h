@@ -1,7 +1,7 @@
 /* Many independent eigenproblems in one solve: every iteration advances all
  * unconverged problems together, one grouped launch per step (residual and
  * Jacobi preconditioner, SpMM, Gram blocks, Rayleigh-Ritz, block update), and
- * the launches of an iteration are replayed from a captured hipGraph.
+ * the launches of an iteration are replayed from one captured hipGraph.
  *
  * No counterpart in the reference, which solves one problem per call
  * (lobpcg_solve, krylov/lobpcg.c:502-1388).  Per problem the iteration is
@@ -41,8 +41,9 @@ typedef struct hip_BatchSettings {
 typedef struct hip_BatchStats {
     double seconds;     /* wall time of the solve (device-resident inputs excluded) */
     int iterations;     /* batch iterations (the slowest problem's count) */
-    int graph_captures; /* iterations captured (the active set changed) */
-    int graph_replays;  /* iterations replayed from a captured graph */
+    int graph_captures; /* graphs captured (one per solve: the active lists and counts
+                           live in device memory) */
+    int graph_replays;  /* iterations replayed from the captured graph */
     long total_rows;    /* sum of n */
 } hip_BatchStats;
 
