@@ -205,8 +205,8 @@ __global__ void __launch_bounds__(kT) k_bgram_sum(const int *act_prob, const int
     }
 }
 
-// ---- batched Rayleigh-Ritz, one workgroup (a single wave) per problem ----
-constexpr int kRT = 64;
+// ---- batched Rayleigh-Ritz, one workgroup per problem ----
+constexpr int kRT = 256;
 // Solves GA v = theta GB v on the leading m = nb kp rows / columns; writes
 // the kp smallest theta to lam and the GB-orthonormal v to C (pencil rows,
 // kp columns, leading dim kM), nbe[p] = blocks used.  A GB that is not
@@ -314,19 +314,46 @@ __global__ void __launch_bounds__(kRT) k_brr(const int *act_prob, const int *nac
             bot[tid] = 2 * tid + 1;
         }
         __syncthreads();
-        // sweeps until one makes no rotation: a pair rotates while
-        // |a_pq| > 1e-15 sqrt(|a_pp a_qq|) (the classical threshold; below it
-        // the rotation would change the eigenvalues by less than rounding)
+        // up to 30 sweeps, every nonzero a_pq rotated: stopping at the
+        // rounding level of the off-diagonal mass (or skipping rotations
+        // below 1e-15 sqrt(|a_pp a_qq|)) left Ritz vectors that slowed LOBPCG
+        // near tight tolerances several-fold on small pencils, so the sweeps
+        // run until the off-diagonal mass is below 1e-32 of the diagonal's
         __shared__ int rotated;
-        for (int sweep = 0; sweep < 40; sweep++) {
-            if (tid == 0) rotated = 0;
+        __shared__ double offs, dia, ro[kRT], rd[kRT];
+        for (int sweep = 0; sweep < 30; sweep++) {
+            {
+                double o = 0.0, d = 0.0;
+                for (int e = tid; e < m * m; e += kRT) {
+                    const int i = e % m, j = e / m;
+                    const double v = A[i][j];
+                    if (i != j)
+                        o = fma(v, v, o);
+                    else
+                        d = fma(v, v, d);
+                }
+                ro[tid] = o;
+                rd[tid] = d;
+            }
             __syncthreads();
+            if (tid == 0) {
+                double o = 0.0, d = 0.0;
+                for (int q = 0; q < kRT; q++) {
+                    o += ro[q];
+                    d += rd[q];
+                }
+                offs = o;
+                dia = d;
+                rotated = 0;
+            }
+            __syncthreads();
+            if (offs <= 1e-32 * dia) break;
             for (int step = 0; step < me - 1; step++) {
                 if (tid < h) {
                     const int a = min(top[tid], bot[tid]), b = max(top[tid], bot[tid]);
                     const double apq = A[a][b];
                     double c = 1.0, s = 0.0;
-                    if (fabs(apq) > 1e-15 * sqrt(fabs(A[a][a] * A[b][b])) && apq != 0.0) {
+                    if (apq != 0.0) {
                         const double tau = (A[b][b] - A[a][a]) / (2.0 * apq);
                         const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
                         c = 1.0 / sqrt(1.0 + t * t);
@@ -390,19 +417,23 @@ __global__ void __launch_bounds__(kRT) k_brr(const int *act_prob, const int *nac
             }
         }
         __syncthreads();
-        // C = S L^-T V(:, order[0..kp)): back substitution per output column
+        // C = S L^-T V(:, order[0..kp)): back substitution per output column,
+        // into A (its diagonal is saved in ev)
         double *cp = C + (size_t)p * kM * kK;
         for (int j = tid; j < kp; j += kRT) {
             const int src = order[j];
-            double y[kM];
             for (int i = m - 1; i >= 0; i--) {
                 double s = V[i][src];
-                for (int q = i + 1; q < m; q++) s -= B[q][i] * y[q];
-                y[i] = s / B[i][i];
+                for (int q = i + 1; q < m; q++) s -= B[q][i] * A[q][j];
+                A[i][j] = s / B[i][i];
             }
-            for (int i = 0; i < kM; i++) cp[i + j * kM] = i < m ? y[i] * sc[i] : 0.0;
-            lam[p * kK + j] = ev[src];
         }
+        __syncthreads();
+        for (int e = tid; e < kM * kp; e += kRT) {
+            const int i = e % kM, j = e / kM;
+            cp[i + j * kM] = i < m ? A[i][j] * sc[i] : 0.0;
+        }
+        for (int j = tid; j < kp; j += kRT) lam[p * kK + j] = ev[order[j]];
         if (tid == 0) {
             nbe[p] = nbt;
             status[p] = 0;
