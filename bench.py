@@ -133,6 +133,28 @@ def cpu_baseline(A, n_side, k, seed_x=1):
                            f"({(t3 - t1) + (t1 - t0):.1f} s of CPU work), scaled by n x{scale:.0f}"
 
 
+def run_batched(count, graph=True, maxit=2000):
+    """The batched workload: count independent 2D 5-point lognormal(0, 0.5)
+    diffusion problems (sides 32..256, k in {4, 8, 16}, Jacobi, FP64,
+    tolerance 1e-6) in one hip_LobpcgSolveBatched call."""
+    from blopex_amd import problems as pb
+    from blopex_amd.lobpcg import lobpcg_batched
+
+    problems = pb.batched_problems(count)
+    t = time.perf_counter()
+    r = lobpcg_batched(problems, tol=1e-6, maxit=maxit, graph=graph)
+    wall = time.perf_counter() - t
+    conv = int(np.sum(r.flags == 0))
+    pairs = sum(k for _, k in problems)
+    return {"workload": f"{count} 2D 5-point lognormal(0,0.5) problems, sides 32..256, "
+                        "k in {4,8,16}, Jacobi, FP64, tol 1e-6 (batched)",
+            "problems": count, "rows": int(r.stats["total_rows"]), "eigenpairs": pairs,
+            "converged": conv, "maxit": maxit, "seconds": r.stats["seconds"],
+            "wall_seconds": wall, "iterations_max": int(r.iterations.max()),
+            "iterations_mean": float(r.iterations.mean()),
+            "graph_replays": r.stats["graph_replays"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,6 +172,13 @@ def main():
                          "(the command the committed rocprofv3 summaries and counters use)")
     ap.add_argument("--profile-only", action="store_true",
                     help="one mixed solve and the SpMM launches only (for rocprofv3)")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched workload (512 independent 2D problems)")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only the batched workload (for rocprofv3)")
+    ap.add_argument("--batched-count", type=int, default=512)
+    ap.add_argument("--batched-no-graph", action="store_true",
+                    help="batched workload with eager launches instead of graph replays")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -162,6 +191,9 @@ def main():
 
     lib = L.load()
     lib.blopex_amd_init()
+    if args.batched_only:
+        print(json.dumps(run_batched(args.batched_count, not args.batched_no_graph)), flush=True)
+        return
     nside, k = args.n, args.k
     t = time.perf_counter()
     A = pb.laplacian_7pt(nside, nside, nside, sigma=1.0, seed=42)
@@ -342,9 +374,12 @@ def main():
             out["cpu_baseline"] = {"value": per_it * its, "unit": "s", "cores": 1,
                                    "kind": "reference", "seconds_per_iteration": per_it,
                                    "sample": sample + f"; x {its} iterations (GPU FP64 count)"}
-    print(json.dumps(out), flush=True)
     for o in (X, X0, dT, dA):
         o.close()
+    if not args.no_batched:
+        out["batched"] = run_batched(args.batched_count, not args.batched_no_graph)
+        log(f"batched: {out['batched']}")
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":
