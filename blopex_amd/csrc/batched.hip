@@ -9,7 +9,7 @@
 // every grouped kernel is launched over the chunks of the unconverged
 // problems, so one launch advances the whole batch:
 //   k_bresid     R = T (AX - X diag(lambda)) and per-chunk partial norms
-//   (CSR SpMM)   AR = A R over the stacked matrix (spmm_csr)
+//   k_bspmm      AR = A R over the active chunks of the stacked CSR matrix
 //   k_bgram      per-chunk partial Gram blocks [X R P]' [AX AR AP] and
 //                [X R P]' [X R P] (upper triangles, the problem's columns only)
 //   k_bgram_sum  per-problem fixed-order sums of the partials
@@ -27,6 +27,7 @@
 #include <cstring>
 #include <vector>
 
+#include "dense.h"
 #include "kernels.h"
 #include "lobpcg_batched.h"
 
@@ -454,14 +455,14 @@ __global__ void __launch_bounds__(kRT) k_brr(const int *act_prob, const int *nac
 // q >= kp; the same for the A-side blocks.  The output columns accumulate in
 // registers; C is read from LDS (the same address across the wave).
 __global__ void __launch_bounds__(kT) k_bupd(Chunks ch, Probs pr, long ld, const double *C,
-                                             const int *nbe, double *X, double *R, double *P,
-                                             double *AX, double *AR, double *AP) {
+                                             const int *nbe, int nb_all, double *X, double *R,
+                                             double *P, double *AX, double *AR, double *AP) {
     __shared__ double cs[kM * kK];
     const int c = blockIdx.x;
     if (c >= *ch.count) return;
     const int p = ch.prob[c];
     const int kp = pr.k[p];
-    const int nb = nbe[p];
+    const int nb = nbe ? nbe[p] : nb_all;
     const long r0 = ch.r0[c];
     const int nr = ch.nr[c];
     for (int e = threadIdx.x; e < kM * kK; e += kT) cs[e] = C[(size_t)p * kM * kK + e];
@@ -497,6 +498,34 @@ __global__ void __launch_bounds__(kT) k_bupd(Chunks ch, Probs pr, long ld, const
                     b0[(size_t)j * ld + r] = ax[j] + ap[j];
                 }
         }
+    }
+}
+
+// ---- AR = A R over the active chunks (CSR rows of the stacked matrix) ----
+__global__ void __launch_bounds__(kT) k_bspmm(Chunks ch, Probs pr, long ld, const int *rowptr,
+                                              const int *colind, const double *val,
+                                              const double *Rb, double *ARb) {
+    const int c = blockIdx.x;
+    if (c >= *ch.count) return;
+    const int p = ch.prob[c];
+    const int kp = pr.k[p];
+    const long r0 = ch.r0[c];
+    const int nr = ch.nr[c];
+    for (int t = threadIdx.x; t < nr; t += kT) {
+        const long r = r0 + t;
+        double acc[kK];
+#pragma unroll
+        for (int j = 0; j < kK; j++) acc[j] = 0.0;
+        for (int e = rowptr[r], e1 = rowptr[r + 1]; e < e1; e++) {
+            const double v = val[e];
+            const long col = colind[e];
+#pragma unroll
+            for (int j = 0; j < kK; j++)
+                if (j < kp) acc[j] = fma(v, Rb[(size_t)j * ld + col], acc[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < kK; j++)
+            if (j < kp) ARb[(size_t)j * ld + r] = acc[j];
     }
 }
 
@@ -646,6 +675,13 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
     double *GA = dev_alloc<double>((size_t)nprob * kM * kM), *GB = dev_alloc<double>((size_t)nprob * kM * kM);
     double *Cm = dev_alloc<double>((size_t)nprob * kM * kK);
     int *nbe = dev_alloc<int>(nprob), *status = dev_alloc<int>(nprob);
+    // the one-workgroup tridiagonal Rayleigh-Ritz solver of the single-problem
+    // path, launched over the batch (explicit X blocks, P dropped when the
+    // pencil is not positive definite, status per problem)
+    const size_t rr_words = rr_fast_work_words(kM);
+    double *rr_work = dev_alloc<double>((size_t)nprob * rr_words);
+    DevStatus *dstat = dev_alloc<DevStatus>(nprob);
+    BX_HIP(hipMemsetAsync(dstat, 0, sizeof(DevStatus) * nprob, stream()));
     double *part_r = dev_alloc<double>((size_t)nch * kK);
     double *part_g = dev_alloc<double>((size_t)nch * 2 * kPairs);
     BX_HIP(hipMemsetAsync(lam, 0, (size_t)nprob * kK * 8, stream()));
@@ -690,15 +726,23 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
     // grids cover every chunk / problem; workgroups past the device counts
     // exit at once, so one captured graph serves the whole solve
     auto step = [&](int nb, bool with_spmm) {
-        if (with_spmm) spmm_csr(HIPMV_FP64, A, R, N, all, AR, N, all);
+        if (with_spmm)
+            hipLaunchKernelGGL(k_bspmm, dim3(nch), dim3(kT), 0, stream(), ch, pr, N, A->rowptr,
+                               A->colind, A->val, R, AR);
         hipLaunchKernelGGL(k_bgram, dim3(nch), dim3(kT), 0, stream(), ch, pr, N, nb, X, R, P, AX,
                            AR, AP, part_g);
         hipLaunchKernelGGL(k_bgram_sum, dim3(nprob), dim3(kT), 0, stream(), d_actp, d_nact, d_pcb,
                            d_pcc, pr, nb, part_g, GA, GB);
-        hipLaunchKernelGGL(k_brr, dim3(nprob), dim3(kRT), 0, stream(), d_actp, d_nact, pr, nb, GA,
-                           GB, lam, Cm, nbe, status);
-        hipLaunchKernelGGL(k_bupd, dim3(nch), dim3(kT), 0, stream(), ch, pr, N, Cm, nbe, X, R, P,
-                           AX, AR, AP);
+        if (nb == 1) {
+            hipLaunchKernelGGL(k_brr, dim3(nprob), dim3(kRT), 0, stream(), d_actp, d_nact, pr, nb,
+                               GA, GB, lam, Cm, nbe, status);
+        } else {
+            RR2Args base{GA, GB, kM, 0, 0, 0, lam, lam, Cm, kM, rr_work, dstat, 1, 1};
+            rr_fast_batched(base, d_actp, d_nact, d_k, nprob, K, nb == 3, (long)kM * kM, kK,
+                            (long)kM * kK, (long)rr_words);
+        }
+        hipLaunchKernelGGL(k_bupd, dim3(nch), dim3(kT), 0, stream(), ch, pr, N, Cm,
+                           nb == 1 ? nbe : nullptr, nb, X, R, P, AX, AR, AP);
         hipLaunchKernelGGL(k_bresid, dim3(nch), dim3(kT), 0, stream(), ch, pr, N, X, AX, lam,
                            s.jacobi ? dv : nullptr, R, part_r);
         hipLaunchKernelGGL(k_bres_sum, dim3(nprob), dim3(64), 0, stream(), d_actp, d_nact, d_pcb,
@@ -714,6 +758,7 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
 
     std::vector<double> res_h((size_t)nprob * kK), lam_h((size_t)nprob * kK);
     std::vector<int> st_h(nprob);
+    std::vector<DevStatus> ds_h(nprob);
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int captures = 0, replays = 0, it = 0;
@@ -721,6 +766,8 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
     for (;;) {
         download_sync(res_h.data(), res2, res_h.size() * 8);
         download_sync(st_h.data(), status, nprob * 4);
+        download_sync(ds_h.data(), dstat, sizeof(DevStatus) * nprob);
+        for (int p = 0; p < nprob; p++) st_h[p] |= ds_h[p].abort;
         bool changed = false;
         for (int p = 0; p < nprob; p++) {
             if (!active[p]) continue;
@@ -785,7 +832,8 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
                     (void *)dv, (void *)d_off, (void *)d_sbase, (void *)d_n, (void *)d_k,
                     (void *)c_prob, (void *)c_nr, (void *)c_r0, (void *)d_actp, (void *)d_pcb,
                     (void *)d_pcc, (void *)lam, (void *)res2, (void *)GA, (void *)GB, (void *)Cm,
-                    (void *)nbe, (void *)status, (void *)part_r, (void *)part_g})
+                    (void *)nbe, (void *)status, (void *)part_r, (void *)part_g,
+                    (void *)rr_work, (void *)dstat})
         BX_HIP(hipFree(b));
     hip_CSRMatrixDestroy(A);
     BX_HIP(hipFree(d_counts));

@@ -251,7 +251,7 @@ __device__ __forceinline__ void householder(int n, int L, double *Am, double *co
     __syncthreads();
 }
 
-__global__ void __launch_bounds__(kT) k_rr_tri(RR2Args r) {
+__device__ void rr_tri_body(RR2Args r) {
     extern __shared__ double lds[];
     __shared__ double dT[kN], eT[kN], dS[kN], e2S[kN], tauT[kN], invu[kN], sdg[kN];
     __shared__ double pvec[kN], pv[kN], col0[kN], col1[kN];
@@ -1007,7 +1007,64 @@ __global__ void __launch_bounds__(kT) k_rr_tri(RR2Args r) {
     }
 }
 
+__global__ void __launch_bounds__(kT) k_rr_tri(RR2Args r) { rr_tri_body(r); }
+
+// one workgroup per problem of a batch: problem act[blockIdx.x] has blocks of
+// width kvec[p] (X and R, and P when with_p); its Gram buffers, Ritz values,
+// coefficients, work space and status sit at p times the given strides
+__global__ void __launch_bounds__(kT)
+    k_rr_tri_batched(RR2Args base, const int *act, const int *nact, const int *kvec, int with_p,
+                     long s_gram, long s_lam, long s_coef, long s_work) {
+    if ((int)blockIdx.x >= *nact) return;
+    const int p = act[blockIdx.x];
+    RR2Args r = base;
+    r.ga += (size_t)p * s_gram;
+    r.gb += (size_t)p * s_gram;
+    r.k = r.a = kvec[p];
+    r.p = with_p ? kvec[p] : 0;
+    r.lam_in += (size_t)p * s_lam;
+    r.lam_out += (size_t)p * s_lam;
+    r.coef += (size_t)p * s_coef;
+    r.work += (size_t)p * s_work;
+    r.st += p;
+    rr_tri_body(r);
+}
+
 }  // namespace
+
+size_t rr_fast_work_words(int n) {
+    const int nE = (n + 1) / 2 * 2;
+    return (size_t)4 * nE * nE + (size_t)2 * n * n + (size_t)n * n + 64;
+}
+
+void rr_fast_batched(const RR2Args &base, const int *act, const int *nact, const int *kvec,
+                     int nprob, int kmax, bool with_p, long s_gram, long s_lam, long s_coef,
+                     long s_work) {
+    const int n0 = (with_p ? 3 : 2) * kmax;
+    BX_REQUIRE(rr_fast_fits(kmax, kmax, with_p ? kmax : 0), "rr_fast_batched: k %d too large",
+               kmax);
+    constexpr size_t kMaxLds = ((size_t)kFastRRN + 3) * 2 * kFastRRN * sizeof(double);
+    const int L = n0 + ((3 - n0) & 31);
+    const int second = n0 > 2 * kmax ? n0 : 2 * kmax;
+    // the largest pencil of the batch sets the dynamic LDS of every workgroup
+    size_t lds = 0;
+    for (int n = 1; n <= n0; n++) {
+        const int Ln = n + ((3 - n) & 31);
+        const size_t b = (size_t)Ln * (n + (n > second ? n : second)) * sizeof(double);
+        if (b > lds) lds = b;
+    }
+    (void)L;
+    BX_REQUIRE(lds <= kMaxLds, "rr_fast_batched: %zu bytes of LDS", lds);
+    static bool attr_set = false;
+    if (!attr_set) {
+        BX_HIP(hipFuncSetAttribute((const void *)k_rr_tri_batched,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds));
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(k_rr_tri_batched, dim3(nprob), dim3(kT), lds, stream(), base, act, nact,
+                       kvec, (int)with_p, s_gram, s_lam, s_coef, s_work);
+    BX_HIP(hipGetLastError());
+}
 
 bool rr_fast_fits(int k, int a, int p) {
     return k + a + p <= kFastRRN && k <= kFastRRK && a <= kFastRRK && p <= kFastRRK && k >= 1;

@@ -3,7 +3,7 @@
 * A batch of small 2D lognormal(0, 0.5) diffusion problems of different sizes
   and block sizes: every problem converges, its eigenvalues are the smallest
   of its matrix (dense numpy eigh) to 1e-9 relative, its eigenvectors are
-  orthonormal (1e-12) with residuals (recomputed on the host) at the
+  orthonormal (1e-11) with residuals (recomputed on the host) at the
   tolerance; the run replayed from hipGraphs equals the eager run bit for bit.
 * The batched workload's generator at a reduced count (32 problems, sides 16
   to 48, k in {4, 8, 16}) against scipy's shift-invert eigsh per problem.
@@ -45,7 +45,9 @@ def _check(problems, r, tol, lam_rtol):
 
             ref = np.sort(sla.eigsh(S, k=k, sigma=0.0, which="LM", return_eigenvectors=False))
         assert np.max(np.abs(lam - ref) / ref) < lam_rtol, (i, lam, ref)
-        assert np.max(np.abs(X.T @ X - np.eye(k))) < 1e-12
+        # X = [X R P] C with C orthonormal in the Gram metric of the basis:
+        # orthonormal to the basis' conditioning times rounding
+        assert np.max(np.abs(X.T @ X - np.eye(k))) < 1e-11
         res = np.linalg.norm(S @ X - X * lam, axis=0)
         assert np.all(res <= tol * 1.01), (i, res)
         np.testing.assert_allclose(res, r.res[i], rtol=1e-6, atol=1e-12)
