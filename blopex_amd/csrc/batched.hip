@@ -13,9 +13,10 @@
 //   k_bgram      per-chunk partial Gram blocks [X R P]' [AX AR AP] and
 //                [X R P]' [X R P] (upper triangles, the problem's columns only)
 //   k_bgram_sum  per-problem fixed-order sums of the partials
-//   k_brr        one workgroup per problem: the Rayleigh-Ritz pencil in LDS,
-//                diagonal scaling, Cholesky, reduction to standard form,
-//                cyclic Jacobi, back-substitution (FP64)
+//   k_rr_tri_batched  one workgroup per problem: the single-problem path's
+//                tridiagonal Rayleigh-Ritz solver (rr_fast.hip) on the
+//                problem's pencil (FP64); the initial Rayleigh-Ritz in span(X)
+//                uses k_brr (diagonal scaling, Cholesky, cyclic Jacobi)
 //   k_bupd       X' = [X R P] C_X, P' = [R P] C_RP and the same for AX, AP
 // The host reads the residual norms once per iteration.  The launches of an
 // iteration are captured once into a hipGraph and replayed: their grids cover
