@@ -56,7 +56,7 @@ class BatchProblem(ctypes.Structure):
 
 class BatchSettings(ctypes.Structure):
     _fields_ = [("tol", c_double), ("maxit", c_int), ("jacobi", c_int), ("seed", c_int),
-                ("graph", c_int)]
+                ("graph", c_int), ("guard", c_int)]
 
 
 class BatchStats(ctypes.Structure):

@@ -35,11 +35,11 @@
 namespace bx {
 namespace {
 
-constexpr int kK = HIP_BATCH_MAX_K;   // largest block width
-constexpr int kM = 3 * kK;            // largest pencil
-constexpr int kT = 256;               // threads per workgroup
-constexpr int kChunk = 2048;          // rows per chunk
-constexpr int kSub = 64;              // rows staged per Gram step
+constexpr int kK = HIP_BATCH_MAX_K + 8;  // largest block width: k wanted plus guard columns
+constexpr int kM = 3 * kK;               // largest pencil
+constexpr int kT = 256;                  // threads per workgroup
+constexpr int kChunk = 2048;             // rows per chunk
+constexpr int kSub = 32;                 // rows staged per Gram step
 constexpr int kPairs = kM * (kM + 1) / 2;
 constexpr int kPairsPerThread = (kPairs + kT - 1) / kT;
 
@@ -218,7 +218,8 @@ __global__ void __launch_bounds__(kRT) k_brr(const int *act_prob, const int *nac
                                              const double *GA, const double *GB, double *lam,
                                              double *C, int *nbe, int *status) {
     if ((int)blockIdx.x >= *nact) return;
-    __shared__ double A[kM][kM + 1], B[kM][kM + 1], V[kM][kM + 1];
+    // the initial Rayleigh-Ritz only: pencils of one block (m = k <= kK)
+    __shared__ double A[kK][kK + 1], B[kK][kK + 1], V[kK][kK + 1];
     __shared__ double sc[kM], ev[kM], cs[kM / 2], sn[kM / 2];
     __shared__ int order[kM], top[kM / 2], bot[kM / 2];
     __shared__ int fail, mm;
@@ -578,6 +579,7 @@ void hip_BatchSettingsDefault(hip_BatchSettings *s) {
     s->jacobi = 1;
     s->seed = 1;
     s->graph = 1;
+    s->guard = 2;
 }
 
 int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_BatchSettings *set,
@@ -594,16 +596,15 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
     int K = 1;
     for (int p = 0; p < nprob; p++) {
         const hip_BatchProblem &q = probs[p];
-        if (q.k < 1 || q.k > kK || q.n < 3L * q.k || !q.rowptr || !q.colind || !q.val ||
+        if (q.k < 1 || q.k > HIP_BATCH_MAX_K || q.n < 3L * q.k || !q.rowptr || !q.colind || !q.val ||
             q.nnz != q.rowptr[q.n]) {
             set_error("hip_LobpcgSolveBatched: problem %d: need 1 <= k <= %d, n >= 3k and a "
                       "consistent CSR (n %ld, k %d, nnz %ld)",
-                      p, kK, q.n, q.k, q.nnz);
+                      p, HIP_BATCH_MAX_K, q.n, q.k, q.nnz);
             return -1;
         }
         N += q.n;
         nnz += q.nnz;
-        K = std::max(K, q.k);
     }
     if (N > 0x7fffffffL || nnz > 0x7fffffffL) {
         set_error("hip_LobpcgSolveBatched: %ld rows / %ld entries exceed int32 indices", N, nnz);
@@ -613,7 +614,8 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
     std::vector<int> rowptr(N + 1), colind(nnz);
     std::vector<double> val(nnz), dinv(N);
     std::vector<long> off(nprob), sbase(nprob);
-    std::vector<int> pn(nprob), pk(nprob);
+    // pk: columns iterated (k plus guard columns), kw: columns wanted
+    std::vector<int> pn(nprob), pk(nprob), kw(nprob);
     std::vector<int> chunk_prob;
     std::vector<long> chunk_r0;
     std::vector<int> chunk_nr, chunk_first(nprob), chunk_cnt(nprob);
@@ -622,9 +624,11 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
         const hip_BatchProblem &q = probs[p];
         off[p] = r;
         sbase[p] = sb;
-        sb += q.n * q.k;
         pn[p] = (int)q.n;
-        pk[p] = q.k;
+        kw[p] = q.k;
+        pk[p] = (int)std::min<long>(std::min(q.k + std::max(s.guard, 0), kK), q.n / 3);
+        K = std::max(K, pk[p]);
+        sb += q.n * pk[p];
         for (long i = 0; i < q.n; i++) {
             rowptr[r + i] = (int)(e + q.rowptr[i]);
             double d = 0.0;
@@ -773,7 +777,7 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
         for (int p = 0; p < nprob; p++) {
             if (!active[p]) continue;
             bool conv = true;
-            for (int j = 0; j < pk[p]; j++) conv = conv && sqrt(fabs(res_h[p * kK + j])) <= s.tol;
+            for (int j = 0; j < kw[p]; j++) conv = conv && sqrt(fabs(res_h[p * kK + j])) <= s.tol;
             if (conv || st_h[p]) {
                 active[p] = 0;
                 changed = true;
@@ -812,18 +816,18 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
     size_t o = 0, xo = 0;
     std::vector<double> col;
     for (int p = 0; p < nprob; p++) {
-        for (int j = 0; j < pk[p]; j++) {
+        for (int j = 0; j < kw[p]; j++) {
             lambda[o + j] = lam_h[p * kK + j];
             resnorm[o + j] = sqrt(fabs(res_h[p * kK + j]));
         }
-        o += pk[p];
+        o += kw[p];
         iterations[p] = iters_h[p];
         flags[p] = flag_h[p];
         if (Xout) {
-            for (int j = 0; j < pk[p]; j++)
+            for (int j = 0; j < kw[p]; j++)
                 download_sync(Xout + xo + (size_t)j * pn[p], X + (size_t)j * N + off[p],
                               (size_t)pn[p] * 8);
-            xo += (size_t)pn[p] * pk[p];
+            xo += (size_t)pn[p] * kw[p];
         }
     }
     BX_HIP(hipStreamSynchronize(stream()));

@@ -326,7 +326,8 @@ class BatchResult:
 
 
 def lobpcg_batched(problems, tol: float = 1e-6, maxit: int = 500, jacobi: bool = True,
-                   seed: int = 1, graph: bool = True, want_X: bool = False) -> BatchResult:
+                   seed: int = 1, graph: bool = True, want_X: bool = False,
+                   guard: int = 2) -> BatchResult:
     """Smallest eigenpairs of many independent problems in one device solve
     (hip_LobpcgSolveBatched).  problems: list of (CSR, k)."""
     lib = L.load()
@@ -339,6 +340,7 @@ def lobpcg_batched(problems, tol: float = 1e-6, maxit: int = 500, jacobi: bool =
     s = L.BatchSettings()
     lib.hip_BatchSettingsDefault(ctypes.byref(s))
     s.tol, s.maxit, s.jacobi, s.seed, s.graph = tol, maxit, int(jacobi), seed, int(graph)
+    s.guard = guard
     ks = [k for _, k in problems]
     lam = np.zeros(sum(ks))
     res = np.zeros(sum(ks))

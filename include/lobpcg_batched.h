@@ -36,6 +36,9 @@ typedef struct hip_BatchSettings {
     int seed;           /* X0: drand48 after srand48(seed), problem by problem, column by
                            column (default 1) */
     int graph;          /* 1: replay each iteration's launches from a hipGraph (default 1) */
+    int guard;          /* extra columns iterated beside the k wanted, up to
+                           HIP_BATCH_MAX_K + 8 columns in all and n/3; only the k wanted
+                           are tested for convergence and returned (default 2) */
 } hip_BatchSettings;
 
 typedef struct hip_BatchStats {

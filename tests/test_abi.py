@@ -64,6 +64,11 @@ def test_struct_layouts():
     # hip_AdjointSettings {int, double, int}, hip_AdjointStats {int, int, double, double}
     assert ctypes.sizeof(L.AdjointSettings) == 24 and L.AdjointSettings.maxit.offset == 16
     assert ctypes.sizeof(L.AdjointStats) == 24 and L.AdjointStats.seconds.offset == 16
+    # hip_BatchProblem {long, long, 3 pointers, int}, hip_BatchSettings {double, 5 ints},
+    # hip_BatchStats {double, 3 ints, long}
+    assert ctypes.sizeof(L.BatchProblem) == 48 and L.BatchProblem.k.offset == 40
+    assert ctypes.sizeof(L.BatchSettings) == 32 and L.BatchSettings.guard.offset == 24
+    assert ctypes.sizeof(L.BatchStats) == 32 and L.BatchStats.total_rows.offset == 24
 
 
 def test_adjoint_defaults_and_argument_checks(lib):
