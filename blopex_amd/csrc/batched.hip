@@ -72,17 +72,15 @@ __global__ void __launch_bounds__(kT) k_bresid(Chunks ch, Probs pr, long ld, con
     const int kp = pr.k[p];
     const long r0 = ch.r0[c];
     const int nr = ch.nr[c];
-    for (int j = 0; j < kK; j++) {
+    // the problem's columns only: R's other columns are never written and stay zero
+    for (int j = 0; j < kp; j++) {
         double acc = 0.0;
-        const double l = j < kp ? lam[p * kK + j] : 0.0;
+        const double l = lam[p * kK + j];
         for (int t = threadIdx.x; t < nr; t += kT) {
             const long r = r0 + t;
-            double v = 0.0;
-            if (j < kp) {
-                v = blk_get(AX, ld, r, j) - l * blk_get(X, ld, r, j);
-                acc = fma(v, v, acc);
-                if (dinv) v *= dinv[r];
-            }
+            double v = blk_get(AX, ld, r, j) - l * blk_get(X, ld, r, j);
+            acc = fma(v, v, acc);
+            if (dinv) v *= dinv[r];
             R[(size_t)j * ld + r] = v;
         }
         red[threadIdx.x] = acc;
@@ -110,42 +108,50 @@ __global__ void k_bres_sum(const int *act_prob, const int *nact, const int *pc_b
 
 // ---- grouped Gram blocks ----
 // pencil index q = b * kp + col (block b of {X, R, P}); partials of the upper
-// triangles of GA = L' A L and GB = L' L over the chunk's rows
+// triangles of GA = L' A L and GB = L' L over the chunk's rows.  Each thread
+// owns 2 x 2 blocks of the (pair-padded) upper triangle: six LDS reads feed
+// eight FMAs.
+constexpr int kHM = kM / 2;                               // block rows / columns
+constexpr int kBlk = kHM * (kHM + 1) / 2;                 // upper 2 x 2 blocks
+constexpr int kBlkPerThread = (kBlk + kT - 1) / kT;
 __global__ void __launch_bounds__(kT) k_bgram(Chunks ch, Probs pr, long ld, int nb, const double *X,
                                               const double *R, const double *P, const double *AX,
                                               const double *AR, const double *AP, double *part) {
-    __shared__ double Ls[kSub][kM + 1];
-    __shared__ double As[kSub][kM + 1];
+    __shared__ double Ls[kSub][kM + 2];
+    __shared__ double As[kSub][kM + 2];
     const int c = blockIdx.x;
     if (c >= *ch.count) return;
     const int p = ch.prob[c];
     const int kp = pr.k[p];
     const int m = nb * kp;
-    const int npairs = m * (m + 1) / 2;
+    const int hm = (m + 1) / 2;
+    const int nblk = hm * (hm + 1) / 2;
     const long r0 = ch.r0[c];
     const int nr = ch.nr[c];
     const double *Lb[3] = {X, R, P};
     const double *Ab[3] = {AX, AR, AP};
-    int pi[kPairsPerThread], pj[kPairsPerThread];
-    double ga[kPairsPerThread], gb[kPairsPerThread];
+    int bi[kBlkPerThread], bj[kBlkPerThread];
+    double ga[kBlkPerThread][4], gb[kBlkPerThread][4];
 #pragma unroll
-    for (int s = 0; s < kPairsPerThread; s++) {
+    for (int s = 0; s < kBlkPerThread; s++) {
         const int o = threadIdx.x + s * kT;
         int j = (int)((sqrt(8.0 * o + 1.0) - 1.0) * 0.5);
         while ((j + 1) * (j + 2) / 2 <= o) j++;
         while (j * (j + 1) / 2 > o) j--;
-        pi[s] = o - j * (j + 1) / 2;
-        pj[s] = j;
-        ga[s] = gb[s] = 0.0;
+        bi[s] = 2 * (o - j * (j + 1) / 2);
+        bj[s] = 2 * j;
+#pragma unroll
+        for (int q = 0; q < 4; q++) ga[s][q] = gb[s][q] = 0.0;
     }
     for (int s0 = 0; s0 < nr; s0 += kSub) {
         const int rows = min(kSub, nr - s0);
         __syncthreads();
-        for (int e = threadIdx.x; e < kSub * m; e += kT) {
+        // staged columns 0 .. 2 hm (an odd m gets a zero column)
+        for (int e = threadIdx.x; e < kSub * 2 * hm; e += kT) {
             const int rr = e % kSub, q = e / kSub;
-            const int b = q / kp, col = q % kp;
             double lv = 0.0, av = 0.0;
-            if (rr < rows) {
+            if (rr < rows && q < m) {
+                const int b = q / kp, col = q % kp;
                 const long r = r0 + s0 + rr;
                 lv = blk_get(Lb[b], ld, r, col);
                 av = blk_get(Ab[b], ld, r, col);
@@ -155,27 +161,45 @@ __global__ void __launch_bounds__(kT) k_bgram(Chunks ch, Probs pr, long ld, int 
         }
         __syncthreads();
 #pragma unroll
-        for (int s = 0; s < kPairsPerThread; s++) {
+        for (int s = 0; s < kBlkPerThread; s++) {
             const int o = threadIdx.x + s * kT;
-            if (o < npairs) {
-                const int i = pi[s], j = pj[s];
-                double a = ga[s], g = gb[s];
+            if (o < nblk) {
+                const int i = bi[s], j = bj[s];
+                double a0 = ga[s][0], a1 = ga[s][1], a2 = ga[s][2], a3 = ga[s][3];
+                double g0 = gb[s][0], g1 = gb[s][1], g2 = gb[s][2], g3 = gb[s][3];
                 for (int rr = 0; rr < rows; rr++) {
-                    const double li = Ls[rr][i];
-                    a = fma(li, As[rr][j], a);
-                    g = fma(li, Ls[rr][j], g);
+                    const double l0 = Ls[rr][i], l1 = Ls[rr][i + 1];
+                    const double x0 = As[rr][j], x1 = As[rr][j + 1];
+                    const double y0 = Ls[rr][j], y1 = Ls[rr][j + 1];
+                    a0 = fma(l0, x0, a0);
+                    a1 = fma(l1, x0, a1);
+                    a2 = fma(l0, x1, a2);
+                    a3 = fma(l1, x1, a3);
+                    g0 = fma(l0, y0, g0);
+                    g1 = fma(l1, y0, g1);
+                    g2 = fma(l0, y1, g2);
+                    g3 = fma(l1, y1, g3);
                 }
-                ga[s] = a;
-                gb[s] = g;
+                ga[s][0] = a0; ga[s][1] = a1; ga[s][2] = a2; ga[s][3] = a3;
+                gb[s][0] = g0; gb[s][1] = g1; gb[s][2] = g2; gb[s][3] = g3;
             }
         }
     }
+    // entries (i + di, j + dj) with i + di <= j + dj < m, at pair index
+    // (j + dj)(j + dj + 1)/2 + i + di
 #pragma unroll
-    for (int s = 0; s < kPairsPerThread; s++) {
+    for (int s = 0; s < kBlkPerThread; s++) {
         const int o = threadIdx.x + s * kT;
-        if (o < npairs) {
-            part[((size_t)c * 2 + 0) * kPairs + o] = ga[s];
-            part[((size_t)c * 2 + 1) * kPairs + o] = gb[s];
+        if (o < nblk) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int ii = bi[s] + (q & 1), jj = bj[s] + (q >> 1);
+                if (ii <= jj && jj < m) {
+                    const int pair = jj * (jj + 1) / 2 + ii;
+                    part[((size_t)c * 2 + 0) * kPairs + pair] = ga[s][q];
+                    part[((size_t)c * 2 + 1) * kPairs + pair] = gb[s][q];
+                }
+            }
         }
     }
 }
@@ -454,8 +478,61 @@ __global__ void __launch_bounds__(kRT) k_brr(const int *act_prob, const int *nac
 
 // ---- block update, row by row ----
 // X' = sum_q C[q, j] L_q (q over the nbe blocks' columns), P' = the part with
-// q >= kp; the same for the A-side blocks.  The output columns accumulate in
-// registers; C is read from LDS (the same address across the wave).
+// q >= kp; the same for the A-side blocks.  Each thread updates two rows
+// (kT apart) so that every coefficient read from LDS (the same address across
+// the wave) feeds two FMAs; the output columns accumulate in registers (KB of
+// them: the problem's column count rounded up to 8, 16 or 24, chosen per
+// workgroup).
+template <int KB>
+__device__ __forceinline__ void bupd_rows(const double *cs, int kp, int nb, long ld, long r0,
+                                          int nr, double *b0, const double *b1, double *b2) {
+    const double *bl[3] = {b0, b1, b2};
+    for (int t0 = 0; t0 < nr; t0 += 2 * kT) {
+        const int ta = t0 + threadIdx.x, tb = ta + kT;
+        const bool va = ta < nr, vb = tb < nr;
+        const long ra = r0 + (va ? ta : 0), rb = r0 + (vb ? tb : 0);
+        double xa[KB], pa[KB], xb[KB], pb[KB];
+#pragma unroll
+        for (int j = 0; j < KB; j++) xa[j] = pa[j] = xb[j] = pb[j] = 0.0;
+        for (int b = 0; b < nb; b++) {
+            const double *src = bl[b];
+            for (int col = 0; col < kp; col++) {
+                const int q = b * kp + col;
+                const double ua = src[(size_t)col * ld + ra], ub = src[(size_t)col * ld + rb];
+                if (b == 0) {
+#pragma unroll
+                    for (int j = 0; j < KB; j++)
+                        if (j < kp) {
+                            const double cq = cs[q + j * kM];
+                            xa[j] = fma(cq, ua, xa[j]);
+                            xb[j] = fma(cq, ub, xb[j]);
+                        }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < KB; j++)
+                        if (j < kp) {
+                            const double cq = cs[q + j * kM];
+                            pa[j] = fma(cq, ua, pa[j]);
+                            pb[j] = fma(cq, ub, pb[j]);
+                        }
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < KB; j++)
+            if (j < kp) {
+                if (va) {
+                    if (nb > 1) b2[(size_t)j * ld + ra] = pa[j];
+                    b0[(size_t)j * ld + ra] = xa[j] + pa[j];
+                }
+                if (vb) {
+                    if (nb > 1) b2[(size_t)j * ld + rb] = pb[j];
+                    b0[(size_t)j * ld + rb] = xb[j] + pb[j];
+                }
+            }
+    }
+}
+
 __global__ void __launch_bounds__(kT) k_bupd(Chunks ch, Probs pr, long ld, const double *C,
                                              const int *nbe, int nb_all, double *X, double *R,
                                              double *P, double *AX, double *AR, double *AP) {
@@ -471,39 +548,38 @@ __global__ void __launch_bounds__(kT) k_bupd(Chunks ch, Probs pr, long ld, const
     __syncthreads();
     for (int side = 0; side < 2; side++) {
         double *b0 = side ? AX : X, *b1 = side ? AR : R, *b2 = side ? AP : P;
-        const double *bl[3] = {b0, b1, b2};
-        for (int t = threadIdx.x; t < nr; t += kT) {
-            const long r = r0 + t;
-            double ax[kK], ap[kK];
-#pragma unroll
-            for (int j = 0; j < kK; j++) ax[j] = ap[j] = 0.0;
-            for (int b = 0; b < nb; b++) {
-                const double *src = bl[b];
-                for (int col = 0; col < kp; col++) {
-                    const int q = b * kp + col;
-                    const double v = src[(size_t)col * ld + r];
-                    if (b == 0) {
-#pragma unroll
-                        for (int j = 0; j < kK; j++)
-                            if (j < kp) ax[j] = fma(cs[q + j * kM], v, ax[j]);
-                    } else {
-#pragma unroll
-                        for (int j = 0; j < kK; j++)
-                            if (j < kp) ap[j] = fma(cs[q + j * kM], v, ap[j]);
-                    }
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < kK; j++)
-                if (j < kp) {
-                    if (nb > 1) b2[(size_t)j * ld + r] = ap[j];
-                    b0[(size_t)j * ld + r] = ax[j] + ap[j];
-                }
-        }
+        if (kp <= 8)
+            bupd_rows<8>(cs, kp, nb, ld, r0, nr, b0, b1, b2);
+        else if (kp <= 16)
+            bupd_rows<16>(cs, kp, nb, ld, r0, nr, b0, b1, b2);
+        else
+            bupd_rows<kK>(cs, kp, nb, ld, r0, nr, b0, b1, b2);
     }
 }
 
 // ---- AR = A R over the active chunks (CSR rows of the stacked matrix) ----
+template <int KB>
+__device__ __forceinline__ void bspmm_rows(int kp, long ld, long r0, int nr, const int *rowptr,
+                                           const int *colind, const double *val,
+                                           const double *Rb, double *ARb) {
+    for (int t = threadIdx.x; t < nr; t += kT) {
+        const long r = r0 + t;
+        double acc[KB];
+#pragma unroll
+        for (int j = 0; j < KB; j++) acc[j] = 0.0;
+        for (int e = rowptr[r], e1 = rowptr[r + 1]; e < e1; e++) {
+            const double v = val[e];
+            const long col = colind[e];
+#pragma unroll
+            for (int j = 0; j < KB; j++)
+                if (j < kp) acc[j] = fma(v, Rb[(size_t)j * ld + col], acc[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < KB; j++)
+            if (j < kp) ARb[(size_t)j * ld + r] = acc[j];
+    }
+}
+
 __global__ void __launch_bounds__(kT) k_bspmm(Chunks ch, Probs pr, long ld, const int *rowptr,
                                               const int *colind, const double *val,
                                               const double *Rb, double *ARb) {
@@ -513,22 +589,12 @@ __global__ void __launch_bounds__(kT) k_bspmm(Chunks ch, Probs pr, long ld, cons
     const int kp = pr.k[p];
     const long r0 = ch.r0[c];
     const int nr = ch.nr[c];
-    for (int t = threadIdx.x; t < nr; t += kT) {
-        const long r = r0 + t;
-        double acc[kK];
-#pragma unroll
-        for (int j = 0; j < kK; j++) acc[j] = 0.0;
-        for (int e = rowptr[r], e1 = rowptr[r + 1]; e < e1; e++) {
-            const double v = val[e];
-            const long col = colind[e];
-#pragma unroll
-            for (int j = 0; j < kK; j++)
-                if (j < kp) acc[j] = fma(v, Rb[(size_t)j * ld + col], acc[j]);
-        }
-#pragma unroll
-        for (int j = 0; j < kK; j++)
-            if (j < kp) ARb[(size_t)j * ld + r] = acc[j];
-    }
+    if (kp <= 8)
+        bspmm_rows<8>(kp, ld, r0, nr, rowptr, colind, val, Rb, ARb);
+    else if (kp <= 16)
+        bspmm_rows<16>(kp, ld, r0, nr, rowptr, colind, val, Rb, ARb);
+    else
+        bspmm_rows<kK>(kp, ld, r0, nr, rowptr, colind, val, Rb, ARb);
 }
 
 // ---- X0: drand48 stream, problem by problem, column by column ----
