@@ -26,6 +26,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "dense.h"
@@ -57,14 +58,16 @@ struct Probs {
     const int *k;
 };
 
-__device__ __forceinline__ double blk_get(const double *b, long ld, long row, int col) {
-    return b[(size_t)col * ld + row];
+template <typename T>
+__device__ __forceinline__ double blk_get(const T *b, long ld, long row, int col) {
+    return (double)b[(size_t)col * ld + row];
 }
 
 // ---- residual, Jacobi, per-chunk squared norms ----
-__global__ void __launch_bounds__(kT) k_bresid(Chunks ch, Probs pr, long ld, const double *X,
-                                               const double *AX, const double *lam,
-                                               const double *dinv, double *R, double *part) {
+template <typename T>
+__global__ void __launch_bounds__(kT) k_bresid(Chunks ch, Probs pr, long ld, const T *X,
+                                               const T *AX, const double *lam,
+                                               const double *dinv, T *R, double *part) {
     __shared__ double red[kT];
     const int c = blockIdx.x;
     if (c >= *ch.count) return;
@@ -81,7 +84,7 @@ __global__ void __launch_bounds__(kT) k_bresid(Chunks ch, Probs pr, long ld, con
             double v = blk_get(AX, ld, r, j) - l * blk_get(X, ld, r, j);
             acc = fma(v, v, acc);
             if (dinv) v *= dinv[r];
-            R[(size_t)j * ld + r] = v;
+            R[(size_t)j * ld + r] = (T)v;
         }
         red[threadIdx.x] = acc;
         __syncthreads();
@@ -114,9 +117,10 @@ __global__ void k_bres_sum(const int *act_prob, const int *nact, const int *pc_b
 constexpr int kHM = kM / 2;                               // block rows / columns
 constexpr int kBlk = kHM * (kHM + 1) / 2;                 // upper 2 x 2 blocks
 constexpr int kBlkPerThread = (kBlk + kT - 1) / kT;
-__global__ void __launch_bounds__(kT) k_bgram(Chunks ch, Probs pr, long ld, int nb, const double *X,
-                                              const double *R, const double *P, const double *AX,
-                                              const double *AR, const double *AP, double *part) {
+template <typename T>
+__global__ void __launch_bounds__(kT) k_bgram(Chunks ch, Probs pr, long ld, int nb, const T *X,
+                                              const T *R, const T *P, const T *AX, const T *AR,
+                                              const T *AP, double *part) {
     __shared__ double Ls[kSub][kM + 2];
     __shared__ double As[kSub][kM + 2];
     const int c = blockIdx.x;
@@ -128,8 +132,8 @@ __global__ void __launch_bounds__(kT) k_bgram(Chunks ch, Probs pr, long ld, int 
     const int nblk = hm * (hm + 1) / 2;
     const long r0 = ch.r0[c];
     const int nr = ch.nr[c];
-    const double *Lb[3] = {X, R, P};
-    const double *Ab[3] = {AX, AR, AP};
+    const T *Lb[3] = {X, R, P};
+    const T *Ab[3] = {AX, AR, AP};
     int bi[kBlkPerThread], bj[kBlkPerThread];
     double ga[kBlkPerThread][4], gb[kBlkPerThread][4];
 #pragma unroll
@@ -483,10 +487,10 @@ __global__ void __launch_bounds__(kRT) k_brr(const int *act_prob, const int *nac
 // the wave) feeds two FMAs; the output columns accumulate in registers (KB of
 // them: the problem's column count rounded up to 8, 16 or 24, chosen per
 // workgroup).
-template <int KB>
+template <int KB, typename T>
 __device__ __forceinline__ void bupd_rows(const double *cs, int kp, int nb, long ld, long r0,
-                                          int nr, double *b0, const double *b1, double *b2) {
-    const double *bl[3] = {b0, b1, b2};
+                                          int nr, T *b0, const T *b1, T *b2) {
+    const T *bl[3] = {b0, b1, b2};
     for (int t0 = 0; t0 < nr; t0 += 2 * kT) {
         const int ta = t0 + threadIdx.x, tb = ta + kT;
         const bool va = ta < nr, vb = tb < nr;
@@ -495,10 +499,11 @@ __device__ __forceinline__ void bupd_rows(const double *cs, int kp, int nb, long
 #pragma unroll
         for (int j = 0; j < KB; j++) xa[j] = pa[j] = xb[j] = pb[j] = 0.0;
         for (int b = 0; b < nb; b++) {
-            const double *src = bl[b];
+            const T *src = bl[b];
             for (int col = 0; col < kp; col++) {
                 const int q = b * kp + col;
-                const double ua = src[(size_t)col * ld + ra], ub = src[(size_t)col * ld + rb];
+                const double ua = (double)src[(size_t)col * ld + ra];
+                const double ub = (double)src[(size_t)col * ld + rb];
                 if (b == 0) {
 #pragma unroll
                     for (int j = 0; j < KB; j++)
@@ -522,20 +527,21 @@ __device__ __forceinline__ void bupd_rows(const double *cs, int kp, int nb, long
         for (int j = 0; j < KB; j++)
             if (j < kp) {
                 if (va) {
-                    if (nb > 1) b2[(size_t)j * ld + ra] = pa[j];
-                    b0[(size_t)j * ld + ra] = xa[j] + pa[j];
+                    if (nb > 1) b2[(size_t)j * ld + ra] = (T)pa[j];
+                    b0[(size_t)j * ld + ra] = (T)(xa[j] + pa[j]);
                 }
                 if (vb) {
-                    if (nb > 1) b2[(size_t)j * ld + rb] = pb[j];
-                    b0[(size_t)j * ld + rb] = xb[j] + pb[j];
+                    if (nb > 1) b2[(size_t)j * ld + rb] = (T)pb[j];
+                    b0[(size_t)j * ld + rb] = (T)(xb[j] + pb[j]);
                 }
             }
     }
 }
 
+template <typename T>
 __global__ void __launch_bounds__(kT) k_bupd(Chunks ch, Probs pr, long ld, const double *C,
-                                             const int *nbe, int nb_all, double *X, double *R,
-                                             double *P, double *AX, double *AR, double *AP) {
+                                             const int *nbe, int nb_all, T *X, T *R, T *P, T *AX,
+                                             T *AR, T *AP) {
     __shared__ double cs[kM * kK];
     const int c = blockIdx.x;
     if (c >= *ch.count) return;
@@ -547,28 +553,27 @@ __global__ void __launch_bounds__(kT) k_bupd(Chunks ch, Probs pr, long ld, const
     for (int e = threadIdx.x; e < kM * kK; e += kT) cs[e] = C[(size_t)p * kM * kK + e];
     __syncthreads();
     for (int side = 0; side < 2; side++) {
-        double *b0 = side ? AX : X, *b1 = side ? AR : R, *b2 = side ? AP : P;
+        T *b0 = side ? AX : X, *b1 = side ? AR : R, *b2 = side ? AP : P;
         if (kp <= 8)
-            bupd_rows<8>(cs, kp, nb, ld, r0, nr, b0, b1, b2);
+            bupd_rows<8, T>(cs, kp, nb, ld, r0, nr, b0, b1, b2);
         else if (kp <= 16)
-            bupd_rows<16>(cs, kp, nb, ld, r0, nr, b0, b1, b2);
+            bupd_rows<16, T>(cs, kp, nb, ld, r0, nr, b0, b1, b2);
         else
-            bupd_rows<kK>(cs, kp, nb, ld, r0, nr, b0, b1, b2);
+            bupd_rows<kK, T>(cs, kp, nb, ld, r0, nr, b0, b1, b2);
     }
 }
 
 // ---- AR = A R over the active chunks (CSR rows of the stacked matrix) ----
-template <int KB>
+template <int KB, typename T>
 __device__ __forceinline__ void bspmm_rows(int kp, long ld, long r0, int nr, const int *rowptr,
-                                           const int *colind, const double *val,
-                                           const double *Rb, double *ARb) {
+                                           const int *colind, const T *val, const T *Rb, T *ARb) {
     for (int t = threadIdx.x; t < nr; t += kT) {
         const long r = r0 + t;
-        double acc[KB];
+        T acc[KB];
 #pragma unroll
-        for (int j = 0; j < KB; j++) acc[j] = 0.0;
+        for (int j = 0; j < KB; j++) acc[j] = T(0);
         for (int e = rowptr[r], e1 = rowptr[r + 1]; e < e1; e++) {
-            const double v = val[e];
+            const T v = val[e];
             const long col = colind[e];
 #pragma unroll
             for (int j = 0; j < KB; j++)
@@ -580,9 +585,10 @@ __device__ __forceinline__ void bspmm_rows(int kp, long ld, long r0, int nr, con
     }
 }
 
+template <typename T>
 __global__ void __launch_bounds__(kT) k_bspmm(Chunks ch, Probs pr, long ld, const int *rowptr,
-                                              const int *colind, const double *val,
-                                              const double *Rb, double *ARb) {
+                                              const int *colind, const T *val, const T *Rb,
+                                              T *ARb) {
     const int c = blockIdx.x;
     if (c >= *ch.count) return;
     const int p = ch.prob[c];
@@ -590,11 +596,11 @@ __global__ void __launch_bounds__(kT) k_bspmm(Chunks ch, Probs pr, long ld, cons
     const long r0 = ch.r0[c];
     const int nr = ch.nr[c];
     if (kp <= 8)
-        bspmm_rows<8>(kp, ld, r0, nr, rowptr, colind, val, Rb, ARb);
+        bspmm_rows<8, T>(kp, ld, r0, nr, rowptr, colind, val, Rb, ARb);
     else if (kp <= 16)
-        bspmm_rows<16>(kp, ld, r0, nr, rowptr, colind, val, Rb, ARb);
+        bspmm_rows<16, T>(kp, ld, r0, nr, rowptr, colind, val, Rb, ARb);
     else
-        bspmm_rows<kK>(kp, ld, r0, nr, rowptr, colind, val, Rb, ARb);
+        bspmm_rows<kK, T>(kp, ld, r0, nr, rowptr, colind, val, Rb, ARb);
 }
 
 // ---- X0: drand48 stream, problem by problem, column by column ----
@@ -609,8 +615,9 @@ __device__ uint64_t jump48(uint64_t x, uint64_t steps) {
     }
     return x;
 }
+template <typename T>
 __global__ void __launch_bounds__(kT) k_bdrand(Chunks ch, Probs pr, long ld, const long *sbase,
-                                               uint64_t x0, double *X) {
+                                               uint64_t x0, T *X) {
     const int c = blockIdx.x;
     const int p = ch.prob[c];
     const int kp = pr.k[p], np = pr.n[p];
@@ -620,9 +627,15 @@ __global__ void __launch_bounds__(kT) k_bdrand(Chunks ch, Probs pr, long ld, con
             const long r = r0 + t;
             const uint64_t pos = (uint64_t)sbase[p] + (uint64_t)j * np + (uint64_t)(r - off);
             const uint64_t s = jump48(x0, pos + 1);
-            X[(size_t)j * ld + r] = 2.0 * ((double)s * (1.0 / 281474976710656.0)) - 1.0;
+            X[(size_t)j * ld + r] = (T)(2.0 * ((double)s * (1.0 / 281474976710656.0)) - 1.0);
         }
 }
+
+template <typename T>
+struct Blocks {
+    using type = T;
+    T *X, *AX, *R, *AR, *P, *AP;
+};
 
 template <typename T>
 T *dev_alloc(size_t count) {
@@ -646,6 +659,8 @@ void hip_BatchSettingsDefault(hip_BatchSettings *s) {
     s->seed = 1;
     s->graph = 1;
     s->guard = 2;
+    s->mode = HIPLOBPCG_FP64;
+    s->switch_factor = 100.0;
 }
 
 int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_BatchSettings *set,
@@ -794,14 +809,26 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
     };
     rebuild();
     const ColMap all = colmap_identity(kK);
+    const bool mixed = s.mode == HIPLOBPCG_MIXED;
+    Blocks<double> b64{X, AX, R, AR, P, AP};
+    Blocks<float> b32{};
+    if (mixed) {
+        float **f[6] = {&b32.X, &b32.AX, &b32.R, &b32.AR, &b32.P, &b32.AP};
+        for (float **q : f) {
+            *q = dev_alloc<float>(nk);
+            BX_HIP(hipMemsetAsync(*q, 0, nk * 4, stream()));
+        }
+    }
     // grids cover every chunk / problem; workgroups past the device counts
-    // exit at once, so one captured graph serves the whole solve
-    auto step = [&](int nb, bool with_spmm) {
+    // exit at once, so one captured graph serves a whole phase
+    auto step = [&](auto &bk, int nb, bool with_spmm) {
+        using T = typename std::remove_reference<decltype(bk)>::type::type;
+        const T *vals = std::is_same<T, float>::value ? (const T *)A->val32 : (const T *)A->val;
         if (with_spmm)
-            hipLaunchKernelGGL(k_bspmm, dim3(nch), dim3(kT), 0, stream(), ch, pr, N, A->rowptr,
-                               A->colind, A->val, R, AR);
-        hipLaunchKernelGGL(k_bgram, dim3(nch), dim3(kT), 0, stream(), ch, pr, N, nb, X, R, P, AX,
-                           AR, AP, part_g);
+            hipLaunchKernelGGL(k_bspmm<T>, dim3(nch), dim3(kT), 0, stream(), ch, pr, N,
+                               A->rowptr, A->colind, vals, (const T *)bk.R, bk.AR);
+        hipLaunchKernelGGL(k_bgram<T>, dim3(nch), dim3(kT), 0, stream(), ch, pr, N, nb, bk.X,
+                           bk.R, bk.P, bk.AX, bk.AR, bk.AP, part_g);
         hipLaunchKernelGGL(k_bgram_sum, dim3(nprob), dim3(kT), 0, stream(), d_actp, d_nact, d_pcb,
                            d_pcc, pr, nb, part_g, GA, GB);
         if (nb == 1) {
@@ -812,71 +839,133 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
             rr_fast_batched(base, d_actp, d_nact, d_k, nprob, K, nb == 3, (long)kM * kM, kK,
                             (long)kM * kK, (long)rr_words);
         }
-        hipLaunchKernelGGL(k_bupd, dim3(nch), dim3(kT), 0, stream(), ch, pr, N, Cm,
-                           nb == 1 ? nbe : nullptr, nb, X, R, P, AX, AR, AP);
-        hipLaunchKernelGGL(k_bresid, dim3(nch), dim3(kT), 0, stream(), ch, pr, N, X, AX, lam,
-                           s.jacobi ? dv : nullptr, R, part_r);
+        hipLaunchKernelGGL(k_bupd<T>, dim3(nch), dim3(kT), 0, stream(), ch, pr, N, Cm,
+                           nb == 1 ? nbe : nullptr, nb, bk.X, bk.R, bk.P, bk.AX, bk.AR, bk.AP);
+        hipLaunchKernelGGL(k_bresid<T>, dim3(nch), dim3(kT), 0, stream(), ch, pr, N,
+                           (const T *)bk.X, (const T *)bk.AX, lam, s.jacobi ? dv : nullptr, bk.R,
+                           part_r);
         hipLaunchKernelGGL(k_bres_sum, dim3(nprob), dim3(64), 0, stream(), d_actp, d_nact, d_pcb,
                            d_pcc, part_r, res2);
         BX_HIP(hipGetLastError());
     };
-    // X0, AX, Rayleigh-Ritz in span(X), first residuals
-    const uint64_t x0 = (((uint64_t)(uint32_t)s.seed) << 16) | 0x330Eull;
-    hipLaunchKernelGGL(k_bdrand, dim3(nch), dim3(kT), 0, stream(), ch, pr, N, d_sbase, x0, X);
-    BX_HIP(hipGetLastError());
-    spmm_csr(HIPMV_FP64, A, X, N, all, AX, N, all);
-    step(1, false);
 
     std::vector<double> res_h((size_t)nprob * kK), lam_h((size_t)nprob * kK);
     std::vector<int> st_h(nprob);
     std::vector<DevStatus> ds_h(nprob);
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    int captures = 0, replays = 0, it = 0;
-    bool all_done = false;
-    for (;;) {
+    int captures = 0, replays = 0, it = 0, it32 = 0;
+    // One phase: from the Rayleigh-Ritz step in span(X) (AX current), iterate
+    // until every active problem's wanted residuals are below thr, its pencil
+    // broke down, or the phase has run it_cap iterations; done(p, conv) is
+    // called as each problem leaves.
+    auto phase = [&](auto &bk, double thr, int it_cap, int min_active, auto done) {
+        step(bk, 1, false);
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        int pit = 0;
+        for (;;) {
+            download_sync(res_h.data(), res2, res_h.size() * 8);
+            download_sync(st_h.data(), status, nprob * 4);
+            download_sync(ds_h.data(), dstat, sizeof(DevStatus) * nprob);
+            bool changed = false;
+            for (int p = 0; p < nprob; p++) {
+                if (!active[p]) continue;
+                bool conv = true;
+                for (int j = 0; j < kw[p]; j++) conv = conv && sqrt(fabs(res_h[p * kK + j])) <= thr;
+                const bool broke = st_h[p] || ds_h[p].abort;
+                if (conv || broke) {
+                    active[p] = 0;
+                    changed = true;
+                    done(p, conv, pit);
+                }
+            }
+            const int left = (int)std::count(active.begin(), active.end(), 1);
+            if (left == 0 || left < min_active || pit >= it_cap) break;
+            if (changed) rebuild();
+            pit++;
+            it++;
+            const int nb = pit == 1 ? 2 : 3;
+            if (s.graph && nb == 3) {
+                if (!exec) {
+                    BX_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeRelaxed));
+                    step(bk, nb, true);
+                    BX_HIP(hipStreamEndCapture(stream(), &graph));
+                    BX_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+                    captures++;
+                } else {
+                    replays++;
+                }
+                BX_HIP(hipGraphLaunch(exec, stream()));
+            } else {
+                step(bk, nb, true);
+            }
+        }
+        if (exec) {
+            BX_HIP(hipGraphExecDestroy(exec));
+            BX_HIP(hipGraphDestroy(graph));
+        }
+        return pit;
+    };
+    const uint64_t x0 = (((uint64_t)(uint32_t)s.seed) << 16) | 0x330Eull;
+    std::vector<int> it_fp32(nprob, 0);
+    if (mixed) {
+        // FP32 phase to switch_factor * tol (FP64 Gram sums and Rayleigh-Ritz)
+        hipLaunchKernelGGL(k_bdrand<float>, dim3(nch), dim3(kT), 0, stream(), ch, pr, N, d_sbase,
+                           x0, b32.X);
+        BX_HIP(hipGetLastError());
+        spmm_csr(HIPMV_FP32, A, b32.X, N, all, b32.AX, N, all);
+        // the phase ends when fewer than 5% of the problems are left in it (the
+        // last ones would keep the whole batch iterating in FP32)
+        it32 = phase(b32, s.switch_factor * s.tol, s.maxit / 2, std::max(1, nprob / 20),
+                     [&](int p, bool, int pit) { it_fp32[p] = pit; });
+        for (int p = 0; p < nprob; p++)
+            if (active[p]) it_fp32[p] = it32;
+        // every problem continues in FP64 from its FP32 block; AX is recomputed
+        // from X, P starts empty
+        convert_block(N, kK, b32.X, N, HIPMV_FP32, X, N, HIPMV_FP64);
+        BX_HIP(hipMemsetAsync(dstat, 0, sizeof(DevStatus) * nprob, stream()));
+        BX_HIP(hipMemsetAsync(status, 0, nprob * 4, stream()));
+        std::fill(active.begin(), active.end(), 1);
+        rebuild();
+        // a block the FP32 phase left without full rank (its Rayleigh-Ritz
+        // in FP64 fails, or its residuals are not finite) starts again from
+        // X0 in FP64
+        spmm_csr(HIPMV_FP64, A, X, N, all, AX, N, all);
+        step(b64, 1, false);
         download_sync(res_h.data(), res2, res_h.size() * 8);
         download_sync(st_h.data(), status, nprob * 4);
-        download_sync(ds_h.data(), dstat, sizeof(DevStatus) * nprob);
-        for (int p = 0; p < nprob; p++) st_h[p] |= ds_h[p].abort;
-        bool changed = false;
+        bool any = false;
         for (int p = 0; p < nprob; p++) {
-            if (!active[p]) continue;
-            bool conv = true;
-            for (int j = 0; j < kw[p]; j++) conv = conv && sqrt(fabs(res_h[p * kK + j])) <= s.tol;
-            if (conv || st_h[p]) {
-                active[p] = 0;
-                changed = true;
-                flag_h[p] = conv ? 0 : 3;
-                iters_h[p] = it;
-            }
+            bool bad = st_h[p] != 0;
+            for (int j = 0; j < pk[p]; j++) bad = bad || !std::isfinite(res_h[p * kK + j]);
+            active[p] = bad;
+            any = any || bad;
         }
-        all_done = std::none_of(active.begin(), active.end(), [](char a) { return a != 0; });
-        if (all_done || it >= s.maxit) break;
-        if (changed) rebuild();
-        it++;
-        const int nb = it == 1 ? 2 : 3;
-        if (s.graph && nb == 3) {
-            if (!exec) {
-                BX_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeRelaxed));
-                step(nb, true);
-                BX_HIP(hipStreamEndCapture(stream(), &graph));
-                BX_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-                captures++;
-            } else {
-                replays++;
-            }
-            BX_HIP(hipGraphLaunch(exec, stream()));
-        } else {
-            step(nb, true);
+        if (any) {
+            rebuild();
+            hipLaunchKernelGGL(k_bdrand<double>, dim3(nact_ch), dim3(kT), 0, stream(), ch, pr, N,
+                               d_sbase, x0, X);
+            BX_HIP(hipGetLastError());
+            BX_HIP(hipMemsetAsync(status, 0, nprob * 4, stream()));
         }
+        std::fill(active.begin(), active.end(), 1);
+        rebuild();
+    } else {
+        hipLaunchKernelGGL(k_bdrand<double>, dim3(nch), dim3(kT), 0, stream(), ch, pr, N,
+                           d_sbase, x0, X);
+        BX_HIP(hipGetLastError());
     }
+    spmm_csr(HIPMV_FP64, A, X, N, all, AX, N, all);
+    phase(b64, s.tol, s.maxit - it32, 1, [&](int p, bool conv, int pit) {
+        flag_h[p] = conv ? 0 : 3;
+        iters_h[p] = it_fp32[p] + pit;
+    });
     for (int p = 0; p < nprob; p++)
         if (active[p]) iters_h[p] = it;
-    if (exec) {
-        BX_HIP(hipGraphExecDestroy(exec));
-        BX_HIP(hipGraphDestroy(graph));
-    }
+    if (mixed)
+        for (float *q : {b32.X, b32.AX, b32.R, b32.AR, b32.P, b32.AP}) {
+            BX_HIP(hipStreamSynchronize(stream()));
+            BX_HIP(hipFree(q));
+        }
     download_sync(lam_h.data(), lam, lam_h.size() * 8);
     download_sync(res_h.data(), res2, res_h.size() * 8);
     size_t o = 0, xo = 0;
@@ -911,6 +1000,7 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
     if (st) {
         st->seconds = secs;
         st->iterations = it;
+        st->iterations_fp32 = it32;
         st->graph_captures = captures;
         st->graph_replays = replays;
         st->total_rows = N;

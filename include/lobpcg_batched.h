@@ -14,6 +14,8 @@
 #ifndef BLOPEX_AMD_LOBPCG_BATCHED_H
 #define BLOPEX_AMD_LOBPCG_BATCHED_H
 
+#include "hip_multivector.h" /* HIPLOBPCG_FP64, HIPLOBPCG_MIXED */
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -39,15 +41,21 @@ typedef struct hip_BatchSettings {
     int guard;          /* extra columns iterated beside the k wanted, up to
                            HIP_BATCH_MAX_K + 8 columns in all and n/3; only the k wanted
                            are tested for convergence and returned (default 2) */
+    int mode;           /* HIPLOBPCG_FP64 (default), or HIPLOBPCG_MIXED: the blocks in FP32
+                           (FP64 Gram sums and Rayleigh-Ritz) until every problem's wanted
+                           residuals are below switch_factor * tol or maxit / 2 iterations
+                           have run, then every problem continues in FP64 from its block */
+    double switch_factor; /* MIXED: see mode (default 100) */
 } hip_BatchSettings;
 
 typedef struct hip_BatchStats {
     double seconds;     /* wall time of the solve (device-resident inputs excluded) */
     int iterations;     /* batch iterations (the slowest problem's count) */
-    int graph_captures; /* graphs captured (one per solve: the active lists and counts
+    int graph_captures; /* graphs captured (one per phase: the active lists and counts
                            live in device memory) */
-    int graph_replays;  /* iterations replayed from the captured graph */
+    int graph_replays;  /* iterations replayed from the captured graphs */
     long total_rows;    /* sum of n */
+    int iterations_fp32; /* MIXED: batch iterations of the FP32 phase */
 } hip_BatchStats;
 
 void hip_BatchSettingsDefault(hip_BatchSettings *s);

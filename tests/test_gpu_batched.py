@@ -83,3 +83,18 @@ def test_batched_argument_errors():
         bl.lobpcg_batched([(A, 4)])
     with pytest.raises(RuntimeError, match="1 <= k"):
         bl.lobpcg_batched([(pb.laplacian_5pt(10, 10), 17)])
+
+
+def test_batched_mixed_precision():
+    """MIXED: the blocks in FP32 (FP64 Gram sums and Rayleigh-Ritz) to 100 x tol,
+    then FP64 from the FP32 blocks: the same eigenpairs as the FP64 solve
+    (1e-9) and the residuals of the returned pairs at the tolerance."""
+    say("batched solve, mixed precision (32 problems of the workload's generator)")
+    problems = pb.batched_problems(32, smin=16, smax=48)
+    r = bl.lobpcg_batched(problems, tol=1e-6, maxit=1000, want_X=True, mode="mixed")
+    say(f"  max iterations {r.iterations.max()} stats {r.stats}")
+    assert r.stats["iterations_fp32"] > 0
+    _check(problems, r, 1e-6, 1e-8)
+    f = bl.lobpcg_batched(problems, tol=1e-6, maxit=1000)
+    for a, b in zip(r.lam, f.lam):
+        assert np.max(np.abs(a - b) / b) < 1e-9
