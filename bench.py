@@ -133,26 +133,36 @@ def cpu_baseline(A, n_side, k, seed_x=1):
                            f"({(t3 - t1) + (t1 - t0):.1f} s of CPU work), scaled by n x{scale:.0f}"
 
 
-def run_batched(count, graph=True, maxit=2000):
+def run_batched(count, graph=True, maxit=2000, mode="mixed", compare_fp64=True):
     """The batched workload: count independent 2D 5-point lognormal(0, 0.5)
-    diffusion problems (sides 32..256, k in {4, 8, 16}, Jacobi, FP64,
-    tolerance 1e-6) in one hip_LobpcgSolveBatched call."""
+    diffusion problems (sides 32..256, k in {4, 8, 16}, Jacobi, tolerance
+    1e-6) in one hip_LobpcgSolveBatched call: FP32 blocks with FP64 Gram sums
+    and Rayleigh-Ritz, then FP64 (mode "mixed"), and the FP64 solve beside it."""
     from blopex_amd import problems as pb
     from blopex_amd.lobpcg import lobpcg_batched
 
     problems = pb.batched_problems(count)
     t = time.perf_counter()
-    r = lobpcg_batched(problems, tol=1e-6, maxit=maxit, graph=graph)
+    r = lobpcg_batched(problems, tol=1e-6, maxit=maxit, graph=graph, mode=mode)
     wall = time.perf_counter() - t
     conv = int(np.sum(r.flags == 0))
     pairs = sum(k for _, k in problems)
-    return {"workload": f"{count} 2D 5-point lognormal(0,0.5) problems, sides 32..256, "
-                        "k in {4,8,16}, Jacobi, FP64, tol 1e-6 (batched)",
-            "problems": count, "rows": int(r.stats["total_rows"]), "eigenpairs": pairs,
-            "converged": conv, "maxit": maxit, "seconds": r.stats["seconds"],
-            "wall_seconds": wall, "iterations_max": int(r.iterations.max()),
-            "iterations_mean": float(r.iterations.mean()),
-            "graph_replays": r.stats["graph_replays"]}
+    out = {"workload": f"{count} 2D 5-point lognormal(0,0.5) problems, sides 32..256, "
+                       f"k in {{4,8,16}}, Jacobi, {mode}, tol 1e-6 (batched)",
+           "problems": count, "rows": int(r.stats["total_rows"]), "eigenpairs": pairs,
+           "converged": conv, "maxit": maxit, "seconds": r.stats["seconds"],
+           "wall_seconds": wall, "iterations_max": int(r.iterations.max()),
+           "iterations_mean": float(r.iterations.mean()),
+           "iterations_fp32": r.stats["iterations_fp32"],
+           "graph_replays": r.stats["graph_replays"]}
+    if compare_fp64 and mode != "fp64":
+        f = lobpcg_batched(problems, tol=1e-6, maxit=maxit, graph=graph)
+        out["fp64"] = {"seconds": f.stats["seconds"], "converged": int(np.sum(f.flags == 0)),
+                       "iterations_mean": float(f.iterations.mean())}
+        both = [i for i in range(count) if r.flags[i] == 0 and f.flags[i] == 0]
+        out["max_rel_eig_diff_vs_fp64"] = float(max(
+            np.max(np.abs(r.lam[i] - f.lam[i]) / np.abs(f.lam[i])) for i in both))
+    return out
 
 
 def main():
@@ -179,6 +189,7 @@ def main():
     ap.add_argument("--batched-count", type=int, default=512)
     ap.add_argument("--batched-no-graph", action="store_true",
                     help="batched workload with eager launches instead of graph replays")
+    ap.add_argument("--batched-mode", default="mixed", choices=["mixed", "fp64"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -192,7 +203,8 @@ def main():
     lib = L.load()
     lib.blopex_amd_init()
     if args.batched_only:
-        print(json.dumps(run_batched(args.batched_count, not args.batched_no_graph)), flush=True)
+        print(json.dumps(run_batched(args.batched_count, not args.batched_no_graph,
+                                     mode=args.batched_mode, compare_fp64=False)), flush=True)
         return
     nside, k = args.n, args.k
     t = time.perf_counter()
@@ -377,7 +389,8 @@ def main():
     for o in (X, X0, dT, dA):
         o.close()
     if not args.no_batched:
-        out["batched"] = run_batched(args.batched_count, not args.batched_no_graph)
+        out["batched"] = run_batched(args.batched_count, not args.batched_no_graph,
+                                     mode=args.batched_mode)
         log(f"batched: {out['batched']}")
     print(json.dumps(out), flush=True)
 

@@ -40,6 +40,7 @@ constexpr int kK = HIP_BATCH_MAX_K + 8;  // largest block width: k wanted plus g
 constexpr int kM = 3 * kK;               // largest pencil
 constexpr int kT = 256;                  // threads per workgroup
 constexpr int kChunk = 2048;             // rows per chunk
+constexpr int kRefresh = 20;             // FP32 iterations between recomputations of AX
 constexpr int kSub = 32;                 // rows staged per Gram step
 constexpr int kPairs = kM * (kM + 1) / 2;
 constexpr int kPairsPerThread = (kPairs + kT - 1) / kT;
@@ -884,6 +885,17 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
             pit++;
             it++;
             const int nb = pit == 1 ? 2 : 3;
+            // FP32 blocks: AX follows X by the update recurrence, whose rounding
+            // accumulates; every kRefresh iterations it is recomputed from X
+            // (residual replacement) so that the FP32 residuals stay those of X
+            if (std::is_same<typename std::remove_reference<decltype(bk)>::type::type,
+                             float>::value &&
+                pit % kRefresh == 0) {
+                using T = typename std::remove_reference<decltype(bk)>::type::type;
+                hipLaunchKernelGGL(k_bspmm<T>, dim3(nch), dim3(kT), 0, stream(), ch, pr, N,
+                                   A->rowptr, A->colind, (const T *)A->val32, (const T *)bk.X,
+                                   bk.AX);
+            }
             if (s.graph && nb == 3) {
                 if (!exec) {
                     BX_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeRelaxed));
