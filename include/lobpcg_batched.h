@@ -42,9 +42,10 @@ typedef struct hip_BatchSettings {
                            HIP_BATCH_MAX_K + 8 columns in all and n/3; only the k wanted
                            are tested for convergence and returned (default 2) */
     int mode;           /* HIPLOBPCG_FP64 (default), or HIPLOBPCG_MIXED: the blocks in FP32
-                           (FP64 Gram sums and Rayleigh-Ritz) until every problem's wanted
-                           residuals are below switch_factor * tol or maxit / 2 iterations
-                           have run, then every problem continues in FP64 from its block */
+                           (FP64 Gram sums and Rayleigh-Ritz, AX recomputed from X every 20
+                           iterations) until all but 5% of the problems have their wanted
+                           residuals below switch_factor * tol (or maxit / 2 iterations
+                           have run), then every problem continues in FP64 from its block */
     double switch_factor; /* MIXED: see mode (default 100) */
 } hip_BatchSettings;
 
