@@ -22,7 +22,7 @@ from oracle import pyoracle as po  # noqa: E402
 
 
 def cases():
-    # config 1: 20^3 constant 7-point, k=10, no B/T, drand48 seed 1, tol 1e-6/1e-50, maxit 200
+    # 20^3 constant 7-point, k=10, no B/T, drand48 seed 1, tol 1e-6/1e-50, maxit 200
     A = pb.laplacian_7pt(20, 20, 20)
     yield "cfg1", dict(A=A, X0=po.drand48_block(A.n, 10, 1, which="ref"), maxit=200)
     # stock serial driver: diag(1..40), k=5, maxit 100 (serial_driver.c:44-175)
@@ -33,7 +33,7 @@ def cases():
     A = pb.dense_to_csr(np.diag(np.arange(1.0, n + 1)))
     yield "testdiag", dict(A=A, X0=po.drand48_block(n, 3, 1, which="ref"), Y=np.eye(n, 3),
                            tdiag=1.0 / np.arange(1.0, n + 1), atol=1e-4, maxit=40)
-    # config 3: config 1's A, B = diag(uniform[0.5, 1.5]) (seed 7), k = 64, no T; capped at 120
+    # generalized: the 20^3 A, B = diag(uniform[0.5, 1.5]) (seed 7), k = 64, no T; capped at 120
     # iterations (the reference runs on and breaks down at iteration 158)
     A = pb.laplacian_7pt(20, 20, 20)
     yield "cfg3", dict(A=A, X0=po.drand48_block(A.n, 64, 1, which="ref"),

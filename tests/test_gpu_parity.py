@@ -5,7 +5,7 @@ Bars (written per test):
 * single kernels in FP64 agree with the oracle / numpy FP64 to 1e-13 relative
   (summation order differs), FP32 kernels to 1e-5 relative;
 * FP64 solves reproduce the reference's eigenvalues to 1e-8 relative at the
-  same tolerance, with the same exit flag and an
+  same tolerance (the parity bar), with the same exit flag and an
   iteration count within max(2, 5%) of the reference's.
 """
 import ctypes
@@ -321,8 +321,8 @@ def test_device_fp64_matches_reference(golden, name):
     assert np.all(r.res[: len(ref_lam)] <= kw.get("atol", 1e-6) + 1e-15)
 
 
-def test_device_fp64_config3_matches_reference(golden):
-    """Config 3: config 1's A, diagonal B, k = 64, no T (pencils of order 192
+def test_device_fp64_generalized_k64_matches_reference(golden):
+    """The 20^3 Laplacian with a diagonal B, k = 64, no T (pencils of order 192
     on the general Rayleigh-Ritz kernel), 120 iterations, where the reference
     stops short of convergence.  Which eigenpairs a run has converged by then
     is path-dependent (the 20^3 Laplacian's eigenvalues come in degenerate
@@ -337,7 +337,7 @@ def test_device_fp64_config3_matches_reference(golden):
     import scipy.sparse as sp
     import scipy.sparse.linalg as sla
 
-    say("device FP64 solve cfg3 (k=64, B diagonal)")
+    say("device FP64 solve, 20^3 generalized (k=64, B diagonal)")
     kw = CASES["cfg3"]
     r = _device_solve(kw, final_rr=True)
     assert r.stats["final_rr"] == 1
@@ -365,7 +365,7 @@ def test_device_fp64_config3_matches_reference(golden):
 def test_generic_path_through_hip_interpreter(golden):
     """lobpcg_solve with the library's small-dense interpreters and scipy's
     LAPACK, every block op through the HIP interpreter slots."""
-    say("generic lobpcg_solve via HIP interpreter, cfg1")
+    say("generic lobpcg_solve via HIP interpreter, 20^3 Laplacian")
     lib = L.load()
     kw = CASES["cfg1"]
     A = bl.DeviceCSR(kw["A"])
@@ -421,7 +421,7 @@ def test_mixed_precision_small(refresh):
     """MIXED mode (FP32 phase with AX recomputed every `refresh` iterations,
     0: never) meets the tolerance with FP64 residuals and the oracle's
     eigenvalues."""
-    say(f"mixed precision cfg1-like (16^3, k=8, jacobi), refresh {refresh}")
+    say(f"mixed precision (16^3, k=8, jacobi), refresh {refresh}")
     A = pb.laplacian_7pt(16, 16, 16, sigma=1.0, seed=5)
     X0 = po.drand48_block(A.n, 8, 1)
     o = po.solve(A=A, X0=X0, tdiag=1.0 / A.diagonal(), maxit=400)
@@ -524,13 +524,13 @@ def test_refinement_iterations():
     assert e1 < e0
 
 
-def test_mixed_config3_against_fp64():
-    """Config 3 in MIXED mode (FP32 blocks with FP64 Gram accumulation and
-    Rayleigh-Ritz, FP64 finish): config 1's A, diagonal B uniform[0.5, 1.5],
+def test_mixed_generalized_k64_against_fp64():
+    """MIXED mode (FP32 blocks with FP64 Gram accumulation and Rayleigh-Ritz,
+    FP64 finish) on the 20^3 Laplacian with a diagonal B uniform[0.5, 1.5],
     k = 64, no T.  Every pair both runs converge is the same eigenpair: the
     eigenvalues agree to 1e-9 relative, and the residuals of the MIXED run,
     recomputed on the host after its final Rayleigh-Ritz, meet the tolerance."""
-    say("config 3, mixed vs fp64 (k=64, B diagonal)")
+    say("20^3 generalized, mixed vs fp64 (k=64, B diagonal)")
     kw = dict(CASES["cfg3"], maxit=400)
     r64 = _device_solve(kw, final_rr=True)
     rmx = _device_solve(kw, mode="mixed", final_rr=True)

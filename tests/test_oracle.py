@@ -3,7 +3,7 @@
 * bit-for-bit against the reference's own lobpcg_solve_double on the golden
   fixtures (tests/golden/ref_fixtures.npz, written by make_golden.py from
   the compiled reference) and, where oracle/_ref is built, live;
-* against closed forms: the Dirichlet Laplacian spectrum (config 1),
+* against closed forms: the Dirichlet Laplacian spectrum (the 20^3 case),
   diag(1..40) -> 1..5 (serial_driver.c), diag(1..n) with 3 constraints ->
   4, 5, 6 (test_diag.m), and a generalized pencil against scipy's eigh;
 * a planted mistake in the oracle's inputs must fail the checks.
@@ -61,7 +61,7 @@ def test_drand48_stream(golden):
     np.testing.assert_array_equal(2 * c - 1, po.drand48_block(40000, 1, 1)[:, 0])
 
 
-def test_config1_analytic():
+def test_laplacian_20cube_analytic():
     kw = _case("cfg1")
     r = po.solve(which="oracle", **kw)
     exact = pb.dirichlet_eigenvalues((20, 20, 20), 10)
