@@ -56,12 +56,14 @@ class BatchProblem(ctypes.Structure):
 
 class BatchSettings(ctypes.Structure):
     _fields_ = [("tol", c_double), ("maxit", c_int), ("jacobi", c_int), ("seed", c_int),
-                ("graph", c_int), ("guard", c_int), ("mode", c_int), ("switch_factor", c_double)]
+                ("graph", c_int), ("guard", c_int), ("mode", c_int), ("switch_factor", c_double),
+                ("autotune", c_int)]
 
 
 class BatchStats(ctypes.Structure):
     _fields_ = [("seconds", c_double), ("iterations", c_int), ("graph_captures", c_int),
-                ("graph_replays", c_int), ("total_rows", c_long), ("iterations_fp32", c_int)]
+                ("graph_replays", c_int), ("total_rows", c_long), ("iterations_fp32", c_int),
+                ("chunk_rows", c_int)]
 
 
 class HipMultiVector(ctypes.Structure):

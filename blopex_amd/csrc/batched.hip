@@ -5,7 +5,7 @@
 // matrix of N = sum n_p rows, and every block (X, AX, R, AR, P, AP) is one
 // N x K FP64 array (K = the largest k, column-major, leading dimension N);
 // problem p owns rows off_p .. off_p + n_p and columns 0 .. k_p.  Rows are
-// cut into chunks of at most kChunk rows that never straddle two problems;
+// cut into chunks (2048 rows, or the autotuner's choice) that never straddle two problems;
 // every grouped kernel is launched over the chunks of the unconverged
 // problems, so one launch advances the whole batch:
 //   k_bresid     R = T (AX - X diag(lambda)) and per-chunk partial norms
@@ -662,6 +662,7 @@ void hip_BatchSettingsDefault(hip_BatchSettings *s) {
     s->guard = 2;
     s->mode = HIPLOBPCG_FP64;
     s->switch_factor = 100.0;
+    s->autotune = 1;
 }
 
 int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_BatchSettings *set,
@@ -727,21 +728,32 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
             }
             dinv[r + i] = d != 0.0 ? 1.0 / d : 1.0;
         }
-        chunk_first[p] = (int)chunk_prob.size();
-        for (long c0 = 0; c0 < q.n; c0 += kChunk) {
-            chunk_prob.push_back(p);
-            chunk_r0.push_back(r + c0);
-            chunk_nr.push_back((int)std::min<long>(kChunk, q.n - c0));
-        }
-        chunk_cnt[p] = (int)chunk_prob.size() - chunk_first[p];
         r += q.n;
         e += q.nnz;
     }
     rowptr[N] = (int)nnz;
+    // chunks of at most `rows` rows, problem by problem
+    auto make_chunks = [&](int rows) {
+        chunk_prob.clear();
+        chunk_r0.clear();
+        chunk_nr.clear();
+        for (int p = 0; p < nprob; p++) {
+            chunk_first[p] = (int)chunk_prob.size();
+            for (long c0 = 0; c0 < pn[p]; c0 += rows) {
+                chunk_prob.push_back(p);
+                chunk_r0.push_back(off[p] + c0);
+                chunk_nr.push_back((int)std::min<long>(rows, pn[p] - c0));
+            }
+            chunk_cnt[p] = (int)chunk_prob.size() - chunk_first[p];
+        }
+        return (int)chunk_prob.size();
+    };
+    const int kChunkCand[] = {512, 1024, 2048, 4096};
+    const int nch_max = make_chunks(s.autotune ? kChunkCand[0] : kChunk);
     const auto t0 = std::chrono::steady_clock::now();
     hip_CSRMatrix *A = hip_CSRMatrixCreate(N, nnz, rowptr.data(), colind.data(), val.data());
     if (!A) return -1;
-    const int nch = (int)chunk_prob.size();
+    int nch = make_chunks(kChunk), chunk_rows = kChunk;
     const size_t nk = (size_t)N * kK;
     double *X = dev_alloc<double>(nk), *AX = dev_alloc<double>(nk), *R = dev_alloc<double>(nk),
            *AR = dev_alloc<double>(nk), *P = dev_alloc<double>(nk), *AP = dev_alloc<double>(nk);
@@ -754,8 +766,8 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
     upload_async(d_sbase, sbase.data(), nprob * 8);
     upload_async(d_n, pn.data(), nprob * 4);
     upload_async(d_k, pk.data(), nprob * 4);
-    int *c_prob = dev_alloc<int>(nch), *c_nr = dev_alloc<int>(nch);
-    long *c_r0 = dev_alloc<long>(nch);
+    int *c_prob = dev_alloc<int>(nch_max), *c_nr = dev_alloc<int>(nch_max);
+    long *c_r0 = dev_alloc<long>(nch_max);
     int *d_actp = dev_alloc<int>(nprob), *d_pcb = dev_alloc<int>(nprob),
         *d_pcc = dev_alloc<int>(nprob), *d_counts = dev_alloc<int>(2);
     double *lam = dev_alloc<double>((size_t)nprob * kK), *res2 = dev_alloc<double>((size_t)nprob * kK);
@@ -769,8 +781,8 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
     double *rr_work = dev_alloc<double>((size_t)nprob * rr_words);
     DevStatus *dstat = dev_alloc<DevStatus>(nprob);
     BX_HIP(hipMemsetAsync(dstat, 0, sizeof(DevStatus) * nprob, stream()));
-    double *part_r = dev_alloc<double>((size_t)nch * kK);
-    double *part_g = dev_alloc<double>((size_t)nch * 2 * kPairs);
+    double *part_r = dev_alloc<double>((size_t)nch_max * kK);
+    double *part_g = dev_alloc<double>((size_t)nch_max * 2 * kPairs);
     BX_HIP(hipMemsetAsync(lam, 0, (size_t)nprob * kK * 8, stream()));
     BX_HIP(hipMemsetAsync(status, 0, nprob * 4, stream()));
 
@@ -917,6 +929,49 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
         }
         return pit;
     };
+    // Autotuner: the rows per chunk (the work of one workgroup in the grouped
+    // kernels) that make the SpMM and the three-block Gram pass fastest on
+    // this batch, timed with HIP events on the blocks of the first phase
+    // (still zero: the kernels' cost does not depend on the values)
+    auto tune = [&](auto &bk) {
+        using T = typename std::remove_reference<decltype(bk)>::type::type;
+        const T *vals = std::is_same<T, float>::value ? (const T *)A->val32 : (const T *)A->val;
+        hipEvent_t e0, e1;
+        BX_HIP(hipEventCreate(&e0));
+        BX_HIP(hipEventCreate(&e1));
+        float best = 1e30f;
+        for (int rows : kChunkCand) {
+            nch = make_chunks(rows);
+            rebuild();
+            float ms = 0.f;
+            for (int rep = 0; rep < 3; rep++) {
+                BX_HIP(hipEventRecord(e0, stream()));
+                hipLaunchKernelGGL(k_bspmm<T>, dim3(nch), dim3(kT), 0, stream(), ch, pr, N,
+                                   A->rowptr, A->colind, vals, (const T *)bk.R, bk.AR);
+                hipLaunchKernelGGL(k_bgram<T>, dim3(nch), dim3(kT), 0, stream(), ch, pr, N, 3,
+                                   bk.X, bk.R, bk.P, bk.AX, bk.AR, bk.AP, part_g);
+                BX_HIP(hipEventRecord(e1, stream()));
+                BX_HIP(hipEventSynchronize(e1));
+                float t = 0.f;
+                BX_HIP(hipEventElapsedTime(&t, e0, e1));
+                if (rep > 0) ms += t;  // the first launch warms up
+            }
+            if (ms < best) {
+                best = ms;
+                chunk_rows = rows;
+            }
+        }
+        BX_HIP(hipEventDestroy(e0));
+        BX_HIP(hipEventDestroy(e1));
+        nch = make_chunks(chunk_rows);
+        rebuild();
+    };
+    if (s.autotune) {
+        if (mixed)
+            tune(b32);
+        else
+            tune(b64);
+    }
     const uint64_t x0 = (((uint64_t)(uint32_t)s.seed) << 16) | 0x330Eull;
     std::vector<int> it_fp32(nprob, 0);
     if (mixed) {
@@ -1013,6 +1068,7 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
         st->seconds = secs;
         st->iterations = it;
         st->iterations_fp32 = it32;
+        st->chunk_rows = chunk_rows;
         st->graph_captures = captures;
         st->graph_replays = replays;
         st->total_rows = N;

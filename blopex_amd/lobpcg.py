@@ -328,7 +328,7 @@ class BatchResult:
 def lobpcg_batched(problems, tol: float = 1e-6, maxit: int = 500, jacobi: bool = True,
                    seed: int = 1, graph: bool = True, want_X: bool = False,
                    guard: int = 2, mode: str = "fp64",
-                   switch_factor: float = 100.0) -> BatchResult:
+                   switch_factor: float = 100.0, autotune: bool = True) -> BatchResult:
     """Smallest eigenpairs of many independent problems in one device solve
     (hip_LobpcgSolveBatched).  problems: list of (CSR, k)."""
     lib = L.load()
@@ -344,6 +344,7 @@ def lobpcg_batched(problems, tol: float = 1e-6, maxit: int = 500, jacobi: bool =
     s.guard = guard
     s.mode = L.HIPLOBPCG_MIXED if mode == "mixed" else L.HIPLOBPCG_FP64
     s.switch_factor = switch_factor
+    s.autotune = int(autotune)
     ks = [k for _, k in problems]
     lam = np.zeros(sum(ks))
     res = np.zeros(sum(ks))

@@ -47,6 +47,9 @@ typedef struct hip_BatchSettings {
                            residuals below switch_factor * tol (or maxit / 2 iterations
                            have run), then every problem continues in FP64 from its block */
     double switch_factor; /* MIXED: see mode (default 100) */
+    int autotune;       /* 1: pick the rows per chunk (the work of one workgroup: 512, 1024,
+                           2048 or 4096) by timing the SpMM and Gram kernels on this batch
+                           before the solve (default 1); 0: 2048 */
 } hip_BatchSettings;
 
 typedef struct hip_BatchStats {
@@ -57,6 +60,7 @@ typedef struct hip_BatchStats {
     int graph_replays;  /* iterations replayed from the captured graphs */
     long total_rows;    /* sum of n */
     int iterations_fp32; /* MIXED: batch iterations of the FP32 phase */
+    int chunk_rows;     /* rows per chunk used */
 } hip_BatchStats;
 
 void hip_BatchSettingsDefault(hip_BatchSettings *s);

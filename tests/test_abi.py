@@ -67,8 +67,8 @@ def test_struct_layouts():
     # hip_BatchProblem {long, long, 3 pointers, int}, hip_BatchSettings {double, 5 ints},
     # hip_BatchStats {double, 3 ints, long}
     assert ctypes.sizeof(L.BatchProblem) == 48 and L.BatchProblem.k.offset == 40
-    assert ctypes.sizeof(L.BatchSettings) == 40 and L.BatchSettings.switch_factor.offset == 32
-    assert ctypes.sizeof(L.BatchStats) == 40 and L.BatchStats.iterations_fp32.offset == 32
+    assert ctypes.sizeof(L.BatchSettings) == 48 and L.BatchSettings.autotune.offset == 40
+    assert ctypes.sizeof(L.BatchStats) == 40 and L.BatchStats.chunk_rows.offset == 36
 
 
 def test_adjoint_defaults_and_argument_checks(lib):

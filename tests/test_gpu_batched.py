@@ -56,11 +56,13 @@ def _check(problems, r, tol, lam_rtol):
 def test_batched_small_problems_and_graph_replay():
     say("batched solve, 12 small problems (graph and eager)")
     problems = _small_batch()
-    r = bl.lobpcg_batched(problems, tol=1e-8, maxit=500, want_X=True)
+    # a fixed chunk size: the autotuner's choice sets the summation order
+    r = bl.lobpcg_batched(problems, tol=1e-8, maxit=500, want_X=True, autotune=False)
     say(f"  iterations {r.iterations.tolist()} stats {r.stats}")
     _check(problems, r, 1e-8, 1e-9)
     assert r.stats["graph_replays"] > 0
-    e = bl.lobpcg_batched(problems, tol=1e-8, maxit=500, want_X=True, graph=False)
+    e = bl.lobpcg_batched(problems, tol=1e-8, maxit=500, want_X=True, graph=False,
+                          autotune=False)
     assert e.stats["graph_replays"] == 0
     np.testing.assert_array_equal(e.iterations, r.iterations)
     for a, b in zip(r.lam, e.lam):
@@ -74,6 +76,7 @@ def test_batched_workload_generator_reduced():
     problems = pb.batched_problems(32, smin=16, smax=48)
     r = bl.lobpcg_batched(problems, tol=1e-6, maxit=1000, want_X=True)
     say(f"  max iterations {r.iterations.max()} stats {r.stats}")
+    assert r.stats["chunk_rows"] in (512, 1024, 2048, 4096)
     _check(problems, r, 1e-6, 1e-8)
 
 
