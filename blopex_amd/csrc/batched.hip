@@ -3,11 +3,12 @@
 //
 // Layout: the problems are stacked row-wise into one block-diagonal CSR
 // matrix of N = sum n_p rows, and every block (X, AX, R, AR, P, AP) is one
-// N x K FP64 array (K = the largest k, column-major, leading dimension N);
-// problem p owns rows off_p .. off_p + n_p and columns 0 .. k_p.  Rows are
-// cut into chunks (2048 rows, or the autotuner's choice) that never straddle two problems;
-// every grouped kernel is launched over the chunks of the unconverged
-// problems, so one launch advances the whole batch:
+// N x 24 array (FP64; FP32 copies for the mixed phase; column-major, leading
+// dimension N); problem p owns rows off_p .. off_p + n_p and its first k_p
+// columns (k wanted plus guard columns).  Rows are cut into chunks (2048
+// rows, or the autotuner's choice) that never straddle two problems; every
+// grouped kernel is launched over the chunks of the unconverged problems, so
+// one launch advances the whole batch:
 //   k_bresid     R = T (AX - X diag(lambda)) and per-chunk partial norms
 //   k_bspmm      AR = A R over the active chunks of the stacked CSR matrix
 //   k_bgram      per-chunk partial Gram blocks [X R P]' [AX AR AP] and
@@ -236,13 +237,15 @@ __global__ void __launch_bounds__(kT) k_bgram_sum(const int *act_prob, const int
     }
 }
 
-// ---- batched Rayleigh-Ritz, one workgroup per problem ----
+// ---- the initial Rayleigh-Ritz in span(X), one workgroup per problem ----
 constexpr int kRT = 256;
-// Solves GA v = theta GB v on the leading m = nb kp rows / columns; writes
-// the kp smallest theta to lam and the GB-orthonormal v to C (pencil rows,
-// kp columns, leading dim kM), nbe[p] = blocks used.  A GB that is not
-// positive definite is retried without P (nb = 3 -> 2); if that fails too,
-// status[p] = 1 and X is left as it is (C = [I; 0]).
+// Solves GA v = theta GB v on the leading m = nb kp rows / columns (launched
+// with nb = 1); writes the kp smallest theta to lam and the GB-orthonormal v
+// to C (pencil rows, kp columns, leading dim kM), nbe[p] = blocks used.  A GB
+// that is not positive definite is retried with one block fewer; if no
+// pencil is, status[p] = 1 and X is left as it is (C = [I; 0]).  The sweeps
+// of its cyclic Jacobi run to completion (off-diagonal mass below 1e-32 of
+// the diagonal's, or 30 sweeps).
 __global__ void __launch_bounds__(kRT) k_brr(const int *act_prob, const int *nact, Probs pr, int nb,
                                              const double *GA, const double *GB, double *lam,
                                              double *C, int *nbe, int *status) {
