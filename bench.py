@@ -165,6 +165,39 @@ def run_batched(count, graph=True, maxit=2000, mode="mixed", compare_fp64=True):
     return out
 
 
+def run_adjoint(N=64, mode="mixed"):
+    """The differentiation workload: the 7-point lognormal(0, 1) operator on an
+    N^3 grid, loss = sum of the 8 smallest eigenvalues + ||x_1||^2 on the
+    central (N/4)^3 box; forward solve (11 columns, tol 1e-10, final
+    Rayleigh-Ritz) and hip_LobpcgAdjoint (gradients of the CSR values)."""
+    from blopex_amd import problems as pb
+    from blopex_amd.lobpcg import adjoint, lobpcg
+
+    c = pb.lognormal_coefficients(N ** 3, 1.0, 64)
+    A = pb.diffusion_stencil((N, N, N), c)
+    t = time.perf_counter()
+    r = lobpcg(None, A, T="jacobi", k=11, tol=1e-10, maxit=4000, mode=mode, final_rr=True)
+    t_fwd = time.perf_counter() - t
+    lo, hi = 3 * N // 8, 5 * N // 8
+    box = np.zeros((N, N, N), dtype=bool)
+    box[lo:hi, lo:hi, lo:hi] = True
+    box = box.reshape(-1)
+    glam = np.r_[np.ones(8), np.zeros(3)]
+    GX = np.zeros_like(r.X)
+    GX[:, 0] = 2.0 * r.X[:, 0] * box
+    t = time.perf_counter()
+    ad = adjoint(A, r.X, r.lam, glam, GX, T="jacobi", mode=mode, tol=1e-10, maxit=5000)
+    t_adj = time.perf_counter() - t
+    return {"workload": f"{N}^3 7-point lognormal(0,1), loss sum(lambda_1..8) + "
+                        f"||x_1 on the central {hi - lo}^3 box||^2, {mode}",
+            "forward_flag": int(r.flag), "forward_iterations": int(r.iterations),
+            "forward_seconds": t_fwd, "adjoint_flag": int(ad.flag),
+            "adjoint_cg_iterations": int(ad.stats["iterations"]),
+            "adjoint_corrections": int(ad.stats["corrections"]),
+            "adjoint_seconds": ad.stats["seconds"], "adjoint_wall_seconds": t_adj,
+            "gradient_entries": int(A.nnz)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -190,6 +223,8 @@ def main():
     ap.add_argument("--batched-no-graph", action="store_true",
                     help="batched workload with eager launches instead of graph replays")
     ap.add_argument("--batched-mode", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the differentiation workload (64^3 forward solve + adjoint)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -392,6 +427,9 @@ def main():
         out["batched"] = run_batched(args.batched_count, not args.batched_no_graph,
                                      mode=args.batched_mode)
         log(f"batched: {out['batched']}")
+    if not args.no_adjoint:
+        out["adjoint"] = run_adjoint()
+        log(f"adjoint: {out['adjoint']}")
     print(json.dumps(out), flush=True)
 
 
