@@ -64,7 +64,52 @@ __global__ void __launch_bounds__(kThreads) k_diag_outer(long n, const double *x
     }
 }
 
+// CG step lengths on the device: a[j] = rz[j] / pq[j] and na[j] = -a[j] for
+// the columns still running (0 for the finished ones); a non-positive pq of a
+// running column sets *bad = j + 1
+__global__ void k_cg_alpha(int m, const double *rz, const double *pq, const int *done, double *a,
+                           double *na, int *bad) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m) return;
+    double v = 0.0;
+    if (!done[j]) {
+        if (!(pq[j] > 0.0) || !isfinite(rz[j]))
+            *bad = j + 1;
+        else
+            v = rz[j] / pq[j];
+    }
+    a[j] = v;
+    na[j] = -v;
+}
+
+// after the residual update: a column finishes once ||r|| <= target; beta =
+// rz_new / rz for the running ones (0 for the finished), rz <- rz_new
+__global__ void k_cg_beta(int m, const double *rr, const double *rzn, const double *target,
+                          int *done, double *rz, double *beta) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m) return;
+    if (!done[j] && sqrt(fmax(rr[j], 0.0)) <= target[j]) done[j] = 1;
+    beta[j] = done[j] ? 0.0 : rzn[j] / rz[j];
+    rz[j] = rzn[j];
+}
+
 }  // namespace
+
+void cg_alpha(int m, const double *rz, const double *pq, const int *done, double *a, double *na,
+              int *bad) {
+    if (m == 0) return;
+    hipLaunchKernelGGL(k_cg_alpha, dim3((m + 63) / 64), dim3(64), 0, stream(), m, rz, pq, done, a,
+                       na, bad);
+    BX_HIP(hipGetLastError());
+}
+
+void cg_beta(int m, const double *rr, const double *rzn, const double *target, int *done,
+             double *rz, double *beta) {
+    if (m == 0) return;
+    hipLaunchKernelGGL(k_cg_beta, dim3((m + 63) / 64), dim3(64), 0, stream(), m, rr, rzn, target,
+                       done, rz, beta);
+    BX_HIP(hipGetLastError());
+}
 
 void axpby_cols(int prec, long n, const double *a, const void *x, long ldx, const double *b,
                 void *y, long ldy, const ColMap &cm) {

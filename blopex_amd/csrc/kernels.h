@@ -150,6 +150,14 @@ void zdense_apply(long n, const void *A, long lda, const void *x, long ldx, cons
 // y(:, c) = a[j] x(:, c) + b[j] y(:, c) for c = cm.idx[j]; a, b device FP64 (cm.n)
 void axpby_cols(int prec, long n, const double *a, const void *x, long ldx, const double *b,
                 void *y, long ldy, const ColMap &cm);
+// CG scalars on the device (m columns): a = rz / pq, na = -a for running
+// columns (done[j] == 0), *bad = j + 1 for a running column with pq <= 0
+void cg_alpha(int m, const double *rz, const double *pq, const int *done, double *a, double *na,
+              int *bad);
+// done[j] |= ||r_j|| <= target[j] (rr = ||r||^2); beta = rzn / rz for running
+// columns, 0 for finished; rz <- rzn
+void cg_beta(int m, const double *rr, const double *rzn, const double *target, int *done,
+             double *rz, double *beta);
 // gval[e] = sum_i gl[i] x_i[r] x_i[c] - (w_i[r] x_i[c] + x_i[r] w_i[c]) / 2 over the
 // stored entries e = (r, c) of A; x, w n x k FP64 (leading dim ld), gl device
 void sampled_outer(const hip_CSRMatrix *A, const double *x, const double *w, long ld, int k,

@@ -50,7 +50,7 @@ struct Arena {
 
 constexpr int kNumBlocks64 = 13;
 constexpr int kNumBlocks32 = 10;
-constexpr int kSmall = 10;  // device arrays of kMaxCols doubles
+constexpr int kSmall = 16;  // device arrays of kMaxCols doubles
 
 // blocks of one precision for the CG sweeps
 struct Set {
@@ -69,6 +69,8 @@ struct Ctx {
     double *G;       // k x k (device)
     double *dots;    // 3 kMaxCols (device)
     double *coef_a, *coef_b, *neglam, *ones;  // kMaxCols each (device)
+    double *rz, *target, *beta;                // CG scalars (device)
+    int *done;                                 // kMaxCols + 1 flags (device)
 };
 
 void upload(double *dst, const std::vector<double> &v) {
@@ -100,65 +102,61 @@ void apply_shifted(Ctx &c, Set &s, const void *P, void *Y) {
 
 // Block CG on the projected systems, D(:, cm) from zero, R(:, cm) the right
 // sides (overwritten by the recursive residuals).  Column j stops once its
-// residual norm is at most target[j].  Returns the iterations done, or -1
-// on a breakdown (p' Q' (A - lambda B) p <= 0: the shifted operator is not
-// positive definite on the complement of X).
+// residual norm is at most target[j].  The step lengths and the convergence
+// flags stay on the device; the host reads the flags every kCheck
+// iterations (a finished column takes zero steps meanwhile).  Returns the
+// iterations done, or -1 on a breakdown (p' Q' (A - lambda B) p <= 0: the
+// shifted operator is not positive definite on the complement of X).
+constexpr int kCheck = 8;
 int block_cg(Ctx &c, Set &s, const std::vector<double> &target, int maxit) {
     const int m = c.cm.n;
     clear_cols(s.prec, c.n, s.D, c.n, c.cm);
     precond_project(c, s, s.R, s.Z);
-    diagdot_cols(s.prec, c.n, s.R, c.n, c.cm, s.Z, c.n, c.cm, c.dots);
-    diagdot_cols(s.prec, c.n, s.R, c.n, c.cm, s.R, c.n, c.cm, c.dots + kMaxCols);
-    std::vector<double> rz(m), h(2 * m), a(m), na(m), beta(m);
-    download_sync(rz.data(), c.dots, m * sizeof(double));
-    download_sync(h.data(), c.dots + kMaxCols, m * sizeof(double));
+    diagdot_cols(s.prec, c.n, s.R, c.n, c.cm, s.Z, c.n, c.cm, c.rz);
+    diagdot_cols(s.prec, c.n, s.R, c.n, c.cm, s.R, c.n, c.cm, c.dots);
     copy_cols(s.prec, c.n, s.Z, c.n, c.cm, s.P, c.n, c.cm);
-    std::vector<char> done(m, 0);
+    std::vector<double> h(m);
+    download_sync(h.data(), c.dots, m * sizeof(double));
+    std::vector<int> done(m + 1, 0);
     bool all = true;
     for (int j = 0; j < m; j++) {
         done[j] = std::sqrt(std::max(h[j], 0.0)) <= target[j];
         all = all && done[j];
     }
-    int it = 0;
-    while (it < maxit && !all) {
+    if (all) return 0;
+    upload(c.target, target);
+    upload_async(c.done, done.data(), (m + 1) * sizeof(int));  // done[m] is the breakdown flag
+    int *bad = c.done + m;
+    auto body = [&]() {
         apply_shifted(c, s, s.P, s.Q);
         diagdot_cols(s.prec, c.n, s.P, c.n, c.cm, s.Q, c.n, c.cm, c.dots);
-        download_sync(h.data(), c.dots, m * sizeof(double));
-        for (int j = 0; j < m; j++) {
-            if (done[j]) {
-                a[j] = na[j] = 0.0;
-                continue;
-            }
-            if (!(h[j] > 0.0) || !std::isfinite(rz[j])) {
-                set_error("hip_LobpcgAdjoint: breakdown in column %d at CG iteration %d "
-                          "(p'(A - lambda B)p = %g): lambda_%d is not below the spectrum left "
-                          "after deflating X",
-                          c.cm.idx[j], it, h[j], c.cm.idx[j]);
-                return -1;
-            }
-            a[j] = rz[j] / h[j];
-            na[j] = -a[j];
-        }
-        upload(c.coef_a, a);
-        upload(c.coef_b, na);
+        cg_alpha(m, c.rz, c.dots, c.done, c.coef_a, c.coef_b, bad);
         axpby_cols(s.prec, c.n, c.coef_a, s.P, c.n, c.ones, s.D, c.n, c.cm);
         axpby_cols(s.prec, c.n, c.coef_b, s.Q, c.n, c.ones, s.R, c.n, c.cm);
         diagdot_cols(s.prec, c.n, s.R, c.n, c.cm, s.R, c.n, c.cm, c.dots);
         precond_project(c, s, s.R, s.Z);
         diagdot_cols(s.prec, c.n, s.R, c.n, c.cm, s.Z, c.n, c.cm, c.dots + kMaxCols);
-        download_sync(h.data(), c.dots, m * sizeof(double));
-        download_sync(h.data() + m, c.dots + kMaxCols, m * sizeof(double));
+        cg_beta(m, c.dots, c.dots + kMaxCols, c.target, c.done, c.rz, c.beta);
+        axpby_cols(s.prec, c.n, c.ones, s.Z, c.n, c.beta, s.P, c.n, c.cm);
+    };
+    int it = 0;
+    while (it < maxit) {
+        body();
         it++;
-        all = true;
-        for (int j = 0; j < m; j++) {
-            if (!done[j] && std::sqrt(std::max(h[j], 0.0)) <= target[j]) done[j] = 1;
-            beta[j] = done[j] ? 0.0 : h[m + j] / rz[j];
-            rz[j] = h[m + j];
-            all = all && done[j];
+        if (it % kCheck == 0 || it >= maxit) {
+            download_sync(done.data(), c.done, (m + 1) * sizeof(int));
+            if (done[m]) {
+                const int j = done[m] - 1;
+                set_error("hip_LobpcgAdjoint: breakdown in column %d by CG iteration %d: "
+                          "p'(A - lambda B)p <= 0, lambda_%d is not below the spectrum left "
+                          "after deflating X",
+                          c.cm.idx[j], it, c.cm.idx[j]);
+                return -1;
+            }
+            all = true;
+            for (int j = 0; j < m; j++) all = all && done[j];
+            if (all) break;
         }
-        if (all) break;
-        upload(c.coef_b, beta);
-        axpby_cols(s.prec, c.n, c.ones, s.Z, c.n, c.coef_b, s.P, c.n, c.cm);
     }
     return it;
 }
@@ -238,6 +236,10 @@ int hip_LobpcgAdjoint(const hip_CSRMatrix *A, const hip_DiagMatrix *B, const hip
     c.coef_b = (double *)ar.take(kMaxCols * 8);
     c.neglam = (double *)ar.take(kMaxCols * 8);
     c.ones = (double *)ar.take(kMaxCols * 8);
+    c.rz = (double *)ar.take(kMaxCols * 8);
+    c.target = (double *)ar.take(kMaxCols * 8);
+    c.beta = (double *)ar.take(kMaxCols * 8);
+    c.done = (int *)ar.take((kMaxCols + 1) * 8);
     double *gl_d = (double *)ar.take(kMaxCols * 8);
     double *lam_d = (double *)ar.take(kMaxCols * 8);
 
