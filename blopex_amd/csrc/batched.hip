@@ -664,7 +664,7 @@ void hip_BatchSettingsDefault(hip_BatchSettings *s) {
     s->graph = 1;
     s->guard = 2;
     s->mode = HIPLOBPCG_FP64;
-    s->switch_factor = 100.0;
+    s->switch_factor = 1000.0;
     s->autotune = 1;
 }
 

@@ -89,7 +89,7 @@ def test_batched_argument_errors():
 
 
 def test_batched_mixed_precision():
-    """MIXED: the blocks in FP32 (FP64 Gram sums and Rayleigh-Ritz) to 100 x tol,
+    """MIXED: the blocks in FP32 (FP64 Gram sums and Rayleigh-Ritz) to 1000 x tol,
     then FP64 from the FP32 blocks: the same eigenpairs as the FP64 solve
     (1e-9) and the residuals of the returned pairs at the tolerance."""
     say("batched solve, mixed precision (32 problems of the workload's generator)")
