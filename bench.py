@@ -225,6 +225,8 @@ def main():
     ap.add_argument("--batched-mode", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the differentiation workload (64^3 forward solve + adjoint)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run the differentiation workload alone and print its object")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -237,6 +239,9 @@ def main():
 
     lib = L.load()
     lib.blopex_amd_init()
+    if args.adjoint_only:
+        print(json.dumps(run_adjoint()), flush=True)
+        return
     if args.batched_only:
         print(json.dumps(run_batched(args.batched_count, not args.batched_no_graph,
                                      mode=args.batched_mode, compare_fp64=False)), flush=True)

@@ -141,16 +141,30 @@ __global__ void __launch_bounds__(kThreads) k_gram(long n, const T *x, long ldx,
     part[((size_t)blockIdx.x * gridDim.y + t) * (kGT * kGT) + threadIdx.x] = acc;
 }
 
-__global__ void k_gram_reduce(const double *part, int nparts, int ntiles, int ntx, int h, int w,
-                              double *g, long ldg) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= ntiles * kGT * kGT) return;
+// Sums the nparts partial tiles: kRE consecutive entries per workgroup, one
+// per lane (coalesced loads), each entry's partials split over
+// kThreads / kRE lanes and the slices summed in LDS.  (One thread per entry
+// would sum a thousand partials serially when the output is a single tile.)
+constexpr int kRE = 16;
+static_assert(kThreads % kRE == 0, "reduction slices");
+
+__global__ void __launch_bounds__(kThreads) k_gram_reduce(const double *part, int nparts,
+                                                          int ntiles, int ntx, int h, int w,
+                                                          double *g, long ldg) {
+    __shared__ double red[kThreads / kRE][kRE + 1];
+    const int le = threadIdx.x % kRE, sl = threadIdx.x / kRE;
+    const int total = ntiles * kGT * kGT;
+    const int e = blockIdx.x * kRE + le;
+    double s = 0.0;
+    if (e < total)
+        for (int p = sl; p < nparts; p += kThreads / kRE) s += part[(size_t)p * total + e];
+    red[sl][le] = s;
+    __syncthreads();
+    if (sl != 0 || e >= total) return;
+    for (int q = 1; q < kThreads / kRE; q++) s += red[q][le];
     const int t = e / (kGT * kGT), l = e % (kGT * kGT);
     const int i = (t % ntx) * kGT + l % kGT, j = (t / ntx) * kGT + l / kGT;
-    if (i >= h || j >= w) return;
-    double s = 0.0;
-    for (int p = 0; p < nparts; p++) s += part[(size_t)p * ntiles * kGT * kGT + e];
-    g[i + (size_t)j * ldg] = s;
+    if (i < h && j < w) g[i + (size_t)j * ldg] = s;
 }
 
 template <typename T>
@@ -410,8 +424,8 @@ void gram_cols(int prec, long n, const void *x, long ldx, const ColMap &xm, cons
                            ldx, xm, (const double *)y, ldy, ym, ntx, part);
     BX_HIP(hipGetLastError());
     const int total = ntiles * kGT * kGT;
-    hipLaunchKernelGGL(k_gram_reduce, dim3((total + 255) / 256), dim3(256), 0, stream(), part,
-                       (int)nparts, ntiles, ntx, xm.n, ym.n, g, ldg);
+    hipLaunchKernelGGL(k_gram_reduce, dim3((total + kRE - 1) / kRE), dim3(kThreads), 0, stream(),
+                       part, (int)nparts, ntiles, ntx, xm.n, ym.n, g, ldg);
     BX_HIP(hipGetLastError());
 }
 
