@@ -84,11 +84,15 @@ __global__ void k_cg_alpha(int m, const double *rz, const double *pq, const int 
 
 // after the residual update: a column finishes once ||r|| <= target; beta =
 // rz_new / rz for the running ones (0 for the finished), rz <- rz_new
+// done[m + 1] counts the iterations in which some column was still running
+// (one workgroup: m <= kMaxCols <= blockDim.x)
 __global__ void k_cg_beta(int m, const double *rr, const double *rzn, const double *target,
                           int *done, double *rz, double *beta) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = threadIdx.x;
+    const bool running = j < m && !done[j];
+    if (__syncthreads_or(running) && j == 0) done[m + 1]++;
     if (j >= m) return;
-    if (!done[j] && sqrt(fmax(rr[j], 0.0)) <= target[j]) done[j] = 1;
+    if (running && sqrt(fmax(rr[j], 0.0)) <= target[j]) done[j] = 1;
     beta[j] = done[j] ? 0.0 : rzn[j] / rz[j];
     rz[j] = rzn[j];
 }
@@ -106,8 +110,9 @@ void cg_alpha(int m, const double *rz, const double *pq, const int *done, double
 void cg_beta(int m, const double *rr, const double *rzn, const double *target, int *done,
              double *rz, double *beta) {
     if (m == 0) return;
-    hipLaunchKernelGGL(k_cg_beta, dim3((m + 63) / 64), dim3(64), 0, stream(), m, rr, rzn, target,
-                       done, rz, beta);
+    BX_REQUIRE(m <= kMaxCols, "cg_beta: %d columns", m);
+    hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(kMaxCols), 0, stream(), m, rr, rzn, target, done,
+                       rz, beta);
     BX_HIP(hipGetLastError());
 }
 

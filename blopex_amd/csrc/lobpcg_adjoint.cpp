@@ -103,10 +103,13 @@ void apply_shifted(Ctx &c, Set &s, const void *P, void *Y) {
 // Block CG on the projected systems, D(:, cm) from zero, R(:, cm) the right
 // sides (overwritten by the recursive residuals).  Column j stops once its
 // residual norm is at most target[j].  The step lengths and the convergence
-// flags stay on the device; the host reads the flags every kCheck
-// iterations (a finished column takes zero steps meanwhile).  Returns the
-// iterations done, or -1 on a breakdown (p' Q' (A - lambda B) p <= 0: the
-// shifted operator is not positive definite on the complement of X).
+// flags stay on the device; the host reads the flags (and the residual
+// norms) at most kCheck iterations apart, sooner when the residuals' rate
+// of decrease says the last column finishes earlier.  A finished column
+// takes zero steps meanwhile, and the count returned is that of the
+// iterations in which some column still ran (kept on the device).  Returns
+// -1 on a breakdown (p' Q' (A - lambda B) p <= 0: the shifted operator is
+// not positive definite on the complement of X).
 constexpr int kCheck = 8;
 int block_cg(Ctx &c, Set &s, const std::vector<double> &target, int maxit) {
     const int m = c.cm.n;
@@ -125,8 +128,21 @@ int block_cg(Ctx &c, Set &s, const std::vector<double> &target, int maxit) {
     }
     if (all) return 0;
     upload(c.target, target);
-    upload_async(c.done, done.data(), (m + 1) * sizeof(int));  // done[m] is the breakdown flag
+    // done[m] is the breakdown flag, done[m + 1] the count of running iterations
+    done.push_back(0);
+    upload_async(c.done, done.data(), (m + 2) * sizeof(int));
     int *bad = c.done + m;
+    // worst log(||r_j|| / target_j) over the running columns at the last check
+    auto worst_log = [&](const std::vector<double> &rr) {
+        double w = 0.0;
+        for (int j = 0; j < m; j++)
+            if (!done[j])
+                w = std::max(w, 0.5 * std::log(std::max(rr[j], 1e-300)) -
+                                    std::log(std::max(target[j], 1e-300)));
+        return w;
+    };
+    double last_worst = worst_log(h);
+    int next_check = kCheck, last_check = 0;
     auto body = [&]() {
         apply_shifted(c, s, s.P, s.Q);
         diagdot_cols(s.prec, c.n, s.P, c.n, c.cm, s.Q, c.n, c.cm, c.dots);
@@ -143,8 +159,9 @@ int block_cg(Ctx &c, Set &s, const std::vector<double> &target, int maxit) {
     while (it < maxit) {
         body();
         it++;
-        if (it % kCheck == 0 || it >= maxit) {
-            download_sync(done.data(), c.done, (m + 1) * sizeof(int));
+        if (it >= next_check || it >= maxit) {
+            download_sync(done.data(), c.done, (m + 2) * sizeof(int));
+            download_sync(h.data(), c.dots, m * sizeof(double));
             if (done[m]) {
                 const int j = done[m] - 1;
                 set_error("hip_LobpcgAdjoint: breakdown in column %d by CG iteration %d: "
@@ -156,9 +173,17 @@ int block_cg(Ctx &c, Set &s, const std::vector<double> &target, int maxit) {
             all = true;
             for (int j = 0; j < m; j++) all = all && done[j];
             if (all) break;
+            // iterations the worst column still needs at the rate since the last check
+            const double worst = worst_log(h), rate = (last_worst - worst) / (it - last_check);
+            int step = kCheck;
+            if (rate > 0.0) step = std::max(1, std::min(kCheck, (int)std::ceil(worst / rate)));
+            last_worst = worst;
+            last_check = it;
+            next_check = it + step;
         }
     }
-    return it;
+    download_sync(done.data(), c.done, (m + 2) * sizeof(int));
+    return done[m + 1];
 }
 
 bool check_block(const hip_Multi_Vector *v, long n, int k, const char *name) {

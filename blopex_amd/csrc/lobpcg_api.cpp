@@ -21,6 +21,7 @@ int device_lobpcg(mv_MultiVectorPtr X, void *Ad, void (*opA)(void *, void *, voi
                   void (*opT)(void *, void *, void *), mv_MultiVectorPtr Y, lobpcg_Tolerance tol,
                   int maxit, int verb, int *iters, double *lam, double *lamh, BlopexInt lamh_gh,
                   double *res, double *resh, BlopexInt resh_gh);
+bool device_lobpcg_fits(int k);
 }
 
 extern "C" {
@@ -38,8 +39,20 @@ int lobpcg_solve_double(mv_MultiVectorPtr X, void *Ad, void (*opA)(void *, void 
             *iters = 0;
             return WRONG_BLOCK_SIZE;
         }
-        return bx::device_lobpcg(X, Ad, opA, Bd, opB, Td, opT, Y, tol, maxit, verb, iters, lam,
-                                 lamh, lamh_gh, res, resh, resh_gh);
+        // blocks wider than the device solver's Rayleigh-Ritz takes run the
+        // generic algorithm over the same device blocks when the caller
+        // passes dpotrf/dsygv, as the reference accepts any block size
+        const int k = mv_MultiVectorWidth(X);
+        if (k > bx::kMaxCols) {
+            bx::set_error("lobpcg_solve_double: %d columns (device blocks take at most %d)", k,
+                          bx::kMaxCols);
+            if (verb) fprintf(stderr, "Error in LOBPCG:\n%s\n", blopex_amd_last_error());
+            *iters = 0;
+            return WRONG_BLOCK_SIZE;
+        }
+        if (bx::device_lobpcg_fits(k) || !blap.dpotrf || !blap.dsygv)
+            return bx::device_lobpcg(X, Ad, opA, Bd, opB, Td, opT, Y, tol, maxit, verb, iters,
+                                     lam, lamh, lamh_gh, res, resh, resh_gh);
     }
     if (!blap.dpotrf || !blap.dsygv) {
         bx::set_error("lobpcg_solve_double: dpotrf/dsygv are required for a host backend");

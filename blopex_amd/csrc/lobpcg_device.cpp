@@ -466,7 +466,14 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
                 for (int t = h - W; t < h; t++) m2 = fmin(m2, gm_hist[t]);
                 stalled = m2 > 0.98 * m1;
             }
-            if (nr == 0 || mr < set.switch_factor * tol_.absolute || stalled) {
+            // near the tolerance: every residual below switch_factor times
+            // its own threshold (the mask's |lambda_i| rtol + atol)
+            bool near = true;
+            for (int i = 0; i < k && near; i++)
+                near = out_.res[i] < set.switch_factor *
+                                         (fabs(out_.lam[i]) * tol_.relative + tol_.absolute);
+            (void)mr;
+            if (nr == 0 || near || stalled) {
                 *switched = true;
                 fp32_exit_ = stalled ? 1 : 0;
                 break;
@@ -708,6 +715,8 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
 
 }  // namespace
 
+bool device_lobpcg_fits(int k) { return k >= 1 && 3 * k <= kMaxRR; }
+
 int device_lobpcg(mv_MultiVectorPtr X, void *Ad, Op opA, void *Bd, Op opB, void *Td, Op opT,
                   mv_MultiVectorPtr Y, lobpcg_Tolerance tol, int maxit, int verb, int *iters,
                   double *lam, double *lamh, BlopexInt lamh_gh, double *res, double *resh,
@@ -720,16 +729,33 @@ int device_lobpcg(mv_MultiVectorPtr X, void *Ad, Op opA, void *Bd, Op opB, void 
         if (verb) fprintf(stderr, "Error in LOBPCG:\nThe bloc size is wrong.\n");
         return WRONG_BLOCK_SIZE;
     }
-    BX_REQUIRE(x->precision == HIPMV_FP64, "lobpcg_solve_double: X must be an FP64 block");
-    BX_REQUIRE(3 * k <= kMaxRR, "device solver: block size %d above %d", k, kMaxRR / 3);
+    if (x->precision != HIPMV_FP64) {
+        set_error("lobpcg_solve_double: X must be an FP64 block");
+        if (verb) fprintf(stderr, "Error in LOBPCG:\n%s\n", blopex_amd_last_error());
+        return WRONG_BLOCK_SIZE;
+    }
+    if (3 * k > kMaxRR) {
+        set_error("device solver: block size %d above %d", k, kMaxRR / 3);
+        if (verb) fprintf(stderr, "Error in LOBPCG:\n%s\n", blopex_amd_last_error());
+        return WRONG_BLOCK_SIZE;
+    }
     hip_Multi_Vector *y = nullptr;
     int ny = 0;
     if (Y) {
-        BX_REQUIRE(blopex_amd_is_hip_interpreter(blopex_amd_mv_interpreter(Y)),
-                   "device solver: constraints must be a hip_Multi_Vector");
+        if (!blopex_amd_is_hip_interpreter(blopex_amd_mv_interpreter(Y))) {
+            set_error("device solver: constraints must be a hip_Multi_Vector");
+            if (verb) fprintf(stderr, "Error in LOBPCG:\n%s\n", blopex_amd_last_error());
+            return WRONG_CONSTRAINTS;
+        }
         y = (hip_Multi_Vector *)mv_MultiVectorGetData(Y);
         ny = y->num_vectors;
-        BX_REQUIRE(y->size == x->size, "constraints have %ld rows, X %ld", y->size, x->size);
+        if (y->size != x->size || y->precision != HIPMV_FP64 || ny > kMaxCols) {
+            set_error("device solver: constraints must be an FP64 block of %ld rows and at most "
+                      "%d columns (got %ld x %d)",
+                      x->size, kMaxCols, y->size, ny);
+            if (verb) fprintf(stderr, "Error in LOBPCG:\n%s\n", blopex_amd_last_error());
+            return WRONG_CONSTRAINTS;
+        }
     }
     const hip_LobpcgSettings set = x->settings;
     memset(&x->stats, 0, sizeof(x->stats));

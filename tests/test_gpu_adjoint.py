@@ -168,3 +168,53 @@ def test_adjoint_64cube_against_finite_differences(cube64, mode):
         assert abs(fd - dd) <= 1e-3 * abs(fd)
     if "fp64" in memo and "mixed" in memo:
         assert rel(memo["mixed"], memo["fp64"]) < 1e-6
+
+
+def test_adjoint_breakdown_is_reported():
+    """X skips the lowest eigenvector: on the complement of X the shifted
+    operator A - lambda_i B is indefinite (lambda_1 - lambda_i < 0), the CG
+    meets p'(A - lambda B)p <= 0 and the call returns -1 with a message."""
+    say("adjoint breakdown: X = eigenvectors 2..4 of an 8^3 operator")
+    dims = (8, 8, 8)
+    A = pb.diffusion_stencil(dims, pb.lognormal_coefficients(512, 1.0, 8))
+    mu, V = np.linalg.eigh(A.to_scipy().toarray())
+    X, lam = V[:, 1:4], mu[1:4]
+    GX = np.random.default_rng(3).standard_normal((512, 3))
+    with pytest.raises(RuntimeError, match="breakdown in column"):
+        bl.adjoint(A, X, lam, np.ones(3), GX, mode="fp64", tol=1e-12, maxit=2000)
+
+
+@pytest.mark.parametrize("maxit", [5, 11])
+def test_adjoint_maxit_exit(maxit):
+    """maxit runs out before the tolerance (at iterations that are not
+    multiples of the host's check interval): flag 1 and exactly maxit CG
+    iterations reported."""
+    say(f"adjoint maxit exit, maxit {maxit}")
+    dims = (10, 10, 10)
+    A = pb.diffusion_stencil(dims, pb.lognormal_coefficients(1000, 1.0, 9))
+    mu, V = np.linalg.eigh(A.to_scipy().toarray())
+    k = 4
+    GX = np.random.default_rng(4).standard_normal((1000, k))
+    r = bl.adjoint(A, V[:, :k], mu[:k], np.ones(k), GX, mode="fp64", tol=1e-14, maxit=maxit)
+    say(f"  stats {r.stats}")
+    assert r.flag == 1
+    assert r.stats["iterations"] == maxit
+
+
+def test_adjoint_counts_only_running_iterations():
+    """The iteration count is that of the iterations in which some column
+    still ran: a converged solve reports the same count whatever the host's
+    check interval, and fewer than the check interval past the last column's
+    finish (the count of a run with a cap just below it fails to converge)."""
+    say("adjoint iteration count, 10^3 FP64")
+    dims = (10, 10, 10)
+    A = pb.diffusion_stencil(dims, pb.lognormal_coefficients(1000, 1.0, 9))
+    mu, V = np.linalg.eigh(A.to_scipy().toarray())
+    k = 4
+    GX = np.random.default_rng(4).standard_normal((1000, k))
+    r = bl.adjoint(A, V[:, :k], mu[:k], np.ones(k), GX, mode="fp64", tol=1e-10, maxit=2000)
+    assert r.flag == 0
+    it = r.stats["iterations"]
+    r2 = bl.adjoint(A, V[:, :k], mu[:k], np.ones(k), GX, mode="fp64", tol=1e-10, maxit=it - 1)
+    say(f"  converged in {it}; capped at {it - 1}: flag {r2.flag}")
+    assert r2.flag == 1

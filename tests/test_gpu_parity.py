@@ -407,6 +407,83 @@ def test_device_exit_codes():
     assert (r.flag, r.iterations) == (-1, 2)
 
 
+def test_wide_block_returns_a_flag_or_runs_generic():
+    """A block wider than the device solver's Rayleigh-Ritz takes (3k > 192)
+    returns WRONG_BLOCK_SIZE (2) with a message when no dsygv is passed, and
+    the process lives on; with scipy's dpotrf/dsygv it runs the generic
+    algorithm over the same device blocks and finds the diag(1..n) spectrum."""
+    say("block of 65 columns on the device")
+    n, k = 400, 65
+    A = pb.CSR(np.arange(n + 1), np.arange(n), np.arange(1, n + 1, dtype=float))
+    r = bl.lobpcg(None, A, k=k, maxit=5)
+    assert r.flag == 2 and r.iterations == 0
+    assert "block size" in L.last_error()
+    lib = L.load()
+    dA = bl.DeviceCSR(A)
+    X = bl.DeviceBlock(n, k)
+    X.set_random(1)
+    xx = _wrap(bl.interpreter(), X)
+    potrf, sygv = po.lapack_pointers()
+    lam, res = np.zeros(k), np.zeros(k)
+    it = ctypes.c_int(0)
+    flag = lib.lobpcg_solve_double(xx, ctypes.cast(dA.ptr, ctypes.c_void_p), dA.op, None, None,
+                                   None, None, None, L.BlasLapack(potrf, sygv),
+                                   L.Tolerance(1e-6, 1e-50), 500, 0, ctypes.byref(it),
+                                   lam.ctypes.data, None, k, res.ctypes.data, None, k)
+    lib.mv_MultiVectorDestroy(xx)
+    X.close()
+    dA.close()
+    say(f"  generic path: flag {flag}, {it.value} iterations")
+    assert flag == 0
+    np.testing.assert_allclose(np.sort(lam), np.arange(1, k + 1), rtol=1e-8)
+
+
+def test_device_rejects_fp32_x_and_bad_constraints():
+    """Argument errors return the reference's codes instead of aborting."""
+    say("device argument errors")
+    A = pb.laplacian_7pt(6, 6, 6)
+    lib = L.load()
+    dA = bl.DeviceCSR(A)
+    for prec, ycols, want in [(L.HIPMV_FP32, 0, 2), (L.HIPMV_FP64, 1, 3)]:
+        X = bl.DeviceBlock(A.n, 3, prec)
+        X.set_random(1)
+        ii = bl.interpreter()
+        xx = _wrap(ii, X)
+        yy, Yb = None, None
+        if ycols:
+            Yb = bl.DeviceBlock(A.n - 1, ycols)  # wrong number of rows
+            yy = _wrap(ii, Yb)
+        lam, res = np.zeros(3), np.zeros(3)
+        it = ctypes.c_int(0)
+        flag = lib.lobpcg_solve_double(xx, ctypes.cast(dA.ptr, ctypes.c_void_p), dA.op, None,
+                                       None, None, None, yy, L.BlasLapack(),
+                                       L.Tolerance(1e-6, 1e-50), 10, 0, ctypes.byref(it),
+                                       lam.ctypes.data, None, 3, res.ctypes.data, None, 3)
+        lib.mv_MultiVectorDestroy(xx)
+        if yy:
+            lib.mv_MultiVectorDestroy(yy)
+            Yb.close()
+        X.close()
+        assert flag == want, (prec, ycols, flag, L.last_error())
+    dA.close()
+
+
+def test_mixed_relative_tolerance_only():
+    """MIXED with atol = 0 and a relative tolerance: the FP32 phase leaves on
+    the per-column threshold (|lambda_i| rtol) and the FP64 phase meets it."""
+    say("mixed, atol 0, rtol 1e-5 (12^3 lognormal, k=6, jacobi)")
+    A = pb.laplacian_7pt(12, 12, 12, sigma=1.0, seed=5)
+    X0 = po.drand48_block(A.n, 6, 1)
+    r = bl.lobpcg(X0, A, T="jacobi", tol=0.0, rtol=1e-5, maxit=500, mode="mixed")
+    say(f"  flag {r.flag} it {r.iterations} fp32 {r.stats['iterations_fp32']} "
+        f"exit {r.stats['fp32_exit']}")
+    assert r.flag == 0
+    assert r.stats["fp32_exit"] == 0 and r.stats["iterations_fp32"] > 0
+    assert np.all(r.res <= np.abs(r.lam) * 1e-5 + np.finfo(float).eps)
+    ref = np.linalg.eigvalsh(A.to_scipy().toarray())[:6]
+    assert relerr(np.sort(r.lam), ref) < 1e-8
+
+
 def test_device_x0_from_seed_matches_host_x0():
     say("device X0 (seeded) solve equals host-X0 solve")
     A = pb.laplacian_7pt(10, 9, 8)
