@@ -29,7 +29,8 @@ class BlasLapack(ctypes.Structure):
 class Settings(ctypes.Structure):
     _fields_ = [("mode", c_int), ("switch_factor", c_double), ("stall_iterations", c_int),
                 ("refine_iterations", c_int), ("profile", c_int), ("refresh_interval", c_int),
-                ("final_rr", c_int)]
+                ("final_rr", c_int), ("guard_columns", c_int), ("eig_rtol", c_double),
+                ("cond_restart", c_int), ("size_guard", c_int), ("guard_seed", c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -37,7 +38,8 @@ class Stats(ctypes.Structure):
                 ("refine_iterations", c_int), ("exit_flag", c_int), ("seconds_small_dense", c_double),
                 ("seconds_spmm", c_double), ("seconds_gram", c_double), ("seconds_update", c_double),
                 ("seconds_other", c_double), ("spmm_bytes", c_double), ("spmm_calls", c_int),
-                ("fp32_exit", c_int), ("final_rr", c_int)]
+                ("fp32_exit", c_int), ("final_rr", c_int), ("guard_columns", c_int),
+                ("cond_restarts", c_int), ("eig_error_bound", c_double)]
 
 
 class AdjointSettings(ctypes.Structure):
@@ -115,6 +117,7 @@ _SIGS = {
     "hip_Multi_VectorDownload": (c_int, [P(HipMultiVector), c_void_p]),
     "hip_Multi_VectorSetRandomValues": (c_int, [P(HipMultiVector), c_int]),
     "hip_Multi_VectorSetSettings": (c_int, [P(HipMultiVector), P(Settings)]),
+    "hip_LobpcgSettingsDefault": (None, [P(Settings)]),
     "hip_Multi_VectorGetStats": (c_int, [P(HipMultiVector), P(Stats)]),
     "HipSetupInterpreter": (c_int, [P(Interpreter)]),
     "hip_CSRMatrixCreate": (P(CSRMatrix), [c_long, c_long, c_void_p, c_void_p, c_void_p]),

@@ -62,6 +62,7 @@ struct DevStatus {
     int p_info;     // Cholesky info of P'BP (nonzero: P dropped this iteration)
     int gevp_info;  // Rayleigh-Ritz failure (dsygv-style info)
     int pad[3];
+    double condest; // log10 of the pencil's B-part condition estimate + 1 (cond_limit > 0)
 };
 
 // Column maps passed to kernels by value (kernel arguments).

@@ -28,13 +28,22 @@ struct RR2Args {
     int explicit_x;         // 1: X'AX, X'BX from the Gram buffers (FP32 iterates)
     int retry_without_p;    // 1: a pencil that is not positive definite is solved again
                             // without P (p_info = -1); the tridiagonal solver only
+    double cond_limit;      // > 0: a pencil whose B-part condition estimate (log10 of the
+                            // Cholesky pivot spread + 1, into st->condest) exceeds it is
+                            // solved again without P (p_info = -2); tridiagonal solver only
 };
 
-// pencils the one-workgroup tridiagonal solver takes (rr_fast.hip)
+// pencils the one-workgroup tridiagonal solver takes (rr_fast.hip): both
+// pencil matrices in LDS up to 96 (k <= 32); the large variant up to 192
+// (k, a, p <= 64) keeps A_n in global memory and B_n in LDS while it fits
 constexpr int kFastRRN = 96;
 constexpr int kFastRRK = 32;
+constexpr int kBigRRN = 192;
+constexpr int kBigRRK = 64;
 bool rr_fast_fits(int k, int a, int p);
 void rr_fast(const RR2Args &r);
+bool rr_big_fits(int k, int a, int p);
+void rr_big(const RR2Args &r);
 // work words of one pencil of order up to n
 size_t rr_fast_work_words(int n);
 // The same solver over a batch: workgroup i takes problem act[i] (i < *nact)
@@ -52,6 +61,7 @@ void rr_fast_batched(const RR2Args &base, const int *act, const int *nact, const
 // st->r_info / abort, st->p_info (P dropped) and st->gevp_info.
 void rr_solve_implicit(const double *ga, const double *gb, int lda, int k, int a, int p,
                        const double *lam_in, double *lam_out, double *coef, int ldc,
-                       DevStatus *st, int explicit_x, int retry_without_p = 0);
+                       DevStatus *st, int explicit_x, int retry_without_p = 0,
+                       double cond_limit = 0.0);
 
 }  // namespace bx

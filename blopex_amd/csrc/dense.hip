@@ -641,7 +641,8 @@ void rr_solve(const double *ga, const double *gb, int lda, int k, int a, int p, 
 
 void rr_solve_implicit(const double *ga, const double *gb, int lda, int k, int a, int p,
                        const double *lam_in, double *lam_out, double *coef, int ldc,
-                       DevStatus *st, int explicit_x, int retry_without_p) {
+                       DevStatus *st, int explicit_x, int retry_without_p,
+                       double cond_limit) {
     const int nmax = k + a + p;
     BX_REQUIRE(nmax <= kMaxRR, "rr_solve_implicit: pencil of order %d (max %d)", nmax, kMaxRR);
     const int nE = (nmax + 1) / 2 * 2;
@@ -649,13 +650,17 @@ void rr_solve_implicit(const double *ga, const double *gb, int lda, int k, int a
                          (size_t)nmax * k + 64;
     double *work = (double *)scratch(7, words * sizeof(double));
     RR2Args r{ga, gb, lda, k, a, p, lam_in, lam_out, coef, ldc, work, st, explicit_x,
-              retry_without_p};
+              retry_without_p, cond_limit};
     static const bool force_jacobi = [] {
         const char *e = getenv("BLOPEX_AMD_RR");
         return e && e[0] == 'j';
     }();
     if (rr_fast_fits(k, a, p) && !force_jacobi && !g_force_jacobi) {
         rr_fast(r);
+        return;
+    }
+    if (rr_big_fits(k, a, p) && !force_jacobi && !g_force_jacobi) {
+        rr_big(r);
         return;
     }
     static bool attr_set = false;

@@ -26,6 +26,11 @@ hip_LobpcgSettings default_settings() {
     s.profile = 0;
     s.refresh_interval = 20;
     s.final_rr = 0;
+    s.guard_columns = 0;
+    s.eig_rtol = 0.0;
+    s.cond_restart = 0;
+    s.size_guard = 0;
+    s.guard_seed = 7;
     return s;
 }
 
@@ -145,10 +150,19 @@ int hip_Multi_VectorSetRandomValues(hip_Multi_Vector *x, BlopexInt seed) {
     return 0;
 }
 
+void hip_LobpcgSettingsDefault(hip_LobpcgSettings *s) {
+    if (s) *s = default_settings();
+}
+
 int hip_Multi_VectorSetSettings(hip_Multi_Vector *x, const hip_LobpcgSettings *s) {
     if (!x || !s) return 1;
     if (s->mode != HIPLOBPCG_FP64 && s->mode != HIPLOBPCG_MIXED) {
         set_error("unknown solver mode %d", s->mode);
+        return 1;
+    }
+    if (s->guard_columns < 0 || !(s->eig_rtol >= 0.0)) {
+        set_error("guard_columns must be >= 0 and eig_rtol >= 0 (got %d, %g)", s->guard_columns,
+                  s->eig_rtol);
         return 1;
     }
     x->settings = *s;

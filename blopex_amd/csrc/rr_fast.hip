@@ -1,5 +1,9 @@
-// Rayleigh-Ritz step of LOBPCG for pencils up to 96 x 96 (k <= 32), one
-// 1024-thread workgroup with both pencil matrices in LDS.
+// Rayleigh-Ritz step of LOBPCG on one 1024-thread workgroup: pencils up to
+// 96 x 96 (k <= 32) with both pencil matrices in LDS, and pencils up to
+// 192 x 192 (k <= 64: guard columns, wide blocks) with A_n in global memory
+// (L2-resident; it is touched only by the tile products, which are parallel)
+// and B_n in LDS while it fits (the sequential Cholesky and triangular
+// inverse run on B_n), else in global memory too.
 //
 // Input: the raw Gram blocks of [X R P] (lower block triangle and diagonal
 // blocks, as the Gram kernel fills them).  The reference B-orthonormalises R
@@ -39,7 +43,18 @@ constexpr int kT = 1024;     // threads
 constexpr int kW = kT / 64;  // waves
 constexpr int kN = kFastRRN;
 constexpr int kK = kFastRRK;
+constexpr int kBigN = kBigRRN;
+constexpr int kBigK = kBigRRK;
 constexpr double kEps = 2.220446049250313e-16;
+
+// where the two pencil matrices live
+constexpr int kBothLds = 0, kALdsB = 1, kBothGlobal = 2;  // kALdsB: A global, B in LDS
+
+// static LDS of the body (the small arrays), for sizing the dynamic part
+constexpr size_t static_lds_bytes(int nm, int km) { return (size_t)(11 * nm + 2 * km + 4) * 8; }
+// leading dimension of the pencil arrays: 3 (mod 32), which spreads rows,
+// columns and 8 x 8 blocks of doubles over the LDS banks
+__host__ __device__ inline int pencil_ld(int n0) { return n0 + ((3 - n0) & 31); }
 
 __device__ __forceinline__ double wmin(double v) {
     for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
@@ -132,9 +147,10 @@ __device__ int sturm_count(const double *d, const double *e2, int n, double x) {
 // to column j of C in LDS, the next column and the next diagonal entry to
 // LDS.  Rows and columns up to j have v = w = 0 and do not change.  Every
 // thread of the workgroup calls this (two barriers per column).
-template <int HW, int HQ, int HB>
+template <int NM, int HW, int HQ, int HB>
 __device__ __forceinline__ void householder(int n, int L, double *Am, double *col0, double *col1,
                                             double *pvec, double *dT, double *eT, double *tauT) {
+    constexpr int XR = (NM + 63) / 64;  // entries of a column vector per lane
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const bool hw = wave < HW;
     const int hc = lane & 15, hr = lane >> 4;
@@ -161,7 +177,7 @@ __device__ __forceinline__ void householder(int n, int L, double *Am, double *co
                 for (int q = 0; q < HQ; q++) cv[q] = M[q][b];
             }
     };
-    for (int i = tid; i < kN; i += kT) {
+    for (int i = tid; i < NM; i += kT) {
         col0[i] = i >= 1 && i < n ? Am[i] : 0.0;
         col1[i] = 0.0;
     }
@@ -171,11 +187,17 @@ __device__ __forceinline__ void householder(int n, int L, double *Am, double *co
         const double *cur = (j & 1) ? col1 : col0;
         double *nxt = (j & 1) ? col0 : col1;
         const int o = j + 1;
-        double vr[HQ], vc[HB], pr[HQ], tau = 0.0, alpha = 0.0, xa = 0.0, xb = 0.0;
+        double vr[HQ], vc[HB], pr[HQ], tau = 0.0, alpha = 0.0, xs[XR];
+#pragma unroll
+        for (int u = 0; u < XR; u++) xs[u] = 0.0;
         if (hw) {
-            xa = cur[lane];
-            xb = lane < kN - 64 ? cur[lane + 64] : 0.0;
-            const double s = wave_sum(xa * xa + xb * xb);
+            double ss = 0.0;
+#pragma unroll
+            for (int u = 0; u < XR; u++) {
+                xs[u] = lane + 64 * u < NM ? cur[lane + 64 * u] : 0.0;
+                ss = u == 0 ? xs[0] * xs[0] : ss + xs[u] * xs[u];
+            }
+            const double s = wave_sum(ss);
             const double x0 = cur[o];
             alpha = s > 0.0 ? -copysign(sqrt(s), x0) : 0.0;
             const double v0 = x0 - alpha;
@@ -201,16 +223,23 @@ __device__ __forceinline__ void householder(int n, int L, double *Am, double *co
                     nxt[o] = 0.0;
                 }
                 // v into column j of Am (rows j+1..n-1; zeros above are ignored)
-                if (lane < n) Am[lane + (size_t)j * L] = xa - (lane == o ? alpha : 0.0);
-                if (lane + 64 < n)
-                    Am[lane + 64 + (size_t)j * L] = xb - (lane + 64 == o ? alpha : 0.0);
+#pragma unroll
+                for (int u = 0; u < XR; u++)
+                    if (lane + 64 * u < n)
+                        Am[lane + 64 * u + (size_t)j * L] =
+                            xs[u] - (lane + 64 * u == o ? alpha : 0.0);
             }
         }
         __syncthreads();
         if (hw) {
-            const double pa = pvec[lane], pb = lane < kN - 64 ? pvec[lane + 64] : 0.0;
-            const double K = 0.5 * tau * wave_sum((xa - (lane == o ? alpha : 0.0)) * pa +
-                                                  (xb - (lane + 64 == o ? alpha : 0.0)) * pb);
+            double vp = 0.0;
+#pragma unroll
+            for (int u = 0; u < XR; u++) {
+                const double pu = lane + 64 * u < NM ? pvec[lane + 64 * u] : 0.0;
+                const double t = (xs[u] - (lane + 64 * u == o ? alpha : 0.0)) * pu;
+                vp = u == 0 ? t : vp + t;
+            }
+            const double K = 0.5 * tau * wave_sum(vp);
             double wr[HQ], wc[HB];
 #pragma unroll
             for (int q = 0; q < HQ; q++) wr[q] = pr[q] - K * vr[q];
@@ -251,21 +280,33 @@ __device__ __forceinline__ void householder(int n, int L, double *Am, double *co
     __syncthreads();
 }
 
-__device__ void rr_tri_body(RR2Args r) {
+// NM, KM: largest pencil order and block width; MODE: where A_n and B_n live
+// (gA, gB: global arrays of pencil_ld(n0) x max(n0, 2k) doubles when not in LDS)
+template <int NM, int KM, int MODE>
+__device__ void rr_tri_body(RR2Args r, double *gA, double *gB) {
     extern __shared__ double lds[];
-    __shared__ double dT[kN], eT[kN], dS[kN], e2S[kN], tauT[kN], invu[kN], sdg[kN];
-    __shared__ double pvec[kN], pv[kN], col0[kN], col1[kN];
-    __shared__ double lamk[kK], proj[kK];
-    __shared__ double sc[4];
+    __shared__ double dT[NM], eT[NM], dS[NM], e2S[NM], tauT[NM], invu[NM], sdg[NM];
+    __shared__ double pvec[NM], pv[NM], col0[NM], col1[NM];
+    __shared__ double lamk[KM], proj[KM];
+    __shared__ double sc[4], cpiv[1];
+    // tile slots per wave of the MFMA phases (the largest tile counts at NM, KM)
+    constexpr int tNm = (NM + 15) / 16, tKm = (KM + 15) / 16, tXRm = (2 * KM + 15) / 16;
+    constexpr int S_RF = (tXRm * tKm + tNm * tKm + 3 * tKm * tKm + kW - 1) / kW;
+    constexpr int S_LF = (4 * tKm * tKm + kW - 1) / kW;
+    constexpr int S_SC = (4 * tKm * tKm + kW - 1) / kW;
+    constexpr int S_WX = (2 * tKm * tKm + kW - 1) / kW;
+    constexpr int S_C1 = (tNm * tNm + kW - 1) / kW;
+    constexpr int S_C2 = (tNm * (tNm + 1) / 2 + kW - 1) / kW;
+    constexpr int S_Y = (tNm * tKm + kW - 1) / kW;
+    constexpr int S_CO = (2 * tKm * tKm + kW - 1) / kW;
     if (r.st->abort) return;
     const int tid = threadIdx.x, tx = tid & 31, ty = tid >> 5;
     const int wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lk = lane >> 4;
     const int k = r.k, a = r.a, pq = r.p, n0 = k + a + pq;
-    // leading dimension = 3 (mod 32): rows, columns and 8 x 8 blocks of
-    // doubles spread over the LDS banks
-    const int L = n0 + ((3 - n0) & 31);
-    double *Am = lds, *Bm = lds + (size_t)L * n0;
+    const int L = pencil_ld(n0);
+    double *Am = MODE == kBothLds ? lds : gA;
+    double *Bm = MODE == kBothLds ? lds + (size_t)L * n0 : (MODE == kALdsB ? lds : gB);
     double *WRg = r.work, *WPg = WRg + (size_t)a * a, *Wg = WPg + (size_t)pq * pq;
     const int oR = k, oP = k + a;
     auto blk = [&](int i) { return i < k ? 0 : (i < oP ? 1 : 2); };
@@ -305,23 +346,26 @@ __device__ void rr_tri_body(RR2Args r) {
     int infoR = 0, infoP = 0;
     const int mRP = a > pq ? a : pq;
     for (int j = 0; j < mRP; j++) {
-        const int c = j + 1 + ty, i = j + 1 + tx;
         if (j < a && !infoR) {
             const double d = Bm[(oR + j) * (L + 1)];
             if (!(d > 0.0) || !isfinite(d)) {
                 infoR = j + 1;
-            } else if (c < a && i <= c) {
-                Bm[(oR + i) + (oR + c) * L] -=
-                    Bm[(oR + j) + (oR + i) * L] * (Bm[(oR + j) + (oR + c) * L] * (1.0 / d));
+            } else {
+                for (int c = j + 1 + ty; c < a; c += 32)
+                    for (int i = j + 1 + tx; i <= c; i += 32)
+                        Bm[(oR + i) + (oR + c) * L] -=
+                            Bm[(oR + j) + (oR + i) * L] * (Bm[(oR + j) + (oR + c) * L] * (1.0 / d));
             }
         }
         if (j < pq && !infoP) {
             const double d = Bm[(oP + j) * (L + 1)];
             if (!(d > 0.0) || !isfinite(d)) {
                 infoP = j + 1;
-            } else if (c < pq && i <= c) {
-                Bm[(oP + i) + (oP + c) * L] -=
-                    Bm[(oP + j) + (oP + i) * L] * (Bm[(oP + j) + (oP + c) * L] * (1.0 / d));
+            } else {
+                for (int c = j + 1 + ty; c < pq; c += 32)
+                    for (int i = j + 1 + tx; i <= c; i += 32)
+                        Bm[(oP + i) + (oP + c) * L] -=
+                            Bm[(oP + j) + (oP + i) * L] * (Bm[(oP + j) + (oP + c) * L] * (1.0 / d));
             }
         }
         __syncthreads();
@@ -340,17 +384,17 @@ __device__ void rr_tri_body(RR2Args r) {
     __syncthreads();
     // U rows scaled (strict upper); the lower triangle with the diagonal
     // becomes the accumulator of the inverse
-    {
-        const int c = ty, i = tx;
-        if (c < a && i < a) {
-            double &e = Bm[(oR + i) + (oR + c) * L];
-            e = i < c ? e / sdg[oR + i] : 0.0;
+    for (int c = ty; c < mRP; c += 32)
+        for (int i = tx; i < mRP; i += 32) {
+            if (c < a && i < a) {
+                double &e = Bm[(oR + i) + (oR + c) * L];
+                e = i < c ? e / sdg[oR + i] : 0.0;
+            }
+            if (c < p && i < p) {
+                double &e = Bm[(oP + i) + (oP + c) * L];
+                e = i < c ? e / sdg[oP + i] : 0.0;
+            }
         }
-        if (c < p && i < p) {
-            double &e = Bm[(oP + i) + (oP + c) * L];
-            e = i < c ? e / sdg[oP + i] : 0.0;
-        }
-    }
     for (int j = tid; j < a + p; j += kT) invu[oR + j] = 1.0 / sdg[oR + j];
     __syncthreads();
 
@@ -358,36 +402,38 @@ __device__ void rr_tri_body(RR2Args r) {
     // acc(i, c) (i <= c, stored at (c, i)) collects -sum_{l>i} U(i,l) W(l,c);
     // when row j is final it is fanned out to the rows above it
     for (int j = mRP - 1; j >= 0; j--) {
-        const int c = j + ty, i = tx;
-        if (j < a && c < a && i < j)
-            Bm[(oR + c) + (oR + i) * L] -= Bm[(oR + i) + (oR + j) * L] *
-                                           ((c == j ? 1.0 : 0.0) + Bm[(oR + c) + (oR + j) * L]) *
-                                           invu[oR + j];
-        if (j < p && c < p && i < j)
-            Bm[(oP + c) + (oP + i) * L] -= Bm[(oP + i) + (oP + j) * L] *
-                                           ((c == j ? 1.0 : 0.0) + Bm[(oP + c) + (oP + j) * L]) *
-                                           invu[oP + j];
+        for (int c = j + ty; c < mRP; c += 32)
+            for (int i = tx; i < j; i += 32) {
+                if (j < a && c < a)
+                    Bm[(oR + c) + (oR + i) * L] -=
+                        Bm[(oR + i) + (oR + j) * L] *
+                        ((c == j ? 1.0 : 0.0) + Bm[(oR + c) + (oR + j) * L]) * invu[oR + j];
+                if (j < p && c < p)
+                    Bm[(oP + c) + (oP + i) * L] -=
+                        Bm[(oP + i) + (oP + j) * L] *
+                        ((c == j ? 1.0 : 0.0) + Bm[(oP + c) + (oP + j) * L]) * invu[oP + j];
+            }
         __syncthreads();
     }
-    {
-        const int c = ty, j = tx;
-        if (c < a && j < a) {
-            double w = 0.0;
-            if (j <= c) {
-                w = ((j == c ? 1.0 : 0.0) + Bm[(oR + c) + (oR + j) * L]) * invu[oR + j];
-                Bm[(oR + j) + (oR + c) * L] = w;
+    for (int c = ty; c < mRP; c += 32)
+        for (int j = tx; j < mRP; j += 32) {
+            if (c < a && j < a) {
+                double w = 0.0;
+                if (j <= c) {
+                    w = ((j == c ? 1.0 : 0.0) + Bm[(oR + c) + (oR + j) * L]) * invu[oR + j];
+                    Bm[(oR + j) + (oR + c) * L] = w;
+                }
+                WRg[j + c * a] = w;
             }
-            WRg[j + c * a] = w;
-        }
-        if (c < p && j < p) {
-            double w = 0.0;
-            if (j <= c) {
-                w = ((j == c ? 1.0 : 0.0) + Bm[(oP + c) + (oP + j) * L]) * invu[oP + j];
-                Bm[(oP + j) + (oP + c) * L] = w;
+            if (c < p && j < p) {
+                double w = 0.0;
+                if (j <= c) {
+                    w = ((j == c ? 1.0 : 0.0) + Bm[(oP + c) + (oP + j) * L]) * invu[oP + j];
+                    Bm[(oP + j) + (oP + c) * L] = w;
+                }
+                WPg[j + c * p] = w;
             }
-            WPg[j + c * p] = w;
         }
-    }
     __syncthreads();
 
     const int tk = (k + 15) / 16, ta = (a + 15) / 16;
@@ -399,10 +445,10 @@ __device__ void rr_tri_body(RR2Args r) {
         const int tXR = (oP + 15) / 16, tN = (n + 15) / 16;
         const int j1 = tXR * ta, j2 = j1 + tN * tp, j3 = j2 + tk * ta, j4 = j3 + tk * tp,
                   j5 = j4 + ta * tp;
-        f64x4 acc[3];
-        int jr0[3], jrend[3], jc0[3], jcend[3], jmat[3];
+        f64x4 acc[S_RF];
+        int jr0[S_RF], jrend[S_RF], jc0[S_RF], jcend[S_RF], jmat[S_RF];
 #pragma unroll
-        for (int s = 0; s < 3; s++) {
+        for (int s = 0; s < S_RF; s++) {
             const int t = wave + s * kW;
             jmat[s] = -1;
             if (t >= j5) continue;
@@ -432,7 +478,7 @@ __device__ void rr_tri_body(RR2Args r) {
         }
         __syncthreads();
 #pragma unroll
-        for (int s = 0; s < 3; s++) {
+        for (int s = 0; s < S_RF; s++) {
             if (jmat[s] < 0) continue;
             double *M = jmat[s] ? Bm : Am;
             const int col = jc0[s] + li;
@@ -447,7 +493,11 @@ __device__ void rr_tri_body(RR2Args r) {
     // ---- left factor: A_n[R, R u P] = W_R' T, A_n[P, P] = W_P' T, B_n[R, P] = W_R' T_B ----
     {
         const int j1 = ta * (ta + tp), j2 = j1 + tp * tp, j3 = j2 + ta * tp;
-        const int t = wave;
+        f64x4 accs[S_LF];
+        int mats[S_LF], r0s[S_LF], rends[S_LF], c0s[S_LF], cends[S_LF];
+#pragma unroll
+        for (int s = 0; s < S_LF; s++) {
+        const int t = wave + s * kW;
         f64x4 acc;
         int mat = -1, r0 = 0, rend = 0, c0 = 0, cend = 0;
         if (t < j3) {
@@ -470,14 +520,23 @@ __device__ void rr_tri_body(RR2Args r) {
                 [&](int l, int col) { return msk(col < cend, M[l + imin(col, cend - 1) * L]); },
                 r0, c0, wo, lend);
         }
+        accs[s] = acc;
+        mats[s] = mat;
+        r0s[s] = r0;
+        rends[s] = rend;
+        c0s[s] = c0;
+        cends[s] = cend;
+        }
         __syncthreads();
-        if (mat >= 0) {
-            double *M = mat ? Bm : Am;
-            const int col = c0 + li;
+#pragma unroll
+        for (int s = 0; s < S_LF; s++) {
+            if (mats[s] < 0) continue;
+            double *M = mats[s] ? Bm : Am;
+            const int col = c0s[s] + li;
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const int row = r0 + lk + 4 * q;
-                if (row < rend && col < cend) M[row + col * L] = acc[q];
+                const int row = r0s[s] + lk + 4 * q;
+                if (row < rends[s] && col < cends[s]) M[row + col * L] = accs[s][q];
             }
         }
         __syncthreads();
@@ -510,7 +569,8 @@ __device__ void rr_tri_body(RR2Args r) {
     // again without P (the [X R] block is kept in global scratch for that)
     const int j0 = r.explicit_x ? 0 : k;  // the identity X block needs no steps
     double *Sg = Wg + (size_t)n0 * n0;
-    const bool may_retry = r.retry_without_p && p > 0;
+    const bool cond_on = r.cond_limit > 0.0;
+    const bool may_retry = (r.retry_without_p || cond_on) && p > 0;
     if (may_retry) {
         for (int c = ty; c < oP; c += 32)
             for (int i = tx; i < oP; i += 32) Sg[i + (size_t)c * oP] = Bm[i + c * L];
@@ -519,22 +579,30 @@ __device__ void rr_tri_body(RR2Args r) {
     for (int attempt = 0; attempt < 2; attempt++) {
         const int m2 = n - j0, t2 = (m2 + 15) / 16;
         if (j0 > 0 && m2 > 0) {
-            // Schur complement B[RP, RP] - E'E, E = B[X, RP]: one tile per wave
-            const int t = wave;
-            f64x4 acc;
-            const bool act = t < t2 * t2;
-            const int r0 = j0 + 16 * (act ? t / t2 : 0), c0 = j0 + 16 * (act ? t % t2 : 0);
-            if (act)
-                acc = tile_mm([&](int row, int l) { return msk(row < n, Bm[l + imin(row, n - 1) * L]); },
-                              [&](int l, int col) { return msk(col < n, Bm[l + imin(col, n - 1) * L]); },
-                              r0, c0, 0, k);
-            __syncthreads();
-            if (act) {
-                const int col = c0 + li;
+            // Schur complement B[RP, RP] - E'E, E = B[X, RP]: tiles dealt over the waves
+            f64x4 acc[S_SC];
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int row = r0 + lk + 4 * q;
-                    if (row < n && col < n) Bm[row + col * L] -= acc[q];
+            for (int s = 0; s < S_SC; s++) {
+                const int t = wave + s * kW;
+                const bool act = t < t2 * t2;
+                const int r0 = j0 + 16 * (act ? t / t2 : 0), c0 = j0 + 16 * (act ? t % t2 : 0);
+                if (act)
+                    acc[s] = tile_mm([&](int row, int l) { return msk(row < n, Bm[l + imin(row, n - 1) * L]); },
+                                     [&](int l, int col) { return msk(col < n, Bm[l + imin(col, n - 1) * L]); },
+                                     r0, c0, 0, k);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int s = 0; s < S_SC; s++) {
+                const int t = wave + s * kW;
+                if (t < t2 * t2) {
+                    const int r0 = j0 + 16 * (t / t2), c0 = j0 + 16 * (t % t2);
+                    const int col = c0 + li;
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const int row = r0 + lk + 4 * q;
+                        if (row < n && col < n) Bm[row + col * L] -= acc[s][q];
+                    }
                 }
             }
             __syncthreads();
@@ -553,7 +621,28 @@ __device__ void rr_tri_body(RR2Args r) {
             }
             __syncthreads();
         }
-        if (!infoB || !may_retry || attempt > 0) break;
+        // condition estimate of B_n (lobpcg.m:846): log10 of the spread of the
+        // Cholesky pivots (the identity X block has pivots 1) + 1
+        bool cond_trip = false;
+        if (!infoB && cond_on) {
+            if (wave == 0) {
+                double lo = j0 > 0 ? 1.0 : 1e308, hi = j0 > 0 ? 1.0 : 0.0;
+                for (int j = j0 + lane; j < n; j += 64) {
+                    const double d = Bm[j * (L + 1)];
+                    lo = fmin(lo, d);
+                    hi = fmax(hi, d);
+                }
+                lo = wmin(lo);
+                hi = wmax(hi);
+                if (lane == 0) cpiv[0] = log10(hi / lo) + 1.0;
+            }
+            __syncthreads();
+            const double cest = cpiv[0];
+            if (tid == 0) r.st->condest = cest;
+            cond_trip = attempt == 0 && p > 0 && cest > r.cond_limit;
+        }
+        const bool retry_pd = infoB && r.retry_without_p && p > 0 && attempt == 0;
+        if (!retry_pd && !cond_trip) break;
         // drop P: restore the [X R] block and factor again
         __syncthreads();
         for (int c = ty; c < oP; c += 32)
@@ -561,7 +650,7 @@ __device__ void rr_tri_body(RR2Args r) {
         p = 0;
         n = oP;
         tp = 0;
-        if (tid == 0) r.st->p_info = -1;
+        if (tid == 0) r.st->p_info = cond_trip ? -2 : -1;
         __syncthreads();
     }
     if (infoB) {
@@ -600,23 +689,31 @@ __device__ void rr_tri_body(RR2Args r) {
     __syncthreads();
     if (j0 > 0 && m2 > 0) {
         // W[X, RP] = -E W2
-        const int t = wave;
-        f64x4 acc;
-        const bool act = t < tk * t2;
-        const int r0 = 16 * (act ? t / t2 : 0), c0 = j0 + 16 * (act ? t % t2 : 0);
-        if (act) {
-            const int lend = c0 + 16 < n ? c0 + 16 : n;
-            acc = tile_mm([&](int row, int l) { return msk(row < k, Bm[imin(row, k - 1) + l * L]); },
-                          [&](int l, int col) { return msk(col < n && l <= col, Bm[l + imin(col, n - 1) * L]); },
-                          r0, c0, j0, lend);
+        f64x4 acc[S_WX];
+#pragma unroll
+        for (int s = 0; s < S_WX; s++) {
+            const int t = wave + s * kW;
+            const bool act = t < tk * t2;
+            const int r0 = 16 * (act ? t / t2 : 0), c0 = j0 + 16 * (act ? t % t2 : 0);
+            if (act) {
+                const int lend = c0 + 16 < n ? c0 + 16 : n;
+                acc[s] = tile_mm([&](int row, int l) { return msk(row < k, Bm[imin(row, k - 1) + l * L]); },
+                                 [&](int l, int col) { return msk(col < n && l <= col, Bm[l + imin(col, n - 1) * L]); },
+                                 r0, c0, j0, lend);
+            }
         }
         __syncthreads();
-        if (act) {
-            const int col = c0 + li;
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int row = r0 + lk + 4 * q;
-                if (row < k && col < n) Bm[row + col * L] = -acc[q];
+        for (int s = 0; s < S_WX; s++) {
+            const int t = wave + s * kW;
+            if (t < tk * t2) {
+                const int r0 = 16 * (t / t2), c0 = j0 + 16 * (t % t2);
+                const int col = c0 + li;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int row = r0 + lk + 4 * q;
+                    if (row < k && col < n) Bm[row + col * L] = -acc[s][q];
+                }
             }
         }
         __syncthreads();
@@ -625,9 +722,9 @@ __device__ void rr_tri_body(RR2Args r) {
     // ---- C = W' A_n W (lower triangle kept); W saved for the back-transform ----
     const int tN = (n + 15) / 16;
     {
-        f64x4 acc[3];
+        f64x4 acc[S_C1];
 #pragma unroll
-        for (int s = 0; s < 3; s++) {
+        for (int s = 0; s < S_C1; s++) {
             const int t = wave + s * kW;
             if (t < tN * tN) {
                 const int r0 = 16 * (t / tN), c0 = 16 * (t % tN);
@@ -640,7 +737,7 @@ __device__ void rr_tri_body(RR2Args r) {
         }
         __syncthreads();
 #pragma unroll
-        for (int s = 0; s < 3; s++) {
+        for (int s = 0; s < S_C1; s++) {
             const int t = wave + s * kW;
             if (t < tN * tN) {
                 const int r0 = 16 * (t / tN), col = 16 * (t % tN) + li;
@@ -656,10 +753,10 @@ __device__ void rr_tri_body(RR2Args r) {
         __syncthreads();
     }
     {
-        f64x4 acc[2];
-        int rr0[2], cc0[2];
+        f64x4 acc[S_C2];
+        int rr0[S_C2], cc0[S_C2];
 #pragma unroll
-        for (int s = 0; s < 2; s++) {
+        for (int s = 0; s < S_C2; s++) {
             int t = wave + s * kW;
             rr0[s] = -1;
             if (t < tN * (tN + 1) / 2) {
@@ -680,7 +777,7 @@ __device__ void rr_tri_body(RR2Args r) {
         }
         __syncthreads();
 #pragma unroll
-        for (int s = 0; s < 2; s++) {
+        for (int s = 0; s < S_C2; s++) {
             if (rr0[s] < 0) continue;
             const int col = cc0[s] + li;
 #pragma unroll
@@ -697,11 +794,16 @@ __device__ void rr_tri_body(RR2Args r) {
 
     // ---- Householder tridiagonalisation of C, held in registers ----
     // pencils up to 48: 4 waves (one per SIMD) with 9 entries per lane; up
-    // to 96: 8 waves with 18
+    // to 96: 8 waves with 18; up to 128: 16 waves with 16; up to 192: 16
+    // waves with 36
     if (n <= 48)
-        householder<4, 3, 3>(n, L, Am, col0, col1, pvec, dT, eT, tauT);
+        householder<NM, 4, 3, 3>(n, L, Am, col0, col1, pvec, dT, eT, tauT);
+    else if (NM <= 96 || n <= 96)
+        householder<NM, 8, 3, 6>(n, L, Am, col0, col1, pvec, dT, eT, tauT);
+    else if (NM <= 128 || n <= 128)
+        householder<NM, 16, 2, 8>(n, L, Am, col0, col1, pvec, dT, eT, tauT);
     else
-        householder<8, 3, 6>(n, L, Am, col0, col1, pvec, dT, eT, tauT);
+        householder<NM, 16, 3, 12>(n, L, Am, col0, col1, pvec, dT, eT, tauT);
 
     // ---- Gershgorin interval; T scaled by a power of two near its norm ----
     if (wave == 0) {
@@ -921,37 +1023,40 @@ __device__ void rr_tri_body(RR2Args r) {
         }
     }
 
-    // ---- Z <- Q Z: reflections n-3 .. 0, 32 lanes per column ----
-    // the lane keeps rows l32 + 32 q of its column in registers; v of
+    // ---- Z <- Q Z: reflections n-3 .. 0, LPC lanes per column ----
+    // the lane keeps rows l + LPC q of its column in registers; v of
     // reflection j is column j of Am below the diagonal (zero above)
-    if ((tid >> 5) < k) {
-        const int c = tid >> 5, l32 = tid & 31;
-        double *z = Z + (size_t)c * L;
-        double zr[3];
+    {
+        constexpr int LPC = KM <= 32 ? 32 : 16, RPL = (NM + LPC - 1) / LPC;
+        if (tid / LPC < k) {
+            const int c = tid / LPC, l = tid % LPC;
+            double *z = Z + (size_t)c * L;
+            double zr[RPL];
 #pragma unroll
-        for (int q = 0; q < 3; q++) {
-            const int i = l32 + 32 * q;
-            zr[q] = i < n ? z[i] : 0.0;
-        }
-#pragma unroll 2
-        for (int j = n - 3; j >= 0; j--) {
-            const double tau = tauT[j];  // tau = 0 leaves z unchanged
-            double vv[3], s = 0.0;
-#pragma unroll
-            for (int q = 0; q < 3; q++) {
-                const int i = l32 + 32 * q, ic = i < n ? i : n - 1;
-                const double v = Am[ic + (size_t)j * L];
-                vv[q] = i > j && i < n ? v : 0.0;
-                s = fma(vv[q], zr[q], s);
+            for (int q = 0; q < RPL; q++) {
+                const int i = l + LPC * q;
+                zr[q] = i < n ? z[i] : 0.0;
             }
-            s = tau * group_sum32(s);
+#pragma unroll 2
+            for (int j = n - 3; j >= 0; j--) {
+                const double tau = tauT[j];  // tau = 0 leaves z unchanged
+                double vv[RPL], s = 0.0;
 #pragma unroll
-            for (int q = 0; q < 3; q++) zr[q] = fma(-s, vv[q], zr[q]);
-        }
+                for (int q = 0; q < RPL; q++) {
+                    const int i = l + LPC * q, ic = i < n ? i : n - 1;
+                    const double v = Am[ic + (size_t)j * L];
+                    vv[q] = i > j && i < n ? v : 0.0;
+                    s = fma(vv[q], zr[q], s);
+                }
+                s = tau * (LPC == 32 ? group_sum32(s) : group_sum16(s));
 #pragma unroll
-        for (int q = 0; q < 3; q++) {
-            const int i = l32 + 32 * q;
-            if (i < n) z[i] = zr[q];
+                for (int q = 0; q < RPL; q++) zr[q] = fma(-s, vv[q], zr[q]);
+            }
+#pragma unroll
+            for (int q = 0; q < RPL; q++) {
+                const int i = l + LPC * q;
+                if (i < n) z[i] = zr[q];
+            }
         }
     }
     __syncthreads();
@@ -959,28 +1064,39 @@ __device__ void rr_tri_body(RR2Args r) {
     // ---- Y = W Z (into Am), then coef = S Y ----
     {
         const int tKc = (k + 15) / 16;
-        const int t = wave;
-        f64x4 acc;
-        const bool act = t < tN * tKc;
-        const int r0 = 16 * (act ? t / tKc : 0), c0 = 16 * (act ? t % tKc : 0);
-        if (act)
-            acc = tile_mm(
-                [&](int row, int l) { return msk(row < n && l >= row, Wg[imin(row, n - 1) + (size_t)l * n]); },
-                [&](int l, int col) { return msk(col < k, Z[(size_t)imin(col, k - 1) * L + l]); },
-                r0, c0, r0, n);
-        __syncthreads();
-        if (act) {
-            const int col = c0 + li;
+        f64x4 acc[S_Y];
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int row = r0 + lk + 4 * q;
-                if (row < n && col < k) Am[row + col * L] = acc[q];
+        for (int s = 0; s < S_Y; s++) {
+            const int t = wave + s * kW;
+            const bool act = t < tN * tKc;
+            const int r0 = 16 * (act ? t / tKc : 0), c0 = 16 * (act ? t % tKc : 0);
+            if (act)
+                acc[s] = tile_mm(
+                    [&](int row, int l) { return msk(row < n && l >= row, Wg[imin(row, n - 1) + (size_t)l * n]); },
+                    [&](int l, int col) { return msk(col < k, Z[(size_t)imin(col, k - 1) * L + l]); },
+                    r0, c0, r0, n);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < S_Y; s++) {
+            const int t = wave + s * kW;
+            if (t < tN * tKc) {
+                const int r0 = 16 * (t / tKc), c0 = 16 * (t % tKc);
+                const int col = c0 + li;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int row = r0 + lk + 4 * q;
+                    if (row < n && col < k) Am[row + col * L] = acc[s][q];
+                }
             }
         }
         __syncthreads();
-        // R and P rows through W_R, W_P (one tile per wave), X rows copied
+        // R and P rows through W_R, W_P (tiles dealt over the waves), X rows copied
         const double *Y = Am;
         const int jr = ta * tKc, jp = jr + tp * tKc;
+#pragma unroll
+        for (int s = 0; s < S_CO; s++) {
+        const int t = wave + s * kW;
         if (t < jp) {
             const bool isR = t < jr;
             const int u = isR ? t : t - jr;
@@ -998,6 +1114,7 @@ __device__ void rr_tri_body(RR2Args r) {
                 if (row < m && col < k) r.coef[(o + row) + (size_t)col * r.ldc] = ac[q];
             }
         }
+        }
         for (int c = ty; c < k; c += 32) {
             for (int i = tx; i < k; i += 32) r.coef[i + (size_t)c * r.ldc] = Y[i + c * L];
             for (int i = n + tx; i < n0; i += 32) r.coef[i + (size_t)c * r.ldc] = 0.0;  // dropped P
@@ -1007,7 +1124,16 @@ __device__ void rr_tri_body(RR2Args r) {
     }
 }
 
-__global__ void __launch_bounds__(kT) k_rr_tri(RR2Args r) { rr_tri_body(r); }
+__global__ void __launch_bounds__(kT) k_rr_tri(RR2Args r) {
+    rr_tri_body<kN, kK, kBothLds>(r, nullptr, nullptr);
+}
+
+// pencils up to 192 (k, a, p <= 64): A_n in global memory, B_n in LDS
+// (MODE kALdsB) or global memory (kBothGlobal)
+template <int NM, int MODE>
+__global__ void __launch_bounds__(kT) k_rr_tri_big(RR2Args r, double *gA, double *gB) {
+    rr_tri_body<NM, kBigK, MODE>(r, gA, gB);
+}
 
 // one workgroup per problem of a batch: problem act[blockIdx.x] has blocks of
 // width kvec[p] (X and R, and P when with_p); its Gram buffers, Ritz values,
@@ -1027,7 +1153,7 @@ __global__ void __launch_bounds__(kT)
     r.coef += (size_t)p * s_coef;
     r.work += (size_t)p * s_work;
     r.st += p;
-    rr_tri_body(r);
+    rr_tri_body<kN, kK, kBothLds>(r, nullptr, nullptr);
 }
 
 }  // namespace
@@ -1063,6 +1189,42 @@ void rr_fast_batched(const RR2Args &base, const int *act, const int *nact, const
     }
     hipLaunchKernelGGL(k_rr_tri_batched, dim3(nprob), dim3(kT), lds, stream(), base, act, nact,
                        kvec, (int)with_p, s_gram, s_lam, s_coef, s_work);
+    BX_HIP(hipGetLastError());
+}
+
+bool rr_big_fits(int k, int a, int p) {
+    return k + a + p <= kBigRRN && k <= kBigRRK && a <= kBigRRK && p <= kBigRRK && k >= 1;
+}
+
+void rr_big(const RR2Args &r) {
+    BX_REQUIRE(rr_big_fits(r.k, r.a, r.p), "rr_big: pencil %d + %d + %d does not fit", r.k, r.a,
+               r.p);
+    const int n0 = r.k + r.a + r.p, L = pencil_ld(n0);
+    const int second = n0 > 2 * r.k ? n0 : 2 * r.k;
+    const size_t a_words = (size_t)L * n0, b_words = (size_t)L * second;
+    constexpr size_t kLds = 163840;  // LDS of one workgroup on gfx950
+    // pencils up to 128 take the instantiation with fewer registers per lane
+    const bool mid = n0 <= 128;
+    const size_t dyn_max = kLds - static_lds_bytes(mid ? 128 : kBigN, kBigK);
+    const bool b_lds = b_words * sizeof(double) <= dyn_max;
+    double *g = (double *)scratch(10, (a_words + (b_lds ? 0 : b_words)) * sizeof(double) + 256);
+    static bool attr_set = false;
+    if (!attr_set) {
+        BX_HIP(hipFuncSetAttribute((const void *)k_rr_tri_big<128, kALdsB>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(kLds - static_lds_bytes(128, kBigK))));
+        BX_HIP(hipFuncSetAttribute((const void *)k_rr_tri_big<kBigN, kALdsB>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(kLds - static_lds_bytes(kBigN, kBigK))));
+        attr_set = true;
+    }
+    auto kern = mid ? (b_lds ? k_rr_tri_big<128, kALdsB> : k_rr_tri_big<128, kBothGlobal>)
+                    : (b_lds ? k_rr_tri_big<kBigN, kALdsB> : k_rr_tri_big<kBigN, kBothGlobal>);
+    if (b_lds)
+        hipLaunchKernelGGL(kern, dim3(1), dim3(kT), b_words * sizeof(double), stream(), r, g,
+                           nullptr);
+    else
+        hipLaunchKernelGGL(kern, dim3(1), dim3(kT), 0, stream(), r, g, g + a_words);
     BX_HIP(hipGetLastError());
 }
 

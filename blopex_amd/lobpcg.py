@@ -105,14 +105,22 @@ def solve_device(A: DeviceCSR, X: DeviceBlock, B=None, T=None, Y: DeviceBlock | 
                  tol: float = 1e-6, rtol: float = 1e-50, maxit: int = 200, verbosity: int = 0,
                  mode: str = "fp64", switch_factor: float = 1.5, stall_iterations: int = 50,
                  refine_iterations: int = 0, histories: bool = True, profile: bool = False,
-                 refresh_interval: int = 20, final_rr: bool = False) -> Result:
+                 refresh_interval: int = 20, final_rr: bool = False, guard_columns: int = 0,
+                 eig_rtol: float = 0.0, cond_restart: bool = False, size_guard: bool = False,
+                 guard_seed: int = 7) -> Result:
     """lobpcg_solve_double on device-resident operands; X is overwritten."""
     lib = L.load()
     k = X.k
-    s = L.Settings(L.HIPLOBPCG_MIXED if mode == "mixed" else L.HIPLOBPCG_FP64, switch_factor,
-                   stall_iterations, refine_iterations, int(profile), refresh_interval,
-                   int(final_rr))
-    lib.hip_Multi_VectorSetSettings(X.ptr, ctypes.byref(s))
+    s = L.Settings()
+    lib.hip_LobpcgSettingsDefault(ctypes.byref(s))
+    s.mode = L.HIPLOBPCG_MIXED if mode == "mixed" else L.HIPLOBPCG_FP64
+    s.switch_factor, s.stall_iterations = switch_factor, stall_iterations
+    s.refine_iterations, s.profile = refine_iterations, int(profile)
+    s.refresh_interval, s.final_rr = refresh_interval, int(final_rr)
+    s.guard_columns, s.eig_rtol = guard_columns, eig_rtol
+    s.cond_restart, s.size_guard, s.guard_seed = int(cond_restart), int(size_guard), guard_seed
+    if lib.hip_Multi_VectorSetSettings(X.ptr, ctypes.byref(s)) != 0:
+        raise ValueError(L.last_error())
     ii = interpreter()
     xx = lib.mv_MultiVectorWrap(ctypes.byref(ii), ctypes.cast(X.ptr, ctypes.c_void_p), 0)
     yy = None

@@ -60,6 +60,24 @@ typedef struct hip_LobpcgSettings {
                                  do a Rayleigh-Ritz step on X with explicit X'AX, X'BX, so
                                  that X is B-orthonormal and the residuals are those of the
                                  returned X (lobpcg.m:922-974; default 0) */
+    int guard_columns;        /* g >= 0: the solver iterates k + g columns (k + g <= 64) and
+                                 tests and returns the first k, the k smallest Ritz pairs.
+                                 The guard columns widen the gap that sets the convergence
+                                 of the k-th pair; they start from 2*drand48()-1 after
+                                 srand48(guard_seed), column by column (default 0) */
+    double eig_rtol;          /* > 0: a wanted column also stays active until the error
+                                 bound of its eigenvalue, min over windows [a, b] of Ritz
+                                 pairs around it of sum_j ||r_j||^2 / gap(a, b), is at most
+                                 eig_rtol * |lambda_i| (default 0: residuals only) */
+    int cond_restart;         /* 1: when the condition estimate of the Rayleigh-Ritz pencil's
+                                 B part jumps (log10 of its Cholesky pivot spread + 1 above
+                                 min(8, max(2, twice its recent mean))), that step is solved
+                                 again without P, a steepest-descent restart (lobpcg.m:
+                                 846-863; default 0) */
+    int size_guard;           /* 1: return PROBLEM_SIZE_TOO_SMALL when n - sizeY < 5 (k + g)
+                                 (lobpcg.m:475-484; lobpcg.c:672-687 has it disabled;
+                                 default 0) */
+    int guard_seed;           /* srand48 seed of the guard columns (default 7) */
 } hip_LobpcgSettings;
 
 typedef struct hip_LobpcgStats {
@@ -82,6 +100,11 @@ typedef struct hip_LobpcgStats {
                                    switch_factor * atol, 1 stall, 2 breakdown (R or pencil
                                    not positive definite in FP32) */
     int final_rr;               /* 1 when the final Rayleigh-Ritz step ran */
+    int guard_columns;          /* guard columns iterated */
+    int cond_restarts;          /* steps solved again without P on a condition-estimate jump */
+    double eig_error_bound;     /* at exit: max over the wanted columns of the eigenvalue
+                                   error bound (see eig_rtol) divided by |lambda_i|; 0 when
+                                   eig_rtol is 0 */
 } hip_LobpcgStats;
 
 typedef struct hip_Multi_Vector {
@@ -117,6 +140,9 @@ int hip_Multi_VectorDownload(const hip_Multi_Vector *x, double *host);
    generated on the device by jumping the LCG ahead per thread */
 int hip_Multi_VectorSetRandomValues(hip_Multi_Vector *x, BlopexInt seed);
 int hip_Multi_VectorSetSettings(hip_Multi_Vector *x, const hip_LobpcgSettings *s);
+/* the defaults every new block carries (FP64 mode, switch_factor 1.5, stall window 50,
+   refresh interval 20, guard seed 7, everything else 0) */
+void hip_LobpcgSettingsDefault(hip_LobpcgSettings *s);
 int hip_Multi_VectorGetStats(const hip_Multi_Vector *x, hip_LobpcgStats *s);
 
 /* fills every multivector slot of the table the solver uses */

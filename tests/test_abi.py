@@ -53,10 +53,12 @@ def test_struct_layouts():
     assert ctypes.sizeof(L.BlasLapack) == 32
     # hip_Multi_Vector: data, size, ld, 2 ints, ptr, 2 ints, settings(40), stats(88)
     assert L.HipMultiVector.settings.offset == 48
-    assert ctypes.sizeof(L.Settings) == 40 and ctypes.sizeof(L.Stats) == 88
+    assert ctypes.sizeof(L.Settings) == 64 and ctypes.sizeof(L.Stats) == 104
     assert L.Settings.refresh_interval.offset == 28 and L.Settings.final_rr.offset == 32
+    assert L.Settings.eig_rtol.offset == 40 and L.Settings.guard_seed.offset == 56
+    assert L.Stats.eig_error_bound.offset == 96
     assert L.Stats.final_rr.offset == 80
-    assert ctypes.sizeof(L.HipMultiVector) == 48 + 40 + 88
+    assert ctypes.sizeof(L.HipMultiVector) == 48 + 64 + 104
     # hip_CSRMatrix: 6 words, ndiag + 16 offsets (68 bytes, padded to 72), diag_ld, 2 pointers,
     # grid[3] (padded to 16)
     assert L.CSRMatrix.diag_ld.offset == 48 + 72 and L.CSRMatrix.grid.offset == 48 + 72 + 24

@@ -106,6 +106,54 @@ def test_rr_matches_scipy(jacobi, k, a, p, degenerate, bdiag):
     assert np.linalg.norm(Pg - Pr) < 1e-9
 
 
+@pytest.mark.parametrize("k,a,p,degenerate,bdiag", [
+    # the large variant: pencils up to 128 (B_n in LDS), up to 192 (B_n in LDS
+    # up to the LDS size, else in global memory), block widths above 32
+    (40, 40, 40, False, False), (40, 40, 40, True, True), (33, 33, 0, False, False),
+    (40, 8, 8, False, True), (44, 42, 42, False, False), (64, 64, 64, False, True),
+    (48, 48, 48, True, False), (64, 3, 3, False, False), (36, 36, 36, False, True),
+    (64, 40, 0, False, False)])
+def test_rr_large_pencils_match_scipy(k, a, p, degenerate, bdiag):
+    say(f"rr large k={k} a={a} p={p} degenerate={degenerate} bdiag={bdiag}")
+    A, B, X, R, P, lam = make_case(k, a, p, seed=k * 100 + a + p + 1, degenerate=degenerate,
+                                   bdiag=bdiag)
+    ev, Xr = reference(A, B, X, R, P, lam)
+    lg, Xg, flags, _ = run_kernel(A, B, X, R, P, lam, 0)
+    assert flags[0] == 0 and flags[2] == 0
+    assert np.max(np.abs(lg - ev) / np.abs(ev)) < 1e-12
+    np.testing.assert_allclose(Xg.T @ B @ Xg, np.eye(k), atol=1e-12)
+    Pg, Pr = Xg @ Xg.T @ B, Xr @ Xr.T @ B
+    assert np.linalg.norm(Pg - Pr) < 1e-9
+
+
+def test_rr_large_explicit_x_and_retry():
+    """The large variant with computed X blocks (explicit_x), then a pencil
+    that is not positive definite solved again without P."""
+    say("rr large explicit_x and retry without P (k=40)")
+    A, B, X, R, P, lam = make_case(40, 40, 40, seed=17, bdiag=True)
+    k = 40
+    Xp = X @ (np.eye(k) + 1e-3 * np.random.default_rng(2).standard_normal((k, k)))
+    Bas = np.hstack([Xp, R, P])
+    ev = sl.eigh(Bas.T @ A @ Bas, Bas.T @ B @ Bas, eigvals_only=True)[:k]
+    lg, Xg, flags, _ = run_kernel(A, B, Xp, R, P, np.zeros(k), 0, explicit_x=1)
+    assert flags[0] == 0
+    assert np.max(np.abs(lg - ev) / np.abs(ev)) < 1e-11
+    np.testing.assert_allclose(Xg.T @ B @ Xg, np.eye(k), atol=1e-11)
+    # R B-orthogonal to X, so that [X R] stays positive definite with X
+    # doubled (at k = 40 the cross block 2 X'B R of random R alone has norm
+    # above one); a P column along X makes the full pencil indefinite
+    R = R - X @ (X.T @ B @ R)
+    Xs = 2.0 * X
+    P[:, 0] = X[:, 0]
+    _, _, flags, _ = run_kernel(A, B, Xs, R, P, lam, 0)
+    assert flags[0] == 1 and flags[3] > 0
+    lg, Xg, flags, coef = run_kernel(A, B, Xs, R, P, lam, 0, retry=1)
+    assert flags[0] == 0 and flags[2] == -1 and flags[3] == 0
+    assert np.all(coef[80:] == 0.0)
+    ev, _ = reference(A, B, Xs, R, P[:, :0], lam)
+    assert np.max(np.abs(lg - ev) / np.abs(ev)) < 1e-12
+
+
 @pytest.mark.parametrize("jacobi", [0, 1])
 def test_rr_explicit_x_blocks(jacobi):
     """explicit_x uses the computed X'AX / X'BX: X need not be B-orthonormal."""
