@@ -110,6 +110,7 @@ _SIGS = {
     "blopex_amd_stream": (c_void_p, []),
     "blopex_amd_synchronize": (c_int, []),
     "blopex_amd_last_error": (ctypes.c_char_p, []),
+    "blopex_amd_release_workspace": (c_int, []),
     "hip_Multi_VectorCreate": (P(HipMultiVector), [c_long, c_int, c_int]),
     "hip_Multi_VectorWrapDevice": (P(HipMultiVector), [c_void_p, c_long, c_int, c_int]),
     "hip_Multi_VectorDestroy": (None, [P(HipMultiVector)]),
@@ -171,6 +172,9 @@ _SIGS = {
     "blopex_amd_kernel_mfma_probe": (c_int, [c_int, c_void_p, c_void_p]),
     "blopex_amd_kernel_rr": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p,
                                      c_int, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    "blopex_amd_kernel_rr_cond": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_int,
+                                          c_void_p, c_int, c_double, c_void_p, c_void_p,
+                                          c_void_p, c_void_p]),
     "hip_IJReadCSR": (c_int, [ctypes.c_char_p, P(c_long), P(c_long), P(P(c_int)), P(P(c_int)),
                               P(P(c_double))]),
     "hip_ParVectorsRead": (c_int, [ctypes.c_char_p, P(c_long), P(c_int), P(P(c_double))]),

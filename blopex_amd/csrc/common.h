@@ -38,8 +38,9 @@ hipStream_t stream();
 int device();
 int num_cus();
 
-// Library-owned device scratch, grown on demand (never shrunk); slot s is
-// independent of the other slots.  Contents are undefined on return.
+// Library-owned device scratch, grown on demand and kept between calls
+// (blopex_amd_release_workspace frees it); slot s is independent of the
+// other slots.  Contents are undefined on return.
 void *scratch(int slot, size_t bytes);
 // Pinned host staging buffer of at least `bytes`.
 void *pinned(int slot, size_t bytes);

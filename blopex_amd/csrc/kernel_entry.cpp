@@ -84,10 +84,12 @@ int blopex_amd_kernel_gram(int prec, long n, int lw, int qw, const double *L, co
     return 0;
 }
 
-int blopex_amd_kernel_rr(int k, int a, int p, const double *ga, const double *gb, int lda,
-                         const double *lam_in, int explicit_x, int retry_without_p, int jacobi,
-                         double *lam_out,
-                         double *coef, int *flags) {
+}  // extern "C"
+
+namespace {
+int kernel_rr(int k, int a, int p, const double *ga, const double *gb, int lda,
+              const double *lam_in, int explicit_x, int retry_without_p, int jacobi,
+              double cond_limit, double *lam_out, double *coef, int *flags, double *condest) {
     const int n = k + a + p;
     const size_t gbytes = (size_t)lda * n * sizeof(double);
     char *d = (char *)scratch(9, 2 * gbytes + (size_t)(2 * k + n * k) * 8 + sizeof(DevStatus) + 64);
@@ -99,7 +101,8 @@ int blopex_amd_kernel_rr(int k, int a, int p, const double *ga, const double *gb
     upload_async(dlam, lam_in, (size_t)k * 8);
     BX_HIP(hipMemsetAsync(st, 0, sizeof(DevStatus), stream()));
     g_force_jacobi = jacobi != 0;
-    rr_solve_implicit(dga, dgb, lda, k, a, p, dlam, dlo, dc, n, st, explicit_x, retry_without_p);
+    rr_solve_implicit(dga, dgb, lda, k, a, p, dlam, dlo, dc, n, st, explicit_x, retry_without_p,
+                      cond_limit);
     g_force_jacobi = false;
     DevStatus h;
     download_sync(&h, st, sizeof(h));
@@ -109,7 +112,25 @@ int blopex_amd_kernel_rr(int k, int a, int p, const double *ga, const double *gb
     flags[1] = h.r_info;
     flags[2] = h.p_info;
     flags[3] = h.gevp_info;
+    if (condest) *condest = h.condest;
     return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int blopex_amd_kernel_rr(int k, int a, int p, const double *ga, const double *gb, int lda,
+                         const double *lam_in, int explicit_x, int retry_without_p, int jacobi,
+                         double *lam_out, double *coef, int *flags) {
+    return kernel_rr(k, a, p, ga, gb, lda, lam_in, explicit_x, retry_without_p, jacobi, 0.0,
+                     lam_out, coef, flags, nullptr);
+}
+
+int blopex_amd_kernel_rr_cond(int k, int a, int p, const double *ga, const double *gb, int lda,
+                              const double *lam_in, int explicit_x, double cond_limit,
+                              double *lam_out, double *coef, int *flags, double *condest) {
+    return kernel_rr(k, a, p, ga, gb, lda, lam_in, explicit_x, 0, 0, cond_limit, lam_out, coef,
+                     flags, condest);
 }
 
 int blopex_amd_kernel_mfma_probe(int prec, int *rows, int *cols) {

@@ -21,7 +21,7 @@ int device_lobpcg(mv_MultiVectorPtr X, void *Ad, void (*opA)(void *, void *, voi
                   void (*opT)(void *, void *, void *), mv_MultiVectorPtr Y, lobpcg_Tolerance tol,
                   int maxit, int verb, int *iters, double *lam, double *lamh, BlopexInt lamh_gh,
                   double *res, double *resh, BlopexInt resh_gh);
-bool device_lobpcg_fits(int k);
+bool device_lobpcg_fits(int k, int guard);
 }
 
 extern "C" {
@@ -50,7 +50,8 @@ int lobpcg_solve_double(mv_MultiVectorPtr X, void *Ad, void (*opA)(void *, void 
             *iters = 0;
             return WRONG_BLOCK_SIZE;
         }
-        if (bx::device_lobpcg_fits(k) || !blap.dpotrf || !blap.dsygv)
+        const int guard = ((hip_Multi_Vector *)mv_MultiVectorGetData(X))->settings.guard_columns;
+        if (bx::device_lobpcg_fits(k, guard) || !blap.dpotrf || !blap.dsygv)
             return bx::device_lobpcg(X, Ad, opA, Bd, opB, Td, opT, Y, tol, maxit, verb, iters,
                                      lam, lamh, lamh_gh, res, resh, resh_gh);
     }

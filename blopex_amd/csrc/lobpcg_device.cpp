@@ -67,6 +67,9 @@ hip_Multi_Vector view(void *p, long n, int w, int prec) {
 // of launches, resolved after the stream has been synchronised.
 enum KClass { kSpmm, kGram, kUpdate, kDense, kOther, kNumClasses };
 
+// library workspace slots of the solver (see scratch() in common.h)
+constexpr int kSlotPool = 11, kSlotSmall = 12, kSlotY = 13, kSlotXw = 14;
+
 // residual level below which FP64 iterations measure X'AX and X'BX.
 // lobpcg.m switches at eps^0.6 (4e-10); here the implicit blocks already broke
 // a 10^3 Jacobi solve down near 1e-10, so the switch comes at eps^0.5 (1.5e-8).
@@ -153,10 +156,13 @@ class Phase {
     int fp32_exit_ = 0;  // why the FP32 phase ended (hip_LobpcgStats.fp32_exit)
     int refine_done_ = 0;  // refinement iterations run (hip_LobpcgStats.refine_iterations)
     int final_rr_done_ = 0;  // 1 once the final Rayleigh-Ritz ran
-    Phase(int prec, long n, int k, int ny, Op opA, void *Ad, Op opB, void *Bd, Op opT, void *Td,
-          const hip_Multi_Vector *Yuser, lobpcg_Tolerance tol, int verb, HostOut out, KTimer *tm)
-        : prec_(prec), n_(n), k_(k), ny_(ny), opA_(opA), Ad_(Ad), opB_(opB), Bd_(Bd), opT_(opT),
-          Td_(Td), Yu_(Yuser), tol_(tol), verb_(verb), out_(out), tm_(tm) {
+    int cond_restarts_ = 0;  // steps solved again without P on a condition-estimate jump
+    // k: columns iterated; kw <= k: the wanted ones (the rest are guard columns)
+    Phase(int prec, long n, int k, int kw, int ny, Op opA, void *Ad, Op opB, void *Bd, Op opT,
+          void *Td, const hip_Multi_Vector *Yuser, lobpcg_Tolerance tol, int verb, HostOut out,
+          KTimer *tm)
+        : prec_(prec), n_(n), k_(k), kw_(kw), ny_(ny), opA_(opA), Ad_(Ad), opB_(opB), Bd_(Bd),
+          opT_(opT), Td_(Td), Yu_(Yuser), tol_(tol), verb_(verb), out_(out), tm_(tm) {
         noB_ = opB == nullptr;
         noT_ = opT == nullptr;
         fuseT_ = opT == HipDiagMatMultiVec;
@@ -165,7 +171,10 @@ class Phase {
         ld_ = (n + 63) / 64 * 64;  // padded leading dimension; padding rows stay zero
         const size_t blk = (size_t)ld_ * k * es_;
         const int nblk = noB_ ? 7 : 10;
-        BX_HIP(hipMalloc(&pool_, blk * nblk + 256));
+        // the block pool, the small dense buffers and the constraint blocks
+        // live in library workspace kept between solves (the FP32 and FP64
+        // phases of one solve never hold theirs at the same time)
+        pool_ = scratch(kSlotPool, blk * nblk + 256);
         BX_HIP(hipMemsetAsync(pool_, 0, blk * nblk + 256, stream()));
         char *p = (char *)pool_;
         for (int i = 0; i < nblk; i++) bufs_.push_back(p + i * blk);
@@ -182,7 +191,7 @@ class Phase {
             BP_ = bufs_[9];
         }
         const size_t small = (size_t)ld3_ * ld3_ * 2 + (size_t)ld3_ * k + 6 * (size_t)k + 64;
-        BX_HIP(hipMalloc(&dsmall_, small * sizeof(double) + sizeof(DevStatus)));
+        dsmall_ = (double *)scratch(kSlotSmall, small * sizeof(double) + sizeof(DevStatus));
         GA_ = dsmall_;
         GB_ = GA_ + (size_t)ld3_ * ld3_;
         coef_ = GB_ + (size_t)ld3_ * ld3_;
@@ -195,7 +204,7 @@ class Phase {
         BX_HIP(hipMemsetAsync(dsmall_, 0, small * sizeof(double) + sizeof(DevStatus), stream()));
         if (ny_ > 0) {
             const size_t yb = (size_t)ld_ * ny_ * es_;
-            BX_HIP(hipMalloc(&ypool_, yb * 2 + ((size_t)ny_ * ny_ + 2 * (size_t)ny_ * k) * 8));
+            ypool_ = scratch(kSlotY, yb * 2 + ((size_t)ny_ * ny_ + 2 * (size_t)ny_ * k) * 8);
             BX_HIP(hipMemsetAsync(ypool_, 0, yb * 2, stream()));
             Y_ = ypool_;
             BY_ = noB_ ? Y_ : (char *)ypool_ + yb;
@@ -204,12 +213,7 @@ class Phase {
             tybx_ = ybx_ + (size_t)ny_ * k;
         }
     }
-    ~Phase() {
-        hipStreamSynchronize(stream());
-        if (pool_) hipFree(pool_);
-        if (dsmall_) hipFree(dsmall_);
-        if (ypool_) hipFree(ypool_);
-    }
+    ~Phase() { hipStreamSynchronize(stream()); }
 
     // Returns the exit flag; *it_io is the iteration count so far (in) and
     // after this phase (out).  fp32_phase: stop when FP32 has done its part.
@@ -286,24 +290,73 @@ class Phase {
     void project_constraints(void *blk, int w);
     void store_hist(int col) {
         if (out_.lamh)
-            for (int i = 0; i < k_; i++) out_.lamh[i + (size_t)col * out_.lamh_gh] = out_.lam[i];
+            for (int i = 0; i < kw_; i++) out_.lamh[i + (size_t)col * out_.lamh_gh] = out_.lam[i];
         if (out_.resh)
-            for (int i = 0; i < k_; i++) out_.resh[i + (size_t)col * out_.resh_gh] = out_.res[i];
+            for (int i = 0; i < kw_; i++) out_.resh[i + (size_t)col * out_.resh_gh] = out_.res[i];
     }
     void dump(int it) {
         printf("\nEigenvalue lambda       Residual              \n");
-        for (int i = 0; i < k_; i++) printf("%22.14e  %22.14e\n", out_.lam[i], out_.res[i]);
+        for (int i = 0; i < kw_; i++) printf("%22.14e  %22.14e\n", out_.lam[i], out_.res[i]);
         printf("\n%d iterations\n", it);
     }
     double maxres() {
         double m = out_.res[0];
-        for (int i = 1; i < k_; i++) m = out_.res[i] > m ? out_.res[i] : m;
+        for (int i = 1; i < kw_; i++) m = out_.res[i] > m ? out_.res[i] : m;
+        return m;
+    }
+    // residual threshold of column i (lobpcg_checkResiduals, lobpcg.c:265-294)
+    double threshold(int i) const {
+        return fabs(out_.lam[i]) * tol_.relative + tol_.absolute + DBL_EPSILON;
+    }
+
+  public:
+    // Eigenvalue error bounds of the wanted Ritz values.  For a window [a, b]
+    // of consecutive Ritz pairs, sum_{j in [a, b]} (theta_j - lambda_j) <=
+    // sum_{j in [a, b]} ||r_j||^2 / delta, delta the distance from the window
+    // to the nearest Ritz value outside it (the cluster form of the quadratic
+    // residual bound, exact for B = I; Ritz values stand in for the
+    // eigenvalues outside the window).  Column i takes the smallest bound over
+    // the windows that contain it, at most kWin pairs either side, with b <
+    // k - 1: the top pair has no neighbour above, and without guard columns
+    // the top wanted pair falls back to the gap below it.  Only columns with
+    // need[i] set are evaluated (the others get 0).
+    void eig_bounds(const std::vector<char> &need, std::vector<double> &est) const {
+        constexpr int kWin = 24;
+        const int k = k_;
+        const double *lam = out_.lam, *res = out_.res;
+        std::vector<double> S(k + 1, 0.0);  // prefix sums of ||r_j||^2
+        for (int j = 0; j < k; j++) S[j + 1] = S[j] + res[j] * res[j];
+        est.assign(kw_, 0.0);
+        for (int i = 0; i < kw_; i++) {
+            if (!need[i]) continue;
+            double best = HUGE_VAL;
+            for (int a = i; a >= 0 && a >= i - kWin; a--) {
+                const double below = a > 0 ? lam[a] - lam[a - 1] : HUGE_VAL;
+                for (int b = i; b <= k - 2 && b <= i + kWin; b++) {
+                    const double gap = fmin(below, lam[b + 1] - lam[b]);
+                    if (gap > 0.0) best = fmin(best, (S[b + 1] - S[a]) / gap);
+                }
+            }
+            if (best == HUGE_VAL && i == k - 1 && i > 0 && lam[i] > lam[i - 1])
+                best = res[i] * res[i] / (lam[i] - lam[i - 1]);
+            est[i] = best;
+        }
+    }
+    double eig_error_bound() const {
+        std::vector<char> need(kw_, 1);
+        std::vector<double> est;
+        eig_bounds(need, est);
+        double m = 0.0;
+        for (int i = 0; i < kw_; i++)
+            m = fmax(m, out_.lam[i] != 0.0 ? est[i] / fabs(out_.lam[i]) : est[i]);
         return m;
     }
 
+  private:
+
     int prec_;
     long n_;
-    int k_, ny_;
+    int k_, kw_, ny_;
     Op opA_;
     void *Ad_;
     Op opB_;
@@ -443,21 +496,39 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
     bool refining = false;
     std::vector<double> gm_hist;  // geometric mean of the k residual norms, per FP32 iteration
     int it = *it_io + 1;
+    std::vector<char> need(kw_);
+    std::vector<double> est;
+    // condition estimates of the pencils (lobpcg.m:488, 518, 912); the first
+    // entry is lobpcg.m's start value, large enough to avoid early restarts
+    std::vector<double> condh(1, -log10(DBL_EPSILON) / 2.0);
     for (; flag == 0 && it <= maxit; it++) {
         act.clear();
-        for (int i = 0; i < k; i++) {
-            mask[i] = out_.res[i] > fabs(out_.lam[i]) * tol_.relative + tol_.absolute + DBL_EPSILON;
-            if (mask[i]) act.push_back(i);
+        // wanted columns: residual above the threshold (lobpcg_checkResiduals)
+        // or, with eig_rtol, the eigenvalue error bound above eig_rtol |lambda|;
+        // a column never returns once it has left.  Guard columns are
+        // iterated while any wanted column is.
+        bool any = false;
+        for (int i = 0; i < kw_; i++) need[i] = mask[i] && !(out_.res[i] > threshold(i));
+        if (set.eig_rtol > 0.0) eig_bounds(need, est);
+        for (int i = 0; i < kw_; i++) {
+            bool m = out_.res[i] > threshold(i);
+            if (!m && set.eig_rtol > 0.0) m = est[i] > set.eig_rtol * fabs(out_.lam[i]);
+            mask[i] = mask[i] && m;
+            any = any || mask[i];
         }
+        for (int i = kw_; i < k; i++) mask[i] = any;
+        for (int i = 0; i < k; i++)
+            if (mask[i]) act.push_back(i);
         int nr = (int)act.size();
         if (fp32_phase) {
             // leave FP32 on convergence, near the tolerance, or on a stall
             // (LOBPCG residuals are not monotone: early on they rise for tens
-            // of iterations, so progress is judged window against window)
+            // of iterations, so progress is judged window against window);
+            // only the wanted columns count
             const double mr = maxres();
             double lg = 0.0;
-            for (int i = 0; i < k; i++) lg += log(out_.res[i] > 1e-300 ? out_.res[i] : 1e-300);
-            gm_hist.push_back(exp(lg / k));
+            for (int i = 0; i < kw_; i++) lg += log(out_.res[i] > 1e-300 ? out_.res[i] : 1e-300);
+            gm_hist.push_back(exp(lg / kw_));
             const int W = set.stall_iterations, h = (int)gm_hist.size();
             bool stalled = false;
             if (W > 0 && h >= 2 * W) {
@@ -469,9 +540,15 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
             // near the tolerance: every residual below switch_factor times
             // its own threshold (the mask's |lambda_i| rtol + atol)
             bool near = true;
-            for (int i = 0; i < k && near; i++)
+            for (int i = 0; i < kw_ && near; i++)
                 near = out_.res[i] < set.switch_factor *
                                          (fabs(out_.lam[i]) * tol_.relative + tol_.absolute);
+            // with eig_rtol, the bounds (quadratic in the residuals) within
+            // switch_factor^2 of theirs too
+            if (near && set.eig_rtol > 0.0)
+                for (int i = 0; i < kw_ && near; i++)
+                    near = !need[i] || est[i] <= set.switch_factor * set.switch_factor *
+                                                     set.eig_rtol * fabs(out_.lam[i]);
             (void)mr;
             if (nr == 0 || near || stalled) {
                 *switched = true;
@@ -575,11 +652,23 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
             }
             run_gram(gb);
         }
+        // lobpcg.m:846-863: restart (solve without P) when the condition
+        // estimate exceeds min(8, max(2, twice its mean over the last
+        // 10 + log(block size) iterations))
+        double cond_limit = 0.0;
+        if (set.cond_restart) {
+            const int w = 10 + (int)lround(log((double)nr));
+            const int hi = (int)condh.size() - 1, lo = hi - w > 0 ? hi - w : 0;
+            double mean = 0.0;
+            for (int t = lo; t <= hi; t++) mean += condh[t];
+            mean /= (hi - lo + 1);
+            cond_limit = fmin(8.0, fmax(2.0 * mean, 2.0));
+        }
         tm_->begin(kDense);
         // FP32 iterates: explicit X blocks, and P is dropped rather than the
         // phase abandoned when the basis has lost rank at FP32 accuracy
         rr_solve_implicit(GA_, GB_, ld3_, k, nr, preq, lam_, lam_new_, coef_, ld3_, st_,
-                          explicit_x, explicit_x);
+                          explicit_x, explicit_x, cond_limit);
         tm_->end();
         // P' = R C_R + P C_P and X' = X C_X + P' (likewise A-, B-blocks), in place
         UpdGroup g[3] = {};
@@ -644,6 +733,13 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
                 it--;
             }
             break;
+        }
+        if (set.cond_restart) {
+            condh.push_back(hs.condest);
+            if (hs.p_info == -2) {
+                cond_restarts_++;
+                if (verb_) printf("Restart on step %d as condestG %5.4e\n", it, hs.condest);
+            }
         }
         std::swap(lam_, lam_new_);
         const double *lam_h = tail.data() + (lam_ - tail_);
@@ -715,7 +811,8 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
 
 }  // namespace
 
-bool device_lobpcg_fits(int k) { return k >= 1 && 3 * k <= kMaxRR; }
+// the block (with the guard columns its settings ask for) fits the device solver
+bool device_lobpcg_fits(int k, int guard) { return k >= 1 && guard >= 0 && 3 * (k + guard) <= kMaxRR; }
 
 int device_lobpcg(mv_MultiVectorPtr X, void *Ad, Op opA, void *Bd, Op opB, void *Td, Op opT,
                   mv_MultiVectorPtr Y, lobpcg_Tolerance tol, int maxit, int verb, int *iters,
@@ -734,8 +831,10 @@ int device_lobpcg(mv_MultiVectorPtr X, void *Ad, Op opA, void *Bd, Op opB, void 
         if (verb) fprintf(stderr, "Error in LOBPCG:\n%s\n", blopex_amd_last_error());
         return WRONG_BLOCK_SIZE;
     }
-    if (3 * k > kMaxRR) {
-        set_error("device solver: block size %d above %d", k, kMaxRR / 3);
+    const int g = x->settings.guard_columns, kb = k + g;
+    if (g < 0 || 3 * kb > kMaxRR) {
+        set_error("device solver: block size %d with %d guard columns above %d", k, g,
+                  kMaxRR / 3);
         if (verb) fprintf(stderr, "Error in LOBPCG:\n%s\n", blopex_amd_last_error());
         return WRONG_BLOCK_SIZE;
     }
@@ -759,7 +858,27 @@ int device_lobpcg(mv_MultiVectorPtr X, void *Ad, Op opA, void *Bd, Op opB, void 
     }
     const hip_LobpcgSettings set = x->settings;
     memset(&x->stats, 0, sizeof(x->stats));
-    HostOut out{lam, lamh, res, resh, (int)lamh_gh, (int)resh_gh};
+    if (set.size_guard && x->size - ny < 5L * kb) {
+        // lobpcg.m:475-484 (disabled in lobpcg.c:672-687)
+        set_error("problem size %ld (minus %d constraints) below 5 x %d columns", x->size, ny, kb);
+        if (verb) fprintf(stderr, "Error in LOBPCG:\nProblem size too small compared to block size\n");
+        return PROBLEM_SIZE_TOO_SMALL;
+    }
+    // the phases iterate kb columns: the caller's k, then the guard columns
+    // (2 drand48() - 1 after srand48(guard_seed), column by column); the
+    // caller's arrays receive the first k
+    std::vector<double> lam_w(kb), res_w(kb);
+    HostOut out{lam_w.data(), lamh, res_w.data(), resh, (int)lamh_gh, (int)resh_gh};
+    void *Xw = x->data;
+    if (g > 0 || x->ld != x->size) {
+        Xw = scratch(kSlotXw, (size_t)x->size * kb * sizeof(double));
+        convert_block(x->size, k, x->data, x->ld, HIPMV_FP64, Xw, x->size, HIPMV_FP64);
+        if (g > 0) {
+            ColMap gm = colmap_identity(g);
+            for (int j = 0; j < g; j++) gm.idx[j] = k + j;
+            drand48_cols(HIPMV_FP64, x->size, set.guard_seed, Xw, x->size, gm);
+        }
+    }
     if (verb) {
         printf("\nSolving %s eigenvalue problem with%s preconditioning\n\n",
                opB ? "generalized" : "standard", opT ? "" : "out");
@@ -773,20 +892,32 @@ int device_lobpcg(mv_MultiVectorPtr X, void *Ad, Op opA, void *Bd, Op opB, void 
     bool switched = false;
     KTimer tm(set.profile != 0);
     if (set.mode == HIPLOBPCG_MIXED) {
-        Phase ph(HIPMV_FP32, x->size, k, ny, opA, Ad, opB, Bd, opT, Td, y, tol, verb, out, &tm);
-        flag = ph.run(x->data, &it, maxit, true, set, &switched);
+        Phase ph(HIPMV_FP32, x->size, kb, k, ny, opA, Ad, opB, Bd, opT, Td, y, tol, verb, out,
+                 &tm);
+        flag = ph.run(Xw, &it, maxit, true, set, &switched);
         x->stats.iterations_fp32 = it;
         x->stats.fp32_exit = ph.fp32_exit_;
+        x->stats.cond_restarts += ph.cond_restarts_;
     }
     if (set.mode != HIPLOBPCG_MIXED || (switched && flag == 0)) {
         const int it0 = it;
-        Phase ph(HIPMV_FP64, x->size, k, ny, opA, Ad, opB, Bd, opT, Td, y, tol, verb, out, &tm);
+        Phase ph(HIPMV_FP64, x->size, kb, k, ny, opA, Ad, opB, Bd, opT, Td, y, tol, verb, out,
+                 &tm);
         bool dummy;
-        flag = ph.run(x->data, &it, maxit, false, set, &dummy);
+        flag = ph.run(Xw, &it, maxit, false, set, &dummy);
         x->stats.iterations_fp64 = it - it0;
         x->stats.refine_iterations = ph.refine_done_;
         x->stats.final_rr = ph.final_rr_done_;
+        x->stats.cond_restarts += ph.cond_restarts_;
+        if (set.eig_rtol > 0.0) x->stats.eig_error_bound = ph.eig_error_bound();
     }
+    for (int i = 0; i < k; i++) {
+        lam[i] = lam_w[i];
+        res[i] = res_w[i];
+    }
+    if (Xw != x->data)
+        convert_block(x->size, k, Xw, x->size, HIPMV_FP64, x->data, x->ld, HIPMV_FP64);
+    x->stats.guard_columns = g;
     *iters = it;
     if (verb == 1) {
         printf("\nEigenvalue lambda       Residual              \n");

@@ -179,4 +179,14 @@ int blopex_amd_synchronize(void) {
 }
 
 const char *blopex_amd_last_error(void) { return bx::g_err; }
+
+int blopex_amd_release_workspace(void) {
+    if (hipStreamSynchronize(bx::stream()) != hipSuccess) return 1;
+    for (int i = 0; i < bx::kSlots; i++) {
+        if (bx::g_scratch[i]) hipFree(bx::g_scratch[i]);
+        bx::g_scratch[i] = nullptr;
+        bx::g_scratch_bytes[i] = 0;
+    }
+    return 0;
+}
 }

@@ -36,6 +36,13 @@ int blopex_amd_kernel_gram(int precision, long n, int lw, int qw, const double *
 int blopex_amd_kernel_rr(int k, int a, int p, const double *ga, const double *gb, int lda,
                          const double *lam_in, int explicit_x, int retry_without_p, int jacobi,
                          double *lam_out, double *coef, int *flags);
+/* The same step with the condition-estimate restart of lobpcg.m:846-863:
+   when log10 of the spread of B_n's Cholesky pivots + 1 (written to
+   *condest) exceeds cond_limit > 0, the step is solved again without P
+   (flags[2] = p_info = -2). */
+int blopex_amd_kernel_rr_cond(int k, int a, int p, const double *ga, const double *gb, int lda,
+                              const double *lam_in, int explicit_x, double cond_limit,
+                              double *lam_out, double *coef, int *flags, double *condest);
 
 /* For each lane l (0..63) and accumulator register r (0..3) of the 16x16x4
    MFMA used by the kernels (FP64 or FP32), the row and column of D it holds:

@@ -125,6 +125,9 @@ int blopex_amd_init(void);              /* optional: 0 on success */
 void *blopex_amd_stream(void);          /* the hipStream_t all work runs on */
 int blopex_amd_synchronize(void);       /* waits for the stream; 0 on success */
 const char *blopex_amd_last_error(void);/* last error message ("" if none) */
+/* frees the device workspace the library keeps between calls (the solver's
+   block pools, Gram partials, Rayleigh-Ritz buffers); 0 on success */
+int blopex_amd_release_workspace(void);
 
 /* ---- blocks ------------------------------------------------------------ */
 hip_Multi_Vector *hip_Multi_VectorCreate(long size, BlopexInt num_vectors, int precision);

@@ -21,27 +21,36 @@ from blopex_amd import problems as pb  # noqa: E402
 from oracle import pyoracle as po  # noqa: E402
 
 
-def cases():
+def cases(which="ref"):
+    """The fixture cases; X0 from the reference's drand48 (which="ref") or the
+    oracle's restatement of it (which="oracle": the same stream, pinned by
+    the fixture's first 64 values), so the GPU tests need no compiled reference."""
     # 20^3 constant 7-point, k=10, no B/T, drand48 seed 1, tol 1e-6/1e-50, maxit 200
     A = pb.laplacian_7pt(20, 20, 20)
-    yield "cfg1", dict(A=A, X0=po.drand48_block(A.n, 10, 1, which="ref"), maxit=200)
+    yield "cfg1", dict(A=A, X0=po.drand48_block(A.n, 10, 1, which=which), maxit=200)
     # stock serial driver: diag(1..40), k=5, maxit 100 (serial_driver.c:44-175)
     A = pb.dense_to_csr(np.diag(np.arange(1.0, 41.0)))
-    yield "diag40", dict(A=A, X0=po.drand48_block(40, 5, 1, which="ref"), maxit=100)
+    yield "diag40", dict(A=A, X0=po.drand48_block(40, 5, 1, which=which), maxit=100)
     # test_diag.m: diag(1..n), 3 constraints eye(n,3), T = inv(A), tol 1e-4, maxit 40
     n = 1000
     A = pb.dense_to_csr(np.diag(np.arange(1.0, n + 1)))
-    yield "testdiag", dict(A=A, X0=po.drand48_block(n, 3, 1, which="ref"), Y=np.eye(n, 3),
+    yield "testdiag", dict(A=A, X0=po.drand48_block(n, 3, 1, which=which), Y=np.eye(n, 3),
                            tdiag=1.0 / np.arange(1.0, n + 1), atol=1e-4, maxit=40)
     # generalized: the 20^3 A, B = diag(uniform[0.5, 1.5]) (seed 7), k = 64, no T; capped at 120
     # iterations (the reference runs on and breaks down at iteration 158)
     A = pb.laplacian_7pt(20, 20, 20)
-    yield "cfg3", dict(A=A, X0=po.drand48_block(A.n, 64, 1, which="ref"),
+    yield "cfg3", dict(A=A, X0=po.drand48_block(A.n, 64, 1, which=which),
                        bdiag=pb.uniform(7, A.n, 0.5, 1.5), maxit=120)
     # generalized + Jacobi on a small variable-coefficient grid
     A = pb.laplacian_7pt(12, 12, 12, sigma=1.0, seed=3)
-    yield "gen12", dict(A=A, X0=po.drand48_block(A.n, 8, 1, which="ref"),
+    yield "gen12", dict(A=A, X0=po.drand48_block(A.n, 8, 1, which=which),
                         bdiag=pb.uniform(5, A.n, 0.5, 1.5), tdiag=1.0 / A.diagonal(), maxit=300)
+    # the 128^3 workload's generator at 20^3: lognormal(0, 1) coefficients
+    # (seed 42), Jacobi, k = 32, tol 1e-6 (at 16^3 the reference breaks down
+    # at iteration 54)
+    A = pb.laplacian_7pt(20, 20, 20, sigma=1.0, seed=42)
+    yield "cfg2s", dict(A=A, X0=po.drand48_block(A.n, 32, 1, which=which),
+                        tdiag=1.0 / A.diagonal(), maxit=1000)
 
 
 REF = "/root/reference"

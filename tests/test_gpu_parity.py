@@ -286,11 +286,7 @@ def test_interpreter_slots_with_masks(prec):
 def _cases():
     from tests.golden.make_golden import cases
 
-    out = {}
-    for name, kw in cases():
-        kw["X0"] = po.drand48_block(kw["X0"].shape[0], kw["X0"].shape[1], 1)
-        out[name] = kw
-    return out
+    return dict(cases(which="oracle"))
 
 
 CASES = _cases()
