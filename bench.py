@@ -8,21 +8,28 @@ preconditioner; k = 32 smallest eigenpairs; X0 = 2*drand48()-1 after
 srand48(1), generated on the device; absolute residual tolerance 1e-6
 (relative 1e-50).
 
-A step is one full solve to tolerance in MIXED mode (FP32 blocks, FP64
-Gram accumulation / small dense / finish), starting from X0 already in HBM.
-`value` is its time-to-tolerance (s, lower is better).  The same solve in
-FP64 mode is timed once after the steps, and the JSON line reports the
-ratio.  The roofline object is the SpMM kernel the library picks for this
-matrix (the z-marching stencil kernel k_zm here; the diagonal-copy kernel
-k_dia or the CSR kernel for other matrices; HIP events around
-back-to-back launches on the library stream, k = 32 columns, FP32 -- the
-precision the mixed solve spends its iterations in).  Its `traffic` (HBM
-bytes per launch) is read from the committed rocprofv3 counter summary of
-`bench.py --roofline-only` (FETCH_SIZE and WRITE_SIZE passes scaled by a
-512 MiB copy); a profiled solve gives the per-kernel-class breakdown.  cpu_baseline is the reference's own
-lobpcg_solve_double (oracle/_ref) on this host, timed on a bounded sample.
+A step is one full solve in MIXED mode (FP32 blocks, FP64 Gram
+accumulation / small dense / finish), starting from X0 already in HBM,
+until every residual is below 1e-6 AND every eigenvalue's error bound
+(settings.eig_rtol) is below 1e-9 relative, then the final Rayleigh-Ritz
+step.  Steps cycle through six residual-replacement intervals, which perturb
+the FP32 trajectory: `value` is the mean time over them (s, lower is
+better).  After the steps: the FP64 oracle (FP64, error bound 1e-12), the
+same solve in FP64 (speed-up), the eigenvalue errors of every run against
+the oracle, the two modes stopped by the residual tolerance alone, and the
+solves with 8 guard columns.  The roofline object is the SpMM kernel the
+library picks for this matrix (the z-marching stencil kernel k_zm here; the
+diagonal-copy kernel k_dia or the CSR kernel for other matrices; HIP events
+around back-to-back launches on the library stream, k = 32 columns, FP32),
+priced at the CSR algorithmic bytes, at the bytes of the format it reads
+and at its measured traffic, with the rocprofv3 trace average beside the
+event time.  Its `traffic` (HBM bytes per launch) is read from the
+committed rocprofv3 counter summary of `bench.py --roofline-only`
+(FETCH_SIZE and WRITE_SIZE passes scaled by a 512 MiB copy).  cpu_baseline
+is the reference's own lobpcg_solve_double (oracle/_ref) on this host and
+this operator, one thread, two iterations.
 
-    python bench.py [--steps K] [--warmup W] [--n 128] [--k 32] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--n 128] [--k 32] [--guard G] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -39,6 +46,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 COUNTERS = os.path.join(ROOT, "profiles", "spmm_roofline_128cube_k32_counters.txt")
+TRACE = os.path.join(ROOT, "profiles", "spmm_roofline_128cube_k32_trace.txt")
+EIG_RTOL = 1e-9  # eigenvalue error bound of the headline solve, relative
 MFMA_SUMMARY = os.path.join(ROOT, "profiles", "lobpcg_mixed_128cube_k32_mfma.txt")
 GRAM_KERNEL = "k_gram<float, 6, 3, 64>"  # the Gram plan of the FP32 iterations (up to 192 columns)
 METRIC = ("time-to-residual 1e-6, k=32, 128³ Laplacian: mixed vs FP64 on GPU; SpMM GB/s")
@@ -108,29 +117,32 @@ def spmm_roofline(lib, dA, A, k, prec, reps=30):
     return ms, algo
 
 
-def cpu_baseline(A, n_side, k, seed_x=1):
-    """Reference lobpcg_solve_double (oracle/_ref) on a bounded sample: the
-    64^3 instance of the same operator family (lognormal(0,1), Jacobi, k=32),
-    2 iterations, 1 thread; scaled to the 128^3 size by n (work per iteration
-    is linear in n) and to time-to-tolerance by the GPU FP64 iteration count."""
-    from blopex_amd import problems as pb
+def cpu_baseline(A, k, seed_x=1):
+    """The reference's own lobpcg_solve_double (oracle/_ref) on this workload
+    (the same CSR operator, Jacobi T, k, X0), one thread, on a bounded sample:
+    two iterations.  The harness stamps CLOCK_MONOTONIC at every operator-A
+    call (the initial AX, then one per iteration), so the time between the
+    first and second iterations' calls is one full-width iteration."""
+    from threadpoolctl import threadpool_limits
+
     from oracle import pyoracle as po
 
     if not po.ref_available():
-        return None, None
-    side = min(64, n_side)
-    As = pb.laplacian_7pt(side, side, side, sigma=1.0, seed=42)
-    X0 = pb.drand48_block(As.n, k, seed_x)
-    os.environ.setdefault("OPENBLAS_NUM_THREADS", "1")
+        return None
+    X0 = po.drand48_block(A.n, k, seed_x)
     t0 = time.perf_counter()
-    po.solve(A=As, X0=X0, tdiag=1.0 / As.diagonal(), maxit=1, atol=1e-6, which="ref")
-    t1 = time.perf_counter()
-    po.solve(A=As, X0=X0, tdiag=1.0 / As.diagonal(), maxit=3, atol=1e-6, which="ref")
-    t3 = time.perf_counter()
-    per_it = ((t3 - t1) - (t1 - t0)) / 2.0  # 2 extra iterations
-    scale = (n_side ** 3) / As.n
-    return per_it * scale, f"reference lobpcg_solve_double, {side}^3 k={k} Jacobi, iterations 2-3 " \
-                           f"({(t3 - t1) + (t1 - t0):.1f} s of CPU work), scaled by n x{scale:.0f}"
+    with threadpool_limits(limits=1):
+        po.solve(A=A, X0=X0, tdiag=1.0 / A.diagonal(), maxit=2, atol=1e-6, which="ref")
+    wall = time.perf_counter() - t0
+    st = po.ref_operator_stamps()
+    if len(st) < 3:
+        return None
+    return {"s_per_iteration": float(st[2] - st[1]), "setup_s": float(st[1] - st[0]),
+            "sample_wall_s": wall,
+            "sample": f"reference lobpcg_solve_double on the same 128^3 operator, Jacobi, k={k}, "
+                      f"X0 = drand48 seed {seed_x}, maxit 2, 1 thread ({wall:.1f} s of CPU "
+                      f"work); one iteration = time between the operator-A calls of "
+                      f"iterations 1 and 2"}
 
 
 def run_batched(count, graph=True, maxit=2000, mode="mixed", compare_fp64=True):
@@ -201,15 +213,14 @@ def run_adjoint(N=64, mode="mixed"):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=6)
+    ap.add_argument("--guard", type=int, default=0, help="guard columns of the headline solve")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=128, help="grid side")
     ap.add_argument("--k", type=int, default=32)
     ap.add_argument("--maxit", type=int, default=6000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--no-spread", action="store_true",
-                    help="skip the five extra mixed solves with other refresh intervals")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the SpMM roofline launches and a device copy of known size "
                          "(the command the committed rocprofv3 summaries and counters use)")
@@ -289,15 +300,22 @@ def main():
         return
 
     if args.profile_only:
-        r = solve("mixed")
+        r = solve("mixed", eig_rtol=EIG_RTOL, final_rr=True, guard_columns=args.guard)
         log(f"profile-only mixed solve: flag {r.flag} it {r.iterations} {r.stats['seconds']:.3f}s")
         ms32, _ = spmm_roofline(lib, dA, A, k, L.HIPMV_FP32, reps=10)
         ms64, _ = spmm_roofline(lib, dA, A, k, L.HIPMV_FP64, reps=10)
         log(f"spmm fp32 {ms32 * 1e3:.1f} us  fp64 {ms64 * 1e3:.1f} us")
         return
 
+    # The headline solve: tolerance 1e-6 on every residual AND the
+    # eigenvalue error bound (settings.eig_rtol) at most 1e-9 |lambda_i|,
+    # then the final Rayleigh-Ritz step (X B-orthonormal, residuals of the
+    # returned X).  Steps cycle through residual-replacement intervals, which
+    # perturb the FP32 trajectory, so `value` is the mean over trajectories.
+    head = dict(eig_rtol=EIG_RTOL, final_rr=True, guard_columns=args.guard)
+    intervals = [20, 15, 25, 30, 35, 40]
     for w in range(args.warmup):
-        r = solve("mixed")
+        r = solve("mixed", **head)
         log(f"warmup {w}: flag {r.flag} iterations {r.iterations} "
             f"(fp32 {r.stats['iterations_fp32']}, fp64 {r.stats['iterations_fp64']}) "
             f"{r.stats['seconds']:.3f}s")
@@ -306,16 +324,16 @@ def main():
     t_all = time.perf_counter()
     for s in range(args.steps):
         t0 = time.perf_counter()
-        r = solve("mixed")
+        r = solve("mixed", refresh_interval=intervals[s % len(intervals)], **head)
         lib.blopex_amd_synchronize()
         times.append(time.perf_counter() - t0)
         results.append(r)
-        log(f"step {s}: flag {r.flag} iterations {r.iterations} "
-            f"(fp32 {r.stats['iterations_fp32']}, fp64 {r.stats['iterations_fp64']}) {times[-1]:.3f}s")
+        log(f"step {s}: refresh {intervals[s % len(intervals)]} flag {r.flag} iterations "
+            f"{r.iterations} (fp32 {r.stats['iterations_fp32']}, fp64 {r.stats['iterations_fp64']}) "
+            f"{times[-1]:.3f}s")
     lib.blopex_amd_synchronize()
     total = time.perf_counter() - t_all
     rm = results[-1]
-    lam_mixed = rm.lam.copy()
 
     out = {"metric": METRIC, "value": float(np.mean(times)), "unit": "s",
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -323,58 +341,81 @@ def main():
            "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
            "data": "synthetic (seeded lognormal coefficients, drand48 X0)",
            "config": {"workload": f"LOBPCG k={k}, {nside}^3 7-point variable-coefficient "
-                                  f"diffusion (lognormal sigma=1), Jacobi, tol 1e-6, mixed precision",
-                      "n": A.n, "nnz": A.nnz, "k": k, "tol": 1e-6, "mode": "mixed"},
-           "mixed": {"flag": rm.flag, "iterations": rm.iterations,
-                     "iterations_fp32": rm.stats["iterations_fp32"],
-                     "iterations_fp64": rm.stats["iterations_fp64"],
-                     "max_residual": float(np.max(rm.res))}}
-
-    if not args.no_spread:
-        # The iteration count moves with rounding (a cluster of six eigenvalues
-        # at the top of the block converges last): the same solve with other
-        # residual-replacement intervals perturbs the trajectory and shows the
-        # spread of the count at an unchanged cost per iteration.
-        sp = []
-        for ri in (15, 25, 30, 35, 40):
-            t0 = time.perf_counter()
-            rs = solve("mixed", refresh_interval=ri)
-            lib.blopex_amd_synchronize()
-            sp.append((ri, rs.iterations, time.perf_counter() - t0))
-            log(f"spread: refresh {ri}: iterations {rs.iterations} {sp[-1][2]:.3f}s")
-        import inspect
-        ri0 = inspect.signature(solve_device).parameters["refresh_interval"].default
-        sp.append((ri0, rm.iterations, float(np.mean(times))))
-        its_all = [v[1] for v in sp]
-        secs_all = [v[2] for v in sp]
-        out["mixed_spread"] = {"refresh_intervals": [v[0] for v in sp], "iterations": its_all,
-                               "seconds": secs_all, "mean_seconds": float(np.mean(secs_all)),
-                               "mean_iterations": float(np.mean(its_all)),
-                               "ms_per_iteration": 1e3 * float(np.sum(secs_all)) / float(np.sum(its_all))}
+                                  f"diffusion (lognormal sigma=1), Jacobi, residual tol 1e-6 and "
+                                  f"eigenvalue error bound {EIG_RTOL:g} relative, mixed precision",
+                      "n": A.n, "nnz": A.nnz, "k": k, "tol": 1e-6, "eig_rtol": EIG_RTOL,
+                      "guard_columns": args.guard, "final_rr": True, "mode": "mixed"},
+           "mixed": {"flag": [int(x.flag) for x in results],
+                     "refresh_intervals": [intervals[s % len(intervals)] for s in range(args.steps)],
+                     "seconds": times,
+                     "iterations": [int(x.iterations) for x in results],
+                     "iterations_fp32": [int(x.stats["iterations_fp32"]) for x in results],
+                     "iterations_fp64": [int(x.stats["iterations_fp64"]) for x in results],
+                     "ms_per_iteration": 1e3 * float(np.sum(times)) /
+                                         float(sum(x.iterations for x in results)),
+                     "max_residual": float(max(np.max(x.res) for x in results)),
+                     "eig_error_bound": float(max(x.stats["eig_error_bound"] for x in results))}}
 
     if not args.no_fp64:
+        # the FP64 oracle: FP64 with the eigenvalue error bound at 1e-12
         t0 = time.perf_counter()
-        r64 = solve("fp64")
+        rref = solve("fp64", eig_rtol=1e-12, final_rr=True, guard_columns=args.guard)
+        lib.blopex_amd_synchronize()
+        tref = time.perf_counter() - t0
+        ref = np.sort(rref.lam)
+        log(f"fp64 oracle (eig bound 1e-12): flag {rref.flag} iterations {rref.iterations} "
+            f"{tref:.3f}s bound {rref.stats['eig_error_bound']:.2e}")
+
+        def accuracy(lam):
+            lam = np.sort(lam)
+            near = np.array([np.min(np.abs(ref - v)) / abs(v) for v in lam])
+            return {"max_rel_err": float(np.max(np.abs(lam - ref) / np.abs(ref))),
+                    "unmatched": int(np.sum(near > 1e-6))}
+
+        t0 = time.perf_counter()
+        r64 = solve("fp64", **head)
         lib.blopex_amd_synchronize()
         t64 = time.perf_counter() - t0
         log(f"fp64: flag {r64.flag} iterations {r64.iterations} {t64:.3f}s")
+        acc_m = [accuracy(x.lam) for x in results]
         out["fp64"] = {"seconds": t64, "flag": r64.flag, "iterations": r64.iterations,
-                       "max_residual": float(np.max(r64.res))}
+                       "ms_per_iteration": 1e3 * t64 / max(r64.iterations, 1),
+                       "max_residual": float(np.max(r64.res)),
+                       "eig_error_bound": r64.stats["eig_error_bound"]}
         out["speedup_mixed_vs_fp64"] = t64 / out["value"]
-        out["eig_rel_diff_mixed_vs_fp64"] = float(np.max(np.abs(lam_mixed - r64.lam) / np.abs(r64.lam)))
-        # LOBPCG with k = 32 can converge to a different 32nd eigenpair in the
-        # two runs (an eigenvector barely present in X0); compare each mixed
-        # eigenvalue with the nearest FP64 one and count the unmatched
-        near = np.array([np.min(np.abs(r64.lam - v) / abs(v)) for v in lam_mixed])
-        out["eig_matched_mixed_vs_fp64"] = {
-            "max_rel_diff": float(np.max(near[near < 1e-4])) if np.any(near < 1e-4) else None,
-            "unmatched": int(np.sum(near >= 1e-4)),
-            "lam_mixed_only": [float(v) for v, d in zip(lam_mixed, near) if d >= 1e-4],
-            "lam_fp64_only": [float(v) for v in r64.lam
-                              if np.min(np.abs(lam_mixed - v) / abs(v)) >= 1e-4]}
+        out["accuracy_vs_fp64_oracle"] = {
+            "oracle": {"mode": "fp64", "eig_rtol": 1e-12, "seconds": tref, "flag": rref.flag,
+                       "iterations": rref.iterations,
+                       "eig_error_bound": rref.stats["eig_error_bound"]},
+            "mixed_max_rel_err": max(a["max_rel_err"] for a in acc_m),
+            "mixed_unmatched": max(a["unmatched"] for a in acc_m),
+            "fp64": accuracy(r64.lam)}
+        # the same two modes stopped by the residual tolerance alone (no
+        # eigenvalue bound, no final Rayleigh-Ritz): faster, and which 32
+        # pairs they return depends on the trajectory
+        plain = {}
+        for mode in ("mixed", "fp64"):
+            t0 = time.perf_counter()
+            rp = solve(mode)
+            lib.blopex_amd_synchronize()
+            plain[mode] = dict(seconds=time.perf_counter() - t0, iterations=rp.iterations,
+                               flag=rp.flag, **accuracy(rp.lam))
+            log(f"tolerance only, {mode}: {plain[mode]}")
+        out["tolerance_only"] = plain
+        if args.guard == 0:
+            # with 8 guard columns (40 iterated, 32 tested and returned)
+            g8 = {}
+            for mode in ("mixed", "fp64"):
+                t0 = time.perf_counter()
+                rg = solve(mode, eig_rtol=EIG_RTOL, final_rr=True, guard_columns=8)
+                lib.blopex_amd_synchronize()
+                g8[mode] = dict(seconds=time.perf_counter() - t0, iterations=rg.iterations,
+                                flag=rg.flag, **accuracy(rg.lam))
+                log(f"guard 8, {mode}: {g8[mode]}")
+            out["guard8"] = g8
 
     # per-class device time of one profiled mixed solve
-    rp = solve("mixed", profile=True)
+    rp = solve("mixed", profile=True, **head)
     st = rp.stats
     out["profile_mixed"] = {kk: st[kk] for kk in ("seconds", "seconds_spmm", "seconds_gram",
                                                   "seconds_update", "seconds_small_dense",
@@ -408,6 +449,17 @@ def main():
     # (the diagonal copy: no index streams), fewer than the CSR count
     out["roofline"]["format_bytes"] = {"bytes": algo32, "achieved": ach,
                                        "frac": ach / HBM_PEAK_GBS}
+    # the same launch priced at the measured HBM traffic, and the rocprofv3
+    # kernel-trace average of the committed summary beside the HIP-event time
+    if traffic:
+        out["roofline"]["traffic_rate"] = {"achieved": traffic / (ms32 * 1e-3) / 1e9,
+                                           "frac": traffic / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    from blopex_amd.profsum import read_trace_avg_us
+    tr_us = read_trace_avg_us(TRACE, ksym) if (nside, k) == (128, 32) else None
+    if tr_us:
+        out["roofline"]["trace_avg_us"] = tr_us
+        out["roofline"]["trace_frac_csr_bytes"] = csr_bytes / (tr_us * 1e-6) / 1e9 / HBM_PEAK_GBS
+        out["roofline"]["trace_source"] = os.path.relpath(TRACE, ROOT)
     out["spmm_fp64"] = {"achieved": algo64 / (ms64 * 1e-3) / 1e9, "avg_launch_us": ms64 * 1e3,
                         "algorithmic_bytes": algo64}
     # MFMA rate of the FP32 Gram kernel in the mixed solve, from the committed
@@ -420,12 +472,18 @@ def main():
                             "source": os.path.relpath(MFMA_SUMMARY, ROOT)}
 
     if not args.no_cpu:
-        per_it, sample = cpu_baseline(A, nside, k)
-        if per_it is not None:
+        cb = cpu_baseline(A, k)
+        if cb is not None:
             its = out.get("fp64", {}).get("iterations", rm.iterations)
-            out["cpu_baseline"] = {"value": per_it * its, "unit": "s", "cores": 1,
-                                   "kind": "reference", "seconds_per_iteration": per_it,
-                                   "sample": sample + f"; x {its} iterations (GPU FP64 count)"}
+            gpu_it = out.get("fp64", {}).get("ms_per_iteration")
+            out["cpu_baseline"] = {
+                "value": cb["s_per_iteration"], "unit": "s per full-width iteration (k=32)",
+                "iterations_per_s": 1.0 / cb["s_per_iteration"], "cores": 1, "kind": "reference",
+                "setup_s": cb["setup_s"], "sample": cb["sample"],
+                "gpu_fp64_ms_per_iteration": gpu_it,
+                "extrapolated_time_to_tolerance_s": cb["s_per_iteration"] * its,
+                "extrapolation": f"s per iteration x {its} iterations (the GPU FP64 count); "
+                                 f"an upper bound, later iterations have fewer active columns"}
     for o in (X, X0, dT, dA):
         o.close()
     if not args.no_batched:

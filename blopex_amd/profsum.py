@@ -148,6 +148,20 @@ def read_mfma(path: str, kernel: str) -> tuple[float, float] | None:
     return None
 
 
+def read_trace_avg_us(path: str, kernel: str) -> float | None:
+    """The `avg us` column of a kernel's row in a committed `stats` summary, or None."""
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.rsplit(None, 6)
+        if len(parts) == 7 and parts[0].strip() == kernel:
+            try:
+                return float(parts[2])
+            except ValueError:
+                return None
+    return None
+
+
 def read_traffic(path: str, kernel: str) -> float | None:
     """The `traffic <kernel> <bytes>` figure of a committed counter summary, or None."""
     if not os.path.exists(path):

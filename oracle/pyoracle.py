@@ -124,6 +124,18 @@ def ref_lib() -> ctypes.CDLL:
     return _load(REF_SO, "ref")
 
 
+def ref_operator_stamps() -> np.ndarray:
+    """CLOCK_MONOTONIC at each operator-A call of the last reference solve
+    (the initial AX, then one per iteration)."""
+    lib = ref_lib()
+    fn = lib.ref_operator_stamps
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    fn.restype = ctypes.c_int
+    out = np.zeros(64)
+    m = fn(out.ctypes.data, 64)
+    return out[:m]
+
+
 @dataclass
 class SolveResult:
     flag: int

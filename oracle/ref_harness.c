@@ -14,6 +14,7 @@
  */
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "interpreter.h"
 #include "lobpcg.h"
@@ -33,9 +34,29 @@ typedef struct {
     const double *d;
 } rh_diag;
 
+/* CLOCK_MONOTONIC at the start of each operator-A call of the last solve:
+   one call for the initial AX, then one per iteration (on R), so the
+   difference of consecutive stamps after the first is the time of one
+   iteration */
+#define RH_MAX_STAMPS 64
+static double rh_stamps[RH_MAX_STAMPS];
+static int rh_nstamps = 0;
+
+int ref_operator_stamps(double *out, int max)
+{
+    int m = rh_nstamps < max ? rh_nstamps : max;
+    for (int i = 0; i < m; i++) out[i] = rh_stamps[i];
+    return m;
+}
+
 static void rh_csr_apply(void *A_, void *X_, void *Y_)
 {
     rh_csr *A = (rh_csr *)A_;
+    if (rh_nstamps < RH_MAX_STAMPS) {
+        struct timespec ts;
+        clock_gettime(CLOCK_MONOTONIC, &ts);
+        rh_stamps[rh_nstamps++] = (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+    }
     serial_Multi_Vector *X = (serial_Multi_Vector *)X_, *Y = (serial_Multi_Vector *)Y_;
     for (int j = 0; j < X->num_active_vectors; j++) {
         const double *x = X->data + (size_t)X->active_indices[j] * X->size;
@@ -81,6 +102,7 @@ int ref_lobpcg_solve(long n, int k, double *Xdata, const int *a_rowptr, const in
                                BlopexInt *))dsygv_ptr;
     tol.absolute = atol;
     tol.relative = rtol;
+    rh_nstamps = 0;
 
     /* wrap caller memory: the serial multivector does not own it */
     serial_Multi_Vector *x = serial_Multi_VectorCreate((BlopexInt)n, k);
