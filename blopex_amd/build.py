@@ -69,7 +69,25 @@ def build(force: bool = False, jobs: int = 8) -> str:
         if r.returncode != 0:
             raise RuntimeError(f"link failed:\n{r.stderr}")
         os.replace(tmp, LIB)
+    build_driver()
     return LIB
+
+
+DRIVER_SRC = os.path.join(HERE, "driver", "hip_driver.c")
+DRIVER = os.path.join(HERE, "hip_driver")
+
+
+def build_driver(out: str = DRIVER) -> str:
+    """The C caller (driver/hip_driver.c): plain C99 against include/*.h,
+    linked to libblopex_amd.so (found next to the binary at run time)."""
+    cmd = [os.environ.get("CC", "gcc"), "-std=c99", "-O2", "-Wall", "-Wextra", "-Werror",
+           "-I", os.path.join(ROOT, "include"), DRIVER_SRC, "-L", HERE, "-lblopex_amd",
+           "-Wl,-rpath," + os.path.dirname(os.path.abspath(LIB)), "-Wl,-rpath,$ORIGIN",
+           "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed:\n{r.stderr}")
+    return out
 
 
 if __name__ == "__main__":

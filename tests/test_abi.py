@@ -111,3 +111,18 @@ def test_complex_solver_requires_zlapack(lib):
                                  None, None, 0)
     assert r == -1 and it.value == 0
     assert "zpotrf" in L.last_error()
+
+
+def test_c_driver_compiles_and_links(tmp_path):
+    """The C caller (blopex_amd/driver/hip_driver.c, the serial driver's
+    pattern) compiles as C99 against include/*.h with -Wall -Wextra -Werror
+    and links to libblopex_amd.so, every symbol resolved (no GPU call)."""
+    import subprocess
+
+    from blopex_amd import build as b
+
+    exe = b.build_driver(str(tmp_path / "hip_driver"))
+    out = subprocess.run(["ldd", "-r", exe], capture_output=True, text=True)
+    assert out.returncode == 0
+    assert "libblopex_amd.so" in out.stdout
+    assert "undefined symbol" not in out.stdout + out.stderr

@@ -390,6 +390,34 @@ def test_generic_path_through_hip_interpreter(golden):
     np.testing.assert_allclose(Xo.T @ Xo, np.eye(k), atol=1e-7)
 
 
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_c_driver_matches_reference(golden, mode):
+    """The compiled C caller (blopex_amd/hip_driver, built from
+    driver/hip_driver.c in the serial driver's pattern) on the 20^3
+    Laplacian, k = 10: the reference's flag and eigenvalues (1e-8), and in
+    FP64 its iteration count within max(2, 5%)."""
+    import os
+    import subprocess
+
+    say(f"C driver, 20^3 Laplacian, k=10, {mode}")
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "blopex_amd", "hip_driver")
+    out = subprocess.run([exe, "20", "20", "20", "10", "200", "1e-6", mode],
+                         capture_output=True, text=True, timeout=120)
+    say(f"  {out.stdout.splitlines()[0] if out.stdout else out.stderr}")
+    lines = out.stdout.split("\n")
+    head = lines[0].split()
+    flag, its = int(head[1]), int(head[3])
+    lam = np.array([float(l.split()[0]) for l in lines[1:11]])
+    res = np.array([float(l.split()[1]) for l in lines[1:11]])
+    assert out.returncode == 0 and flag == int(golden["cfg1_flag"]) == 0
+    ref_it = int(golden["cfg1_iters"])
+    if mode == "fp64":
+        assert abs(its - ref_it) <= max(2, ref_it // 20)
+    assert relerr(lam, golden["cfg1_lam"]) < 1e-8
+    assert np.all(res <= 1e-6)
+
+
 def test_device_exit_codes():
     say("device exit codes")
     A = pb.laplacian_7pt(6, 6, 6)
