@@ -119,6 +119,8 @@ _SIGS = {
     "hip_Multi_VectorSetRandomValues": (c_int, [P(HipMultiVector), c_int]),
     "hip_Multi_VectorSetSettings": (c_int, [P(HipMultiVector), P(Settings)]),
     "hip_LobpcgSettingsDefault": (None, [P(Settings)]),
+    "hip_Multi_VectorInnerProdNoAlloc": (c_int, [P(HipMultiVector), P(HipMultiVector), c_int,
+                                                 c_int, c_int, c_void_p]),
     "hip_Multi_VectorGetStats": (c_int, [P(HipMultiVector), P(Stats)]),
     "HipSetupInterpreter": (c_int, [P(Interpreter)]),
     "hip_CSRMatrixCreate": (P(CSRMatrix), [c_long, c_long, c_void_p, c_void_p, c_void_p]),

@@ -406,15 +406,33 @@ void convert_block(long n, int k, const void *src, long lds, int prec_s, void *d
     BX_HIP(hipGetLastError());
 }
 
+// partial tiles gram_cols' grid writes for these shapes
+static unsigned gram_parts(long n, int ntiles) {
+    size_t blocks = ((size_t)n + kGRows - 1) / kGRows;
+    size_t cap = (size_t)num_cus() * 4 / (size_t)ntiles + 1;
+    return (unsigned)(blocks < 1 ? 1 : (blocks > cap ? cap : blocks));
+}
+
 void gram_cols(int prec, long n, const void *x, long ldx, const ColMap &xm, const void *y,
                long ldy, const ColMap &ym, double *g, long ldg) {
     if (xm.n == 0 || ym.n == 0) return;
+    const int ntiles = ((xm.n + kGT - 1) / kGT) * ((ym.n + kGT - 1) / kGT);
+    const size_t words = (size_t)gram_parts(n, ntiles) * ntiles * kGT * kGT;
+    gram_cols_into(prec, n, x, ldx, xm, y, ldy, ym, g, ldg,
+                   (double *)scratch(0, words * sizeof(double)), words);
+}
+
+void gram_cols_into(int prec, long n, const void *x, long ldx, const ColMap &xm, const void *y,
+                    long ldy, const ColMap &ym, double *g, long ldg, double *part,
+                    size_t part_words) {
+    if (xm.n == 0 || ym.n == 0) return;
     const int ntx = (xm.n + kGT - 1) / kGT, nty = (ym.n + kGT - 1) / kGT;
     const int ntiles = ntx * nty;
-    size_t blocks = ((size_t)n + kGRows - 1) / kGRows;
-    size_t cap = (size_t)num_cus() * 4 / (size_t)ntiles + 1;
-    unsigned nparts = (unsigned)(blocks < 1 ? 1 : (blocks > cap ? cap : blocks));
-    double *part = (double *)scratch(0, (size_t)nparts * ntiles * kGT * kGT * sizeof(double));
+    unsigned nparts = gram_parts(n, ntiles);
+    const size_t fit = part_words / ((size_t)ntiles * kGT * kGT);
+    BX_REQUIRE(fit >= 1, "gram_cols_into: workspace of %zu words below one partial set",
+               part_words);
+    if (nparts > fit) nparts = (unsigned)fit;
     dim3 grid(nparts, ntiles);
     if (prec == HIPMV_FP32)
         hipLaunchKernelGGL(k_gram<float>, grid, dim3(kThreads), 0, stream(), n, (const float *)x,

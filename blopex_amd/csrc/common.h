@@ -42,6 +42,9 @@ int num_cus();
 // (blopex_amd_release_workspace frees it); slot s is independent of the
 // other slots.  Contents are undefined on return.
 void *scratch(int slot, size_t bytes);
+// Device address of the fixed array of hip_Multi_VectorInnerProdNoAlloc on
+// the library's device, looked up once per device (nullptr on failure).
+double *noalloc_setup();
 // Pinned host staging buffer of at least `bytes`.
 void *pinned(int slot, size_t bytes);
 // Small host->device copy ordered on the stream; the host array may be reused

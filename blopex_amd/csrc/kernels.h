@@ -25,6 +25,12 @@ void convert_block(long n, int k, const void *src, long lds, int prec_s, void *d
 // g (device FP64, h x w, leading dim ldg) = x(:, xm)' * y(:, ym), FP64 accumulation
 void gram_cols(int prec, long n, const void *x, long ldx, const ColMap &xm, const void *y,
                long ldy, const ColMap &ym, double *g, long ldg);
+// The same with the caller's workspace for the per-workgroup partial tiles
+// (part_words doubles; fewer partial sets when it holds fewer than
+// gram_cols would use).  Allocates nothing.
+void gram_cols_into(int prec, long n, const void *x, long ldx, const ColMap &xm, const void *y,
+                    long ldy, const ColMap &ym, double *g, long ldg, double *part,
+                    size_t part_words);
 // d (device FP64, xm.n) = diag(x(:, xm)' y(:, ym))
 void diagdot_cols(int prec, long n, const void *x, long ldx, const ColMap &xm, const void *y,
                   long ldy, const ColMap &ym, double *d);

@@ -164,7 +164,9 @@ extern "C" {
 
 int blopex_amd_init(void) {
     bx::stream();
-    return 0;
+    // the device address of the no-allocation entry point's array, looked up
+    // once per device here rather than at its first call
+    return bx::noalloc_setup() ? 0 : 1;
 }
 
 void *blopex_amd_stream(void) { return (void *)bx::stream(); }

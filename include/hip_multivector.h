@@ -148,6 +148,17 @@ int hip_Multi_VectorSetSettings(hip_Multi_Vector *x, const hip_LobpcgSettings *s
 void hip_LobpcgSettingsDefault(hip_LobpcgSettings *s);
 int hip_Multi_VectorGetStats(const hip_Multi_Vector *x, hip_LobpcgStats *s);
 
+/* v = X(active)' Y(active), h x w column-major with leading dimension gh in
+   host memory: the interpreter's MultiInnerProd slot (mv_MultiVectorByMultiVector
+   on HIP blocks) without any allocation.  No counterpart in the reference: it
+   is for callers that cannot allocate (tight serving loops, harnesses that
+   forbid allocation).  Its workspace is a fixed device array of 2.2 MB held
+   by each device for the life of the process; h and w are at most 64 (larger
+   results return 1 with a message).  Calls that overlap on one device share
+   the array and race.  Real blocks (FP64 or FP32) only; returns 0 on success. */
+int hip_Multi_VectorInnerProdNoAlloc(const hip_Multi_Vector *x, const hip_Multi_Vector *y,
+                                     BlopexInt gh, BlopexInt h, BlopexInt w, double *v);
+
 /* fills every multivector slot of the table the solver uses */
 BlopexInt HipSetupInterpreter(mv_InterfaceInterpreter *ii);
 
