@@ -81,7 +81,7 @@ class HipEvents:
         return ms.value
 
 
-def spmm_roofline(lib, dA, A, k, prec, reps=30):
+def spmm_roofline(lib, dA, A, k, prec, reps=30, force_csr=False):
     """Average duration of the SpMM kernel (HipCSRMatMultiVec) over `reps` back-to-back launches;
     algorithmic bytes of the format the kernel reads (the z-marching and diagonal kernels read
     the diagonal copy, the CSR kernel values, columns and row pointers)."""
@@ -91,6 +91,10 @@ def spmm_roofline(lib, dA, A, k, prec, reps=30):
     X = DeviceBlock(A.n, k, prec)
     X.set_random(3)
     Y = DeviceBlock(A.n, k, prec)
+    c = dA.ptr.contents
+    nd = c.ndiag
+    if force_csr:  # the general CSR kernel on the same matrix (no diagonal copy)
+        c.ndiag = 0
     lib.HipCSRMatMultiVec.argtypes = [ctypes.c_void_p] * 3
     call = lambda: lib.HipCSRMatMultiVec(ctypes.cast(dA.ptr, ctypes.c_void_p),
                                          ctypes.cast(X.ptr, ctypes.c_void_p),
@@ -106,9 +110,9 @@ def spmm_roofline(lib, dA, A, k, prec, reps=30):
         call()
     ev.record(b, s)
     ms = ev.elapsed_ms(a, b) / reps
+    c.ndiag = nd
     es = 4 if prec == L.HIPMV_FP32 else 8
-    nd = dA.ptr.contents.ndiag
-    if nd:  # diagonal copy: nd value streams, no index stream
+    if nd and not force_csr:  # diagonal copy: nd value streams, no index stream
         algo = nd * dA.ptr.contents.diag_ld * es + 2 * A.n * k * es
     else:  # CSR: values, column indices, row pointers
         algo = A.nnz * (es + 4) + (A.n + 1) * 4 + 2 * A.n * k * es
@@ -284,8 +288,10 @@ def main():
     if args.roofline_only:
         ms32, algo32 = spmm_roofline(lib, dA, A, k, L.HIPMV_FP32)
         ms64, algo64 = spmm_roofline(lib, dA, A, k, L.HIPMV_FP64)
+        msc, algoc = spmm_roofline(lib, dA, A, k, L.HIPMV_FP32, force_csr=True)
         log(f"spmm fp32 {ms32 * 1e3:.1f} us ({algo32 / ms32 / 1e6:.0f} GB/s)  "
-            f"fp64 {ms64 * 1e3:.1f} us ({algo64 / ms64 / 1e6:.0f} GB/s)")
+            f"fp64 {ms64 * 1e3:.1f} us ({algo64 / ms64 / 1e6:.0f} GB/s)  "
+            f"general CSR fp32 {msc * 1e3:.1f} us ({algoc / msc / 1e6:.0f} GB/s)")
         # reference for the counters: one device-to-device copy of 512 MiB
         nb = 512 << 20
         src, dst = ctypes.c_void_p(), ctypes.c_void_p()
@@ -462,6 +468,20 @@ def main():
         out["roofline"]["trace_source"] = os.path.relpath(TRACE, ROOT)
     out["spmm_fp64"] = {"achieved": algo64 / (ms64 * 1e-3) / 1e9, "avg_launch_us": ms64 * 1e3,
                         "algorithmic_bytes": algo64}
+    # the general CSR kernel (k_spmm<float, 8>: matrix slices staged through
+    # LDS, thread per row) on the same matrix, for matrices without a
+    # diagonal copy
+    msc, algoc = spmm_roofline(lib, dA, A, k, L.HIPMV_FP32, force_csr=True)
+    out["csr_general"] = {"kernel": "k_spmm<float, 8>", "avg_launch_us": msc * 1e3,
+                          "algorithmic_bytes": algoc,
+                          "achieved": algoc / (msc * 1e-3) / 1e9,
+                          "frac": algoc / (msc * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    trc = read_trace_avg_us(TRACE, "k_spmm<float, 8>") if (nside, k) == (128, 32) else None
+    if trc:
+        out["csr_general"]["trace_avg_us"] = trc
+    tfc = read_traffic(COUNTERS, "k_spmm<float, 8>") if (nside, k) == (128, 32) else None
+    if tfc:
+        out["csr_general"]["traffic"] = tfc
     # MFMA rate of the FP32 Gram kernel in the mixed solve, from the committed
     # rocprofv3 summary (MOPS counters x 512 over the kernel-trace durations)
     from blopex_amd.profsum import read_mfma

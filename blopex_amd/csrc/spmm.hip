@@ -23,10 +23,11 @@
 // neighbours outside the grid (zero coefficients) enter as zeros, so the
 // bits equal those of the diagonal and CSR kernels.
 //
-// General CSR: one thread per row.  A row's (column, value) pairs are loaded once into
-// registers and reused for every block column, so the matrix streams from
-// HBM once per product; for the stencil matrices the X reads of neighbouring
-// lanes hit consecutive addresses of the same column.  Blocks are renumbered
+// General CSR: one thread per row.  A row group's (column, value) pairs are
+// staged through LDS with coalesced loads, then each row's pairs sit in
+// registers for every block column, so the matrix streams from HBM once per
+// product; for the stencil matrices the X reads of neighbouring lanes hit
+// consecutive addresses of the same column.  Blocks are renumbered
 // so that each of the 8 XCDs walks one contiguous range of rows (workgroups
 // are dealt to XCDs round-robin): the X rows a workgroup needs from the
 // previous/next grid plane were then fetched into that XCD's L2 a short time
@@ -45,15 +46,37 @@ __device__ __forceinline__ long xcd_block(unsigned b, unsigned nb) {
     return (long)(b % kXcds) * (nb / kXcds) + b / kXcds;
 }
 
+// General CSR: a workgroup owns kSpThreads consecutive rows.  Their (column,
+// value) pairs are one contiguous range of the CSR arrays, staged into LDS
+// with coalesced non-temporal loads (they stream past L2, which is left to
+// X); each thread then takes its row's pairs from LDS into registers and
+// reuses them for every block column.  Y streams out non-temporally.
+constexpr int kSpSeg = kSpThreads * 10;  // staged pairs per row group (rows of <= 10 on average)
+
 template <typename T, int MAXE>
 __global__ void __launch_bounds__(kSpThreads) k_spmm(long n, const int *__restrict__ rowptr,
                                                      const int *__restrict__ colind,
                                                      const T *__restrict__ val, const T *x, long ldx,
                                                      ColMap xm, T *y, long ldy, ColMap ym) {
-    const long r = xcd_block(blockIdx.x, gridDim.x) * kSpThreads + threadIdx.x;
+    __shared__ int sc[kSpSeg];
+    __shared__ T sv[kSpSeg];
+    const long rb = xcd_block(blockIdx.x, gridDim.x) * kSpThreads;
+    if (rb >= n) return;  // uniform over the workgroup
+    const long r = rb + threadIdx.x;
+    const long rend = rb + kSpThreads < n ? rb + kSpThreads : n;
+    const int s0 = rowptr[rb], s1 = rowptr[rend];
+    const bool staged = s1 - s0 <= kSpSeg;  // uniform
+    if (staged)
+        for (int e = threadIdx.x; e < s1 - s0; e += kSpThreads) {
+            sc[e] = __builtin_nontemporal_load(colind + s0 + e);
+            sv[e] = __builtin_nontemporal_load(val + s0 + e);
+        }
+    __syncthreads();
     if (r >= n) return;
     const int e0 = rowptr[r], e1 = rowptr[r + 1];
     const int len = e1 - e0;
+    auto col = [&](int e) { return staged ? sc[e - s0] : colind[e]; };
+    auto value = [&](int e) { return staged ? sv[e - s0] : val[e]; };
     if (len <= MAXE) {
         // short rows padded with (r, 0): fma(0, x, s) == s, so the padding
         // changes no bit and the inner loop has no branch
@@ -62,22 +85,24 @@ __global__ void __launch_bounds__(kSpThreads) k_spmm(long n, const int *__restri
 #pragma unroll
         for (int e = 0; e < MAXE; e++) {
             const bool ok = e < len;
-            ci[e] = ok ? colind[e0 + e] : (int)r;
-            v[e] = ok ? val[e0 + e] : T(0);
+            ci[e] = ok ? col(e0 + e) : (int)r;
+            v[e] = ok ? value(e0 + e) : T(0);
         }
+        // one block column at a time: the gathers of a wave then touch one
+        // column's lines, which the neighbouring rows reuse from L1
         for (int j = 0; j < xm.n; j++) {
             const T *xa = x + (size_t)xm.idx[j] * ldx;
             T sa = T(0);
 #pragma unroll
             for (int e = 0; e < MAXE; e++) sa = fma(v[e], xa[ci[e]], sa);
-            y[(size_t)ym.idx[j] * ldy + r] = sa;
+            __builtin_nontemporal_store(sa, y + (size_t)ym.idx[j] * ldy + r);
         }
     } else {
         for (int j = 0; j < xm.n; j++) {
             const T *xa = x + (size_t)xm.idx[j] * ldx;
             T s = T(0);
-            for (int e = e0; e < e1; e++) s = fma(val[e], xa[colind[e]], s);
-            y[(size_t)ym.idx[j] * ldy + r] = s;
+            for (int e = e0; e < e1; e++) s = fma(value(e), xa[col(e)], s);
+            __builtin_nontemporal_store(s, y + (size_t)ym.idx[j] * ldy + r);
         }
     }
 }
