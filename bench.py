@@ -181,6 +181,43 @@ def run_batched(count, graph=True, maxit=2000, mode="mixed", compare_fp64=True):
     return out
 
 
+def run_config3():
+    """The generalized workload: the 20^3 constant 7-point Laplacian (n =
+    8000), B = diag(uniform[0.5, 1.5]) (seed 7), k = 64 (Rayleigh-Ritz
+    pencils up to 192 x 192), no preconditioner, X0 = drand48 seed 1, tol
+    1e-6, with the condition-estimate restart and the final Rayleigh-Ritz
+    step; FP64 and MIXED, eigenvalues against scipy shift-invert."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+
+    from blopex_amd import problems as pb
+    from blopex_amd.lobpcg import lobpcg
+    from oracle import pyoracle as po
+
+    A = pb.laplacian_7pt(20, 20, 20)
+    Bd = pb.uniform(7, A.n, 0.5, 1.5)
+    X0 = po.drand48_block(A.n, 64, 1)
+    true = np.sort(sla.eigsh(A.to_scipy(), k=70, M=sp.diags(Bd), sigma=0.0, which="LM",
+                             return_eigenvectors=False))[:64]
+    out = {"workload": "20^3 7-point Laplacian, B = diag(uniform[0.5,1.5]), k=64, no T, tol 1e-6, "
+                       "condition-estimate restart, final Rayleigh-Ritz"}
+    lam = {}
+    for mode in ("fp64", "mixed"):
+        lobpcg(X0, A, B=Bd, tol=1e-6, maxit=1000, mode=mode, final_rr=True, cond_restart=True)
+        t = time.perf_counter()
+        r = lobpcg(X0, A, B=Bd, tol=1e-6, maxit=1000, mode=mode, final_rr=True, cond_restart=True)
+        dt = time.perf_counter() - t
+        lam[mode] = np.sort(r.lam)
+        out[mode] = {"seconds": dt, "solver_seconds": r.stats["seconds"], "flag": r.flag,
+                     "iterations": r.iterations, "iterations_fp32": r.stats["iterations_fp32"],
+                     "converged": int(np.sum(r.res < 1e-6)), "restarts": r.stats["cond_restarts"],
+                     "max_rel_err_vs_shift_invert": float(np.max(np.abs(lam[mode] - true) / true))}
+    out["speedup_mixed_vs_fp64"] = out["fp64"]["solver_seconds"] / out["mixed"]["solver_seconds"]
+    out["max_rel_diff_mixed_vs_fp64"] = float(np.max(np.abs(lam["mixed"] - lam["fp64"]) /
+                                                     lam["fp64"]))
+    return out
+
+
 def run_adjoint(N=64, mode="mixed"):
     """The differentiation workload: the 7-point lognormal(0, 1) operator on an
     N^3 grid, loss = sum of the 8 smallest eigenvalues + ||x_1||^2 on the
@@ -238,6 +275,8 @@ def main():
     ap.add_argument("--batched-no-graph", action="store_true",
                     help="batched workload with eager launches instead of graph replays")
     ap.add_argument("--batched-mode", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--no-config3", action="store_true",
+                    help="skip the generalized k=64 workload (20^3, diagonal B)")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the differentiation workload (64^3 forward solve + adjoint)")
     ap.add_argument("--adjoint-only", action="store_true",
@@ -510,6 +549,9 @@ def main():
         out["batched"] = run_batched(args.batched_count, not args.batched_no_graph,
                                      mode=args.batched_mode)
         log(f"batched: {out['batched']}")
+    if not args.no_config3:
+        out["config3"] = run_config3()
+        log(f"config3: {out['config3']}")
     if not args.no_adjoint:
         out["adjoint"] = run_adjoint()
         log(f"adjoint: {out['adjoint']}")

@@ -620,7 +620,14 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
         // steepest-descent restart of lobpcg.m:845-860)
         double min_active = HUGE_VAL;
         for (int t = 0; t < nr; t++) min_active = fmin(min_active, out_.res[act[t]]);
-        const int explicit_x = prec_ == HIPMV_FP32 || min_active <= kExplicitGramRes ? 1 : 0;
+        // likewise once every wanted residual meets its threshold and only
+        // the eigenvalue error bound (eig_rtol) keeps the iteration going:
+        // the residuals must then fall well below the tolerance, which the
+        // implicit blocks do not sustain
+        bool bound_only = set.eig_rtol > 0.0;
+        for (int i = 0; i < kw_ && bound_only; i++) bound_only = !(out_.res[i] > threshold(i));
+        const int explicit_x =
+            prec_ == HIPMV_FP32 || min_active <= kExplicitGramRes || bound_only ? 1 : 0;
         void *BRp = noB_ ? R_ : BR_, *BPp = noB_ ? P_ : BP_;
         BXp = noB_ ? X_ : BX_;
         {
@@ -667,8 +674,11 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
         tm_->begin(kDense);
         // FP32 iterates: explicit X blocks, and P is dropped rather than the
         // phase abandoned when the basis has lost rank at FP32 accuracy
+        // with cond_restart (lobpcg.m's remedies) a pencil that is not
+        // positive definite is also solved again without P, in FP64 too;
+        // otherwise FP64 stops there as lobpcg.c does (dsygv failure)
         rr_solve_implicit(GA_, GB_, ld3_, k, nr, preq, lam_, lam_new_, coef_, ld3_, st_,
-                          explicit_x, explicit_x, cond_limit);
+                          explicit_x, explicit_x || set.cond_restart, cond_limit);
         tm_->end();
         // P' = R C_R + P C_P and X' = X C_X + P' (likewise A-, B-blocks), in place
         UpdGroup g[3] = {};

@@ -71,9 +71,10 @@ typedef struct hip_LobpcgSettings {
                                  eig_rtol * |lambda_i| (default 0: residuals only) */
     int cond_restart;         /* 1: when the condition estimate of the Rayleigh-Ritz pencil's
                                  B part jumps (log10 of its Cholesky pivot spread + 1 above
-                                 min(8, max(2, twice its recent mean))), that step is solved
-                                 again without P, a steepest-descent restart (lobpcg.m:
-                                 846-863; default 0) */
+                                 min(8, max(2, twice its recent mean))), or the pencil is not
+                                 positive definite, that step is solved again without P, a
+                                 steepest-descent restart (lobpcg.m:846-863); without it FP64
+                                 stops with -1 on such a pencil, as lobpcg.c does (default 0) */
     int size_guard;           /* 1: return PROBLEM_SIZE_TOO_SMALL when n - sizeY < 5 (k + g)
                                  (lobpcg.m:475-484; lobpcg.c:672-687 has it disabled;
                                  default 0) */
