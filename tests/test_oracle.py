@@ -145,3 +145,21 @@ def test_stencil_conventions():
     # 128^3 nnz formula 7n - 6*128^2 at a small size: 7n - 2(nx ny + ny nz + nx nz)
     A3 = pb.laplacian_7pt(8, 8, 8)
     assert A3.nnz == 7 * 512 - 6 * 64
+
+
+@pytest.mark.parametrize("n,ld", [(1, 1), (7, 9), (24, 24)])
+def test_oracle_upper_inv_against_numpy(n, ld):
+    """oracle_upper_inv (the restatement of utilities_FortranMatrixUpperInv,
+    fortran_matrix.c:1368-1416) inverts the upper triangle in place: equal to
+    numpy's inverse of the triangular matrix to 1e-12, entries below the
+    diagonal and padding rows of the leading dimension untouched."""
+    rng = np.random.default_rng(n)
+    U = np.triu(rng.standard_normal((n, n))) + np.diag(2.0 + rng.random(n))
+    buf = np.full((ld, n), 7.0, order="F")
+    buf[:n, :n] = U + np.tril(np.full((n, n), 5.0), -1)
+    po.oracle_lib().oracle_upper_inv(n, ld, buf.ctypes.data)
+    inv = np.linalg.inv(U)
+    got = np.triu(buf[:n, :n])
+    np.testing.assert_allclose(got, inv, rtol=1e-12, atol=1e-13 * np.abs(inv).max())
+    assert np.all(np.tril(buf[:n, :n], -1)[np.tril_indices(n, -1)] == 5.0)
+    assert np.all(buf[n:, :] == 7.0)
