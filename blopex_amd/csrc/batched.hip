@@ -12,7 +12,8 @@
 //   k_bresid     R = T (AX - X diag(lambda)) and per-chunk partial norms
 //   k_bspmm      AR = A R over the active chunks of the stacked CSR matrix
 //   k_bgram      per-chunk partial Gram blocks [X R P]' [AX AR AP] and
-//                [X R P]' [X R P] (upper triangles, the problem's columns only)
+//                [X R P]' [X R P] (upper triangles, the problem's columns only);
+//                FP32 blocks take k_bgram_mfma (FP32 MFMA tiles)
 //   k_bgram_sum  per-problem fixed-order sums of the partials
 //   k_rr_tri_batched  one workgroup per problem: the single-problem path's
 //                tridiagonal Rayleigh-Ritz solver (rr_fast.hip) on the
@@ -33,6 +34,7 @@
 #include "dense.h"
 #include "kernels.h"
 #include "lobpcg_batched.h"
+#include "mfma.h"
 
 namespace bx {
 namespace {
@@ -119,12 +121,15 @@ __global__ void k_bres_sum(const int *act_prob, const int *nact, const int *pc_b
 constexpr int kHM = kM / 2;                               // block rows / columns
 constexpr int kBlk = kHM * (kHM + 1) / 2;                 // upper 2 x 2 blocks
 constexpr int kBlkPerThread = (kBlk + kT - 1) / kT;
+// FP32 blocks are staged in FP32 and each kSub-row step is summed in FP32
+// before it is added to the FP64 accumulators (FP32 rounding never spans
+// more than kSub rows); FP64 blocks accumulate in FP64 throughout.
 template <typename T>
 __global__ void __launch_bounds__(kT) k_bgram(Chunks ch, Probs pr, long ld, int nb, const T *X,
                                               const T *R, const T *P, const T *AX, const T *AR,
                                               const T *AP, double *part) {
-    __shared__ double Ls[kSub][kM + 2];
-    __shared__ double As[kSub][kM + 2];
+    __shared__ T Ls[kSub][kM + 2];
+    __shared__ T As[kSub][kM + 2];
     const int c = blockIdx.x;
     if (c >= *ch.count) return;
     const int p = ch.prob[c];
@@ -155,12 +160,12 @@ __global__ void __launch_bounds__(kT) k_bgram(Chunks ch, Probs pr, long ld, int 
         // staged columns 0 .. 2 hm (an odd m gets a zero column)
         for (int e = threadIdx.x; e < kSub * 2 * hm; e += kT) {
             const int rr = e % kSub, q = e / kSub;
-            double lv = 0.0, av = 0.0;
+            T lv = T(0), av = T(0);
             if (rr < rows && q < m) {
                 const int b = q / kp, col = q % kp;
                 const long r = r0 + s0 + rr;
-                lv = blk_get(Lb[b], ld, r, col);
-                av = blk_get(Ab[b], ld, r, col);
+                lv = Lb[b][(size_t)col * ld + r];
+                av = Ab[b][(size_t)col * ld + r];
             }
             Ls[rr][q] = lv;
             As[rr][q] = av;
@@ -171,12 +176,16 @@ __global__ void __launch_bounds__(kT) k_bgram(Chunks ch, Probs pr, long ld, int 
             const int o = threadIdx.x + s * kT;
             if (o < nblk) {
                 const int i = bi[s], j = bj[s];
-                double a0 = ga[s][0], a1 = ga[s][1], a2 = ga[s][2], a3 = ga[s][3];
-                double g0 = gb[s][0], g1 = gb[s][1], g2 = gb[s][2], g3 = gb[s][3];
+                // FP64: the accumulators themselves; FP32: this step's sums
+                constexpr bool f64 = std::is_same<T, double>::value;
+                T a0 = f64 ? (T)ga[s][0] : T(0), a1 = f64 ? (T)ga[s][1] : T(0);
+                T a2 = f64 ? (T)ga[s][2] : T(0), a3 = f64 ? (T)ga[s][3] : T(0);
+                T g0 = f64 ? (T)gb[s][0] : T(0), g1 = f64 ? (T)gb[s][1] : T(0);
+                T g2 = f64 ? (T)gb[s][2] : T(0), g3 = f64 ? (T)gb[s][3] : T(0);
                 for (int rr = 0; rr < rows; rr++) {
-                    const double l0 = Ls[rr][i], l1 = Ls[rr][i + 1];
-                    const double x0 = As[rr][j], x1 = As[rr][j + 1];
-                    const double y0 = Ls[rr][j], y1 = Ls[rr][j + 1];
+                    const T l0 = Ls[rr][i], l1 = Ls[rr][i + 1];
+                    const T x0 = As[rr][j], x1 = As[rr][j + 1];
+                    const T y0 = Ls[rr][j], y1 = Ls[rr][j + 1];
                     a0 = fma(l0, x0, a0);
                     a1 = fma(l1, x0, a1);
                     a2 = fma(l0, x1, a2);
@@ -186,8 +195,13 @@ __global__ void __launch_bounds__(kT) k_bgram(Chunks ch, Probs pr, long ld, int 
                     g2 = fma(l0, y1, g2);
                     g3 = fma(l1, y1, g3);
                 }
-                ga[s][0] = a0; ga[s][1] = a1; ga[s][2] = a2; ga[s][3] = a3;
-                gb[s][0] = g0; gb[s][1] = g1; gb[s][2] = g2; gb[s][3] = g3;
+                if (f64) {
+                    ga[s][0] = a0; ga[s][1] = a1; ga[s][2] = a2; ga[s][3] = a3;
+                    gb[s][0] = g0; gb[s][1] = g1; gb[s][2] = g2; gb[s][3] = g3;
+                } else {
+                    ga[s][0] += a0; ga[s][1] += a1; ga[s][2] += a2; ga[s][3] += a3;
+                    gb[s][0] += g0; gb[s][1] += g1; gb[s][2] += g2; gb[s][3] += g3;
+                }
             }
         }
     }
@@ -205,6 +219,120 @@ __global__ void __launch_bounds__(kT) k_bgram(Chunks ch, Probs pr, long ld, int 
                     part[((size_t)c * 2 + 0) * kPairs + pair] = ga[s][q];
                     part[((size_t)c * 2 + 1) * kPairs + pair] = gb[s][q];
                 }
+            }
+        }
+    }
+}
+
+// FP32 blocks on MFMA: the chunk's rows in 32-row steps staged in LDS (the
+// problem's m = nb kp columns of L = [X R P] and of [AX AR AP], m padded to
+// whole 16-column tiles; the next step is loaded into registers while this
+// one is multiplied), every upper tile product (ti <= tj) of GA = L' A L and
+// GB = L' L dealt over the four waves, v_mfma_f32_16x16x4 over the chunk's
+// rows in FP32 (the FP32 phase stops at switch_factor * tol, far above what
+// this rounding affects); the chunk partials are summed in FP64 by
+// k_bgram_sum.  Partials in k_bgram's layout.
+constexpr int kMT = (kM + 15) / 16;                    // tiles per side
+constexpr int kMP = kMT * 16;                          // padded columns
+constexpr int kTilePairs = kMT * (kMT + 1) / 2;        // upper tile pairs
+constexpr int kMProd = 2 * kTilePairs;                 // GA and GB
+constexpr int kMPerWave = (kMProd + kT / 64 - 1) / (kT / 64);
+constexpr int kMRows = 32;                             // rows per step
+constexpr int kMS = kMP + 4;                           // LDS row stride (floats)
+__global__ void __launch_bounds__(kT) k_bgram_mfma(Chunks ch, Probs pr, long ld, int nb,
+                                                   const float *X, const float *R,
+                                                   const float *P, const float *AX,
+                                                   const float *AR, const float *AP,
+                                                   double *part) {
+    __shared__ float Ls[kMRows * kMS];
+    __shared__ float As[kMRows * kMS];
+    const int c = blockIdx.x;
+    if (c >= *ch.count) return;
+    const int p = ch.prob[c];
+    const int kp = pr.k[p];
+    const int m = nb * kp;
+    const int tm = (m + 15) / 16;
+    const int npair = tm * (tm + 1) / 2;
+    const long r0 = ch.r0[c];
+    const int nr = ch.nr[c];
+    const float *Lb[3] = {X, R, P};
+    const float *Ab[3] = {AX, AR, AP};
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+    // product s of this wave: index w = wave + 4 s; w < npair: GA tile pair
+    // w, else GB tile pair w - npair; pair index -> (ti, tj), ti <= tj
+    int pti[kMPerWave], ptj[kMPerWave], pga[kMPerWave];
+    f32x4 acc[kMPerWave];
+#pragma unroll
+    for (int s = 0; s < kMPerWave; s++) {
+        int w = wave + 4 * s;
+        pga[s] = w < npair ? 1 : (w < 2 * npair ? 0 : -1);
+        if (w >= npair) w -= npair;
+        int tj = 0;
+        while ((tj + 1) * (tj + 2) / 2 <= w) tj++;
+        pti[s] = w - tj * (tj + 1) / 2;
+        ptj[s] = tj;
+        acc[s] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+    const int mp = tm * 16;
+    // staging: element e = threadIdx.x + kT u (row fastest, so each column's
+    // rows are one coalesced read; padding rows and columns are zeros).  The
+    // next step's elements are loaded into registers while this step's
+    // MFMAs run.
+    constexpr int kPer = (kMRows * kMP + kT - 1) / kT;
+    float lreg[kPer], areg[kPer];
+    auto fetch = [&](int s0) {
+        const int rows = min(kMRows, nr - s0);
+#pragma unroll
+        for (int u = 0; u < kPer; u++) {
+            const int e = threadIdx.x + kT * u, rr = e % kMRows, q = e / kMRows;
+            lreg[u] = areg[u] = 0.f;
+            if (q < m && rr < rows) {
+                const int b = q / kp, col = q % kp;
+                const size_t o = (size_t)col * ld + (size_t)(r0 + s0 + rr);
+                lreg[u] = Lb[b][o];
+                areg[u] = Ab[b][o];
+            }
+        }
+    };
+    fetch(0);
+    for (int s0 = 0; s0 < nr; s0 += kMRows) {
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < kPer; u++) {
+            const int e = threadIdx.x + kT * u, rr = e % kMRows, q = e / kMRows;
+            if (q < mp) {
+                Ls[rr * kMS + q] = lreg[u];
+                As[rr * kMS + q] = areg[u];
+            }
+        }
+        __syncthreads();
+        if (s0 + kMRows < nr) fetch(s0 + kMRows);
+#pragma unroll
+        for (int s = 0; s < kMPerWave; s++) {
+            if (pga[s] < 0) continue;  // uniform in the wave
+            const int i0 = 16 * pti[s], j0 = 16 * ptj[s];
+            const float *Q = pga[s] ? As : Ls;
+            f32x4 d = acc[s];
+#pragma unroll
+            for (int k0 = 0; k0 < kMRows; k0 += 4) {
+                const float a = Ls[(k0 + lk) * kMS + i0 + li];
+                const float b = Q[(k0 + lk) * kMS + j0 + li];
+                d = Mfma<float>::mma(a, b, d);
+            }
+            acc[s] = d;
+        }
+    }
+    // lane (li, lk), register q: entry (i0 + drow(lk, q), j0 + li)
+#pragma unroll
+    for (int s = 0; s < kMPerWave; s++) {
+        if (pga[s] < 0) continue;
+        const int i0 = 16 * pti[s], j0 = 16 * ptj[s];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int ii = i0 + Mfma<float>::drow(lk, q), jj = j0 + li;
+            if (ii <= jj && jj < m) {
+                const int pair = jj * (jj + 1) / 2 + ii;
+                part[((size_t)c * 2 + (pga[s] ? 0 : 1)) * kPairs + pair] = (double)acc[s][q];
             }
         }
     }
@@ -486,58 +614,49 @@ __global__ void __launch_bounds__(kRT) k_brr(const int *act_prob, const int *nac
 
 // ---- block update, row by row ----
 // X' = sum_q C[q, j] L_q (q over the nbe blocks' columns), P' = the part with
-// q >= kp; the same for the A-side blocks.  Each thread updates two rows
-// (kT apart) so that every coefficient read from LDS (the same address across
-// the wave) feeds two FMAs; the output columns accumulate in registers (KB of
+// q >= kp; the same for the A-side blocks.  A thread owns one row; the
+// output columns accumulate in registers in the blocks' precision (KB of
 // them: the problem's column count rounded up to 8, 16 or 24, chosen per
-// workgroup).
+// workgroup), the coefficients sit in LDS in that precision (each read, the
+// same address across the wave, feeds the FMAs of a whole wave), and the
+// loads of four input columns are issued before their FMAs.
 template <int KB, typename T>
-__device__ __forceinline__ void bupd_rows(const double *cs, int kp, int nb, long ld, long r0,
-                                          int nr, T *b0, const T *b1, T *b2) {
+__device__ __forceinline__ void bupd_rows(const T *cs, int kp, int nb, long ld, long r0, int nr,
+                                          T *b0, const T *b1, T *b2) {
     const T *bl[3] = {b0, b1, b2};
-    for (int t0 = 0; t0 < nr; t0 += 2 * kT) {
-        const int ta = t0 + threadIdx.x, tb = ta + kT;
-        const bool va = ta < nr, vb = tb < nr;
-        const long ra = r0 + (va ? ta : 0), rb = r0 + (vb ? tb : 0);
-        double xa[KB], pa[KB], xb[KB], pb[KB];
+    for (int t = threadIdx.x; t < nr; t += kT) {
+        const long r = r0 + t;
+        T xa[KB], pa[KB];
 #pragma unroll
-        for (int j = 0; j < KB; j++) xa[j] = pa[j] = xb[j] = pb[j] = 0.0;
+        for (int j = 0; j < KB; j++) xa[j] = pa[j] = T(0);
         for (int b = 0; b < nb; b++) {
-            const T *src = bl[b];
-            for (int col = 0; col < kp; col++) {
-                const int q = b * kp + col;
-                const double ua = (double)src[(size_t)col * ld + ra];
-                const double ub = (double)src[(size_t)col * ld + rb];
-                if (b == 0) {
+            const T *src = bl[b] + r;
+            for (int c0 = 0; c0 < kp; c0 += 4) {
+                T u[4];
 #pragma unroll
-                    for (int j = 0; j < KB; j++)
-                        if (j < kp) {
-                            const double cq = cs[q + j * kM];
-                            xa[j] = fma(cq, ua, xa[j]);
-                            xb[j] = fma(cq, ub, xb[j]);
-                        }
-                } else {
+                for (int d = 0; d < 4; d++)
+                    u[d] = c0 + d < kp ? src[(size_t)(c0 + d) * ld] : T(0);
 #pragma unroll
-                    for (int j = 0; j < KB; j++)
-                        if (j < kp) {
-                            const double cq = cs[q + j * kM];
-                            pa[j] = fma(cq, ua, pa[j]);
-                            pb[j] = fma(cq, ub, pb[j]);
-                        }
+                for (int d = 0; d < 4; d++) {
+                    if (c0 + d >= kp) break;
+                    const T *cq = cs + (b * kp + c0 + d);
+                    if (b == 0) {
+#pragma unroll
+                        for (int j = 0; j < KB; j++)
+                            if (j < kp) xa[j] = fma(cq[j * kM], u[d], xa[j]);
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < KB; j++)
+                            if (j < kp) pa[j] = fma(cq[j * kM], u[d], pa[j]);
+                    }
                 }
             }
         }
 #pragma unroll
         for (int j = 0; j < KB; j++)
             if (j < kp) {
-                if (va) {
-                    if (nb > 1) b2[(size_t)j * ld + ra] = (T)pa[j];
-                    b0[(size_t)j * ld + ra] = (T)(xa[j] + pa[j]);
-                }
-                if (vb) {
-                    if (nb > 1) b2[(size_t)j * ld + rb] = (T)pb[j];
-                    b0[(size_t)j * ld + rb] = (T)(xb[j] + pb[j]);
-                }
+                if (nb > 1) b2[(size_t)j * ld + r] = pa[j];
+                b0[(size_t)j * ld + r] = xa[j] + pa[j];
             }
     }
 }
@@ -546,7 +665,7 @@ template <typename T>
 __global__ void __launch_bounds__(kT) k_bupd(Chunks ch, Probs pr, long ld, const double *C,
                                              const int *nbe, int nb_all, T *X, T *R, T *P, T *AX,
                                              T *AR, T *AP) {
-    __shared__ double cs[kM * kK];
+    __shared__ T cs[kM * kK];
     const int c = blockIdx.x;
     if (c >= *ch.count) return;
     const int p = ch.prob[c];
@@ -554,7 +673,7 @@ __global__ void __launch_bounds__(kT) k_bupd(Chunks ch, Probs pr, long ld, const
     const int nb = nbe ? nbe[p] : nb_all;
     const long r0 = ch.r0[c];
     const int nr = ch.nr[c];
-    for (int e = threadIdx.x; e < kM * kK; e += kT) cs[e] = C[(size_t)p * kM * kK + e];
+    for (int e = threadIdx.x; e < kM * kK; e += kT) cs[e] = (T)C[(size_t)p * kM * kK + e];
     __syncthreads();
     for (int side = 0; side < 2; side++) {
         T *b0 = side ? AX : X, *b1 = side ? AR : R, *b2 = side ? AP : P;
@@ -664,7 +783,7 @@ void hip_BatchSettingsDefault(hip_BatchSettings *s) {
     s->graph = 1;
     s->guard = 2;
     s->mode = HIPLOBPCG_FP64;
-    s->switch_factor = 1000.0;
+    s->switch_factor = 100.0;
     s->autotune = 1;
 }
 
@@ -835,6 +954,16 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
             BX_HIP(hipMemsetAsync(*q, 0, nk * 4, stream()));
         }
     }
+    // the Gram partials: FP32 blocks on MFMA, FP64 blocks on FP64 FMA
+    auto launch_gram = [&](auto &bk, int nb) {
+        using T = typename std::remove_reference<decltype(bk)>::type::type;
+        if constexpr (std::is_same<T, float>::value)
+            hipLaunchKernelGGL(k_bgram_mfma, dim3(nch), dim3(kT), 0, stream(), ch, pr, N, nb,
+                               bk.X, bk.R, bk.P, bk.AX, bk.AR, bk.AP, part_g);
+        else
+            hipLaunchKernelGGL(k_bgram<T>, dim3(nch), dim3(kT), 0, stream(), ch, pr, N, nb,
+                               bk.X, bk.R, bk.P, bk.AX, bk.AR, bk.AP, part_g);
+    };
     // grids cover every chunk / problem; workgroups past the device counts
     // exit at once, so one captured graph serves a whole phase
     auto step = [&](auto &bk, int nb, bool with_spmm) {
@@ -843,8 +972,7 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
         if (with_spmm)
             hipLaunchKernelGGL(k_bspmm<T>, dim3(nch), dim3(kT), 0, stream(), ch, pr, N,
                                A->rowptr, A->colind, vals, (const T *)bk.R, bk.AR);
-        hipLaunchKernelGGL(k_bgram<T>, dim3(nch), dim3(kT), 0, stream(), ch, pr, N, nb, bk.X,
-                           bk.R, bk.P, bk.AX, bk.AR, bk.AP, part_g);
+        launch_gram(bk, nb);
         hipLaunchKernelGGL(k_bgram_sum, dim3(nprob), dim3(kT), 0, stream(), d_actp, d_nact, d_pcb,
                            d_pcc, pr, nb, part_g, GA, GB);
         if (nb == 1) {
@@ -951,8 +1079,7 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
                 BX_HIP(hipEventRecord(e0, stream()));
                 hipLaunchKernelGGL(k_bspmm<T>, dim3(nch), dim3(kT), 0, stream(), ch, pr, N,
                                    A->rowptr, A->colind, vals, (const T *)bk.R, bk.AR);
-                hipLaunchKernelGGL(k_bgram<T>, dim3(nch), dim3(kT), 0, stream(), ch, pr, N, 3,
-                                   bk.X, bk.R, bk.P, bk.AX, bk.AR, bk.AP, part_g);
+                launch_gram(bk, 3);
                 BX_HIP(hipEventRecord(e1, stream()));
                 BX_HIP(hipEventSynchronize(e1));
                 float t = 0.f;

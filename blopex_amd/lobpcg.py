@@ -336,7 +336,7 @@ class BatchResult:
 def lobpcg_batched(problems, tol: float = 1e-6, maxit: int = 500, jacobi: bool = True,
                    seed: int = 1, graph: bool = True, want_X: bool = False,
                    guard: int = 2, mode: str = "fp64",
-                   switch_factor: float = 1000.0, autotune: bool = True) -> BatchResult:
+                   switch_factor: float = 100.0, autotune: bool = True) -> BatchResult:
     """Smallest eigenpairs of many independent problems in one device solve
     (hip_LobpcgSolveBatched).  problems: list of (CSR, k)."""
     lib = L.load()

@@ -42,12 +42,14 @@ typedef struct hip_BatchSettings {
                            HIP_BATCH_MAX_K + 8 columns in all and n/3; only the k wanted
                            are tested for convergence and returned (default 2) */
     int mode;           /* HIPLOBPCG_FP64 (default), or HIPLOBPCG_MIXED: the blocks in FP32
-                           (FP64 Gram sums and Rayleigh-Ritz, AX recomputed from X every 20
-                           iterations) until all but 5% of the problems have their wanted
-                           residuals below switch_factor * tol (or maxit / 2 iterations
-                           have run), then every problem continues in FP64 from its block */
-    double switch_factor; /* MIXED: see mode (default 1000: on the 2D 512-problem workload
-                             100, 300, 1000, 3000 and 1e4 took 5.1, 4.9, 4.7, 4.7 and 5.0 s) */
+                           (FP32 block updates, Gram chunks on FP32 MFMA summed in FP64,
+                           FP64 Rayleigh-Ritz, AX recomputed from X every 20 iterations)
+                           until all but 5% of the problems have their wanted residuals
+                           below switch_factor * tol (or maxit / 2 iterations have run),
+                           then every problem continues in FP64 from its block */
+    double switch_factor; /* MIXED: see mode (default 100: on the 2D 512-problem workload
+                             30, 100, 300, 1000 and 3000 took 3.15, 3.07, 3.08, 3.21 and
+                             3.36 s) */
     int autotune;       /* 1: pick the rows per chunk (the work of one workgroup: 512, 1024,
                            2048 or 4096) by timing the SpMM and Gram kernels on this batch
                            before the solve (default 1); 0: 2048 */
