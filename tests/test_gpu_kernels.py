@@ -77,3 +77,29 @@ def test_gram(prec, n, lw, qw):
     scale = np.abs(Lm).T @ np.abs(Qm)
     tol = 1e-14 if prec == L.HIPMV_FP64 else 4e-6
     assert np.max(np.abs(G - ref) / scale) < tol
+
+
+def test_fp32_gram_error_does_not_grow_with_n():
+    """The FP32 Gram (FP32 MFMA over one staged chunk of at most 256 rows,
+    the chunk sums added in FP64): its error against the exact product of the
+    same FP32 data is bounded by the chunk's rounding, c u32 |L|'|Q| with c
+    <= 256 rows, whatever n -- a plain FP32 accumulation over n rows would
+    carry n u32.  Measured at n = 2^16 and 2^20: both within 256 u32, and the
+    larger not growing with n (at most 2x the smaller)."""
+    u32 = 2.0 ** -24
+    errs = []
+    for n in (1 << 16, 1 << 20):
+        say(f"fp32 gram error bound n={n}")
+        rng = np.random.default_rng(3)
+        Lm = f(rng.standard_normal((n, 16)).astype(np.float32))
+        Qm = f(rng.standard_normal((n, 16)).astype(np.float32))
+        G = np.zeros((16, 16), order="F")
+        L.load().blopex_amd_kernel_gram(L.HIPMV_FP32, n, 16, 16, Lm.ctypes.data, Qm.ctypes.data,
+                                        G.ctypes.data)
+        exact = Lm.T @ Qm  # FP64 of FP32 data: exact to 1e-16 relative here
+        scale = np.abs(Lm).T @ np.abs(Qm)
+        errs.append(np.max(np.abs(G - exact) / scale))
+        say(f"  max |G - exact| / (|L|'|Q|) = {errs[-1]:.2e} (256 u32 = {256 * u32:.2e}, "
+            f"n u32 = {n * u32:.2e})")
+    assert max(errs) < 256 * u32
+    assert errs[1] < 2.0 * errs[0] + 4 * u32
