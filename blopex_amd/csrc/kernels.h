@@ -148,6 +148,12 @@ void zvecmat_cols(long n, const void *x, long ldx, const ColMap &xm, const void 
 // y(:, ym[j]) = d[j] x(:, xm[j]), d device komplex
 void zscale_cols(long n, const void *d, const void *x, long ldx, const ColMap &xm, void *y,
                  long ldy, const ColMap &ym);
+// Realified blocks of the complex solve: column j of r (2n rows, precision
+// rprec) = [Re z_j; Im z_j] for z (n x w komplex, leading dim ldz); with
+// jpair also column w + j of r = [-Im z_j; Re z_j] (the column of i z_j).
+void z_to_real(long n, int w, const void *z, long ldz, void *r, long ldr, int rprec, bool jpair);
+// z_j = r_j[0:n] + i r_j[n:2n] for the w columns of r
+void real_to_z(long n, int w, const void *r, long ldr, int rprec, void *z, long ldz);
 // y(:, ym) = A x(:, xm), A dense komplex n x n
 void zdense_apply(long n, const void *A, long lda, const void *x, long ldx, const ColMap &xm,
                   void *y, long ldy, const ColMap &ym);

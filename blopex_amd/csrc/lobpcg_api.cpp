@@ -1,8 +1,9 @@
 // lobpcg_solve_double / lobpcg_solve_complex entry points
-// (krylov/lobpcg.c:332-498): a real block wrapped with the HIP interpreter
-// runs the device-resident solver; complex blocks and other backends run the
-// generic algorithm with the FortranMatrix routines of this library and the
-// caller's LAPACK.
+// (krylov/lobpcg.c:332-498): a block wrapped with the HIP interpreter runs
+// the device-resident solver (complex blocks through the real form of their
+// pencil); other backends, and blocks wider than the device solver takes, run
+// the generic algorithm with the FortranMatrix routines of this library and
+// the caller's LAPACK.
 #include <cstdio>
 
 #include "common.h"
@@ -22,6 +23,12 @@ int device_lobpcg(mv_MultiVectorPtr X, void *Ad, void (*opA)(void *, void *, voi
                   int maxit, int verb, int *iters, double *lam, double *lamh, BlopexInt lamh_gh,
                   double *res, double *resh, BlopexInt resh_gh);
 bool device_lobpcg_fits(int k, int guard);
+int device_lobpcg_complex(mv_MultiVectorPtr X, void *Ad, void (*opA)(void *, void *, void *),
+                          void *Bd, void (*opB)(void *, void *, void *), void *Td,
+                          void (*opT)(void *, void *, void *), mv_MultiVectorPtr Y,
+                          lobpcg_Tolerance tol, int maxit, int verb, int *iters, komplex *lam,
+                          komplex *lamh, BlopexInt lamh_gh, double *res, double *resh,
+                          BlopexInt resh_gh);
 }
 
 extern "C" {
@@ -74,15 +81,23 @@ int lobpcg_solve_complex(mv_MultiVectorPtr X, void *Ad, void (*opA)(void *, void
                          lobpcg_BLASLAPACKFunctions blap, lobpcg_Tolerance tol, int maxit,
                          int verb, int *iters, komplex *lam, komplex *lamh, BlopexInt lamh_gh,
                          double *res, double *resh, BlopexInt resh_gh) {
-    // complex Hermitian problems run the generic algorithm: block work through
-    // X's interpreter (HIPMV_C64 blocks for the device), small dense work
-    // through the complex FortranMatrix routines and the caller's zpotrf/zhegv
-    if (blopex_amd_is_hip_interpreter(blopex_amd_mv_interpreter(X)) &&
-        ((hip_Multi_Vector *)mv_MultiVectorGetData(X))->precision != HIPMV_C64) {
-        bx::set_error("lobpcg_solve_complex: the block is not complex (HIPMV_C64)");
-        if (verb) fprintf(stderr, "Error in LOBPCG:\n%s\n", blopex_amd_last_error());
-        if (iters) *iters = 0;
-        return WRONG_BLOCK_SIZE;
+    // a complex HIP block runs on the device solver through the real form of
+    // the pencil (lobpcg_complex.cpp) when twice its width fits; wider blocks
+    // and other backends run the generic algorithm (block work through X's
+    // interpreter, small dense work through the complex FortranMatrix
+    // routines and the caller's zpotrf/zhegv)
+    if (blopex_amd_is_hip_interpreter(blopex_amd_mv_interpreter(X))) {
+        const hip_Multi_Vector *x = (const hip_Multi_Vector *)mv_MultiVectorGetData(X);
+        if (x->precision != HIPMV_C64) {
+            bx::set_error("lobpcg_solve_complex: the block is not complex (HIPMV_C64)");
+            if (verb) fprintf(stderr, "Error in LOBPCG:\n%s\n", blopex_amd_last_error());
+            if (iters) *iters = 0;
+            return WRONG_BLOCK_SIZE;
+        }
+        const int k = (int)x->num_vectors;
+        if (bx::device_lobpcg_fits(2 * k, 0) || !blap.zpotrf || !blap.zhegv)
+            return bx::device_lobpcg_complex(X, Ad, opA, Bd, opB, Td, opT, Y, tol, maxit, verb,
+                                             iters, lam, lamh, lamh_gh, res, resh, resh_gh);
     }
     if (!blap.zpotrf || !blap.zhegv) {
         bx::set_error("lobpcg_solve_complex: zpotrf/zhegv are required");

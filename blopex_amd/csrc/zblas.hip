@@ -22,6 +22,36 @@ __device__ __forceinline__ double2 cmulc(double2 a, double2 b) {
     return make_double2(fma(a.x, b.x, a.y * b.y), fma(a.x, b.y, -a.y * b.x));
 }
 
+// Realified columns: complex z (n rows) <-> real r (2n rows: the real parts,
+// then the imaginary parts).  With jpair, column w + j of r also receives
+// J z_j = i z_j, that is [-Im z_j; Re z_j].
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_z_to_real(long n, const double2 *z, long ldz, T *r,
+                                                        long ldr, int w, int jpair) {
+    const int j = blockIdx.y;
+    const double2 *zc = z + (size_t)j * ldz;
+    T *rc = r + (size_t)j * ldr, *rp = r + (size_t)(j + w) * ldr;
+    for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < n; i += (long)gridDim.x * kThreads) {
+        const double2 v = zc[i];
+        rc[i] = (T)v.x;
+        rc[n + i] = (T)v.y;
+        if (jpair) {
+            rp[i] = (T)-v.y;
+            rp[n + i] = (T)v.x;
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_real_to_z(long n, const T *r, long ldr, double2 *z,
+                                                        long ldz) {
+    const int j = blockIdx.y;
+    const T *rc = r + (size_t)j * ldr;
+    double2 *zc = z + (size_t)j * ldz;
+    for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < n; i += (long)gridDim.x * kThreads)
+        zc[i] = make_double2((double)rc[i], (double)rc[n + i]);
+}
+
 unsigned zrow_grid(long n) {
     size_t blocks = ((size_t)n + kThreads - 1) / kThreads;
     size_t cap = (size_t)num_cus() * 4;
@@ -175,6 +205,32 @@ void zdense_apply(long n, const void *A, long lda, const void *x, long ldx, cons
     hipLaunchKernelGGL(k_zdense, dim3(zrow_grid(n), xm.n), dim3(kThreads), 0, stream(), n,
                        (const double2 *)A, lda, (const double2 *)x, ldx, xm, (double2 *)y, ldy,
                        ym);
+    BX_HIP(hipGetLastError());
+}
+
+void z_to_real(long n, int w, const void *z, long ldz, void *r, long ldr, int rprec, bool jpair) {
+    if (n == 0 || w == 0) return;
+    BX_REQUIRE(rprec == HIPMV_FP64 || rprec == HIPMV_FP32, "z_to_real: precision %d", rprec);
+    dim3 g(zrow_grid(n), w);
+    if (rprec == HIPMV_FP64)
+        hipLaunchKernelGGL(k_z_to_real<double>, g, dim3(kThreads), 0, stream(), n,
+                           (const double2 *)z, ldz, (double *)r, ldr, w, (int)jpair);
+    else
+        hipLaunchKernelGGL(k_z_to_real<float>, g, dim3(kThreads), 0, stream(), n,
+                           (const double2 *)z, ldz, (float *)r, ldr, w, (int)jpair);
+    BX_HIP(hipGetLastError());
+}
+
+void real_to_z(long n, int w, const void *r, long ldr, int rprec, void *z, long ldz) {
+    if (n == 0 || w == 0) return;
+    BX_REQUIRE(rprec == HIPMV_FP64 || rprec == HIPMV_FP32, "real_to_z: precision %d", rprec);
+    dim3 g(zrow_grid(n), w);
+    if (rprec == HIPMV_FP64)
+        hipLaunchKernelGGL(k_real_to_z<double>, g, dim3(kThreads), 0, stream(), n,
+                           (const double *)r, ldr, (double2 *)z, ldz);
+    else
+        hipLaunchKernelGGL(k_real_to_z<float>, g, dim3(kThreads), 0, stream(), n,
+                           (const float *)r, ldr, (double2 *)z, ldz);
     BX_HIP(hipGetLastError());
 }
 

@@ -289,20 +289,39 @@ def _zlapack() -> tuple[int, int]:
 
 
 def lobpcg_complex(X0: np.ndarray, A: np.ndarray, tol: float = 1e-6, maxit: int = 1000,
-                   verbosity: int = 0, rtol: float = 1e-50) -> Result:
-    """Smallest eigenpairs of a dense complex Hermitian A through
-    lobpcg_solve_complex on HIPMV_C64 blocks (the reference's complex
-    driver, blopex_serial_complex/driver/serial_driver.c, on the device)."""
+                   verbosity: int = 0, rtol: float = 1e-50, B: np.ndarray | None = None,
+                   Y: np.ndarray | None = None, mode: str = "fp64") -> Result:
+    """Smallest eigenpairs of a dense complex Hermitian A (B: dense Hermitian
+    positive definite, Y: constraints) through lobpcg_solve_complex on
+    HIPMV_C64 blocks (the reference's complex driver,
+    blopex_serial_complex/driver/serial_driver.c, on the device).  Blocks of
+    up to 32 columns run on the device solver; wider ones on the generic
+    algorithm with scipy's zpotrf/zhegv."""
     lib = L.load()
     n, k = X0.shape
     res_objs = []
+    dense = L.symbol_address("HipDenseMatMultiVec")
     try:
         dA = DeviceBlock.from_numpy(np.asarray(A, dtype=np.complex128))
         res_objs.append(dA)
+        dB = None
+        if B is not None:
+            dB = DeviceBlock.from_numpy(np.asarray(B, dtype=np.complex128))
+            res_objs.append(dB)
         X = DeviceBlock.from_numpy(np.asarray(X0, dtype=np.complex128))
         res_objs.append(X)
+        s = L.Settings()
+        lib.hip_LobpcgSettingsDefault(ctypes.byref(s))
+        s.mode = L.HIPLOBPCG_MIXED if mode == "mixed" else L.HIPLOBPCG_FP64
+        if lib.hip_Multi_VectorSetSettings(X.ptr, ctypes.byref(s)) != 0:
+            raise ValueError(L.last_error())
         ii = interpreter()
         xx = lib.mv_MultiVectorWrap(ctypes.byref(ii), ctypes.cast(X.ptr, ctypes.c_void_p), 0)
+        yy = None
+        if Y is not None:
+            dY = DeviceBlock.from_numpy(np.asarray(Y, dtype=np.complex128))
+            res_objs.append(dY)
+            yy = lib.mv_MultiVectorWrap(ctypes.byref(ii), ctypes.cast(dY.ptr, ctypes.c_void_p), 0)
         zp, zh = _zlapack()
         blap = L.BlasLapack(None, None, zp, zh)
         lam = np.zeros(k, dtype=np.complex128)
@@ -311,13 +330,19 @@ def lobpcg_complex(X0: np.ndarray, A: np.ndarray, tol: float = 1e-6, maxit: int 
         rh = np.zeros((k, maxit + 1), order="F")
         it = ctypes.c_int(0)
         flag = lib.lobpcg_solve_complex(
-            xx, ctypes.cast(dA.ptr, ctypes.c_void_p), L.symbol_address("HipDenseMatMultiVec"),
-            None, None, None, None, None, blap, L.Tolerance(tol, rtol), maxit, verbosity,
-            ctypes.byref(it), lam.ctypes.data, lh.ctypes.data, k, res.ctypes.data,
+            xx, ctypes.cast(dA.ptr, ctypes.c_void_p), dense,
+            None if dB is None else ctypes.cast(dB.ptr, ctypes.c_void_p),
+            None if dB is None else dense, None, None, yy, blap, L.Tolerance(tol, rtol), maxit,
+            verbosity, ctypes.byref(it), lam.ctypes.data, lh.ctypes.data, k, res.ctypes.data,
             rh.ctypes.data, k)
+        st = L.Stats()
+        lib.hip_Multi_VectorGetStats(X.ptr, ctypes.byref(st))
         lib.mv_MultiVectorDestroy(xx)
+        if yy:
+            lib.mv_MultiVectorDestroy(yy)
         m = it.value
-        return Result(X.to_numpy(), lam, flag, m, lh[:, : m + 1], rh[:, : m + 1], res)
+        stats = {f: getattr(st, f) for f, _ in L.Stats._fields_}
+        return Result(X.to_numpy(), lam, flag, m, lh[:, : m + 1], rh[:, : m + 1], res, stats)
     finally:
         for o in reversed(res_objs):
             o.close()

@@ -71,12 +71,15 @@ int lobpcg_solve_double(mv_MultiVectorPtr blockVectorX, void *operatorAData,
                         BlopexInt lambdaHistory_gh, double *residualNorms_values,
                         double *residualNormsHistory_values, BlopexInt residualNormsHistory_gh);
 
-/* Complex Hermitian problems (lobpcg.c:417-498): the generic algorithm with
+/* Complex Hermitian problems (lobpcg.c:417-498).  A block wrapped with the
+   HIP interpreter must be HIPMV_C64 (WRONG_BLOCK_SIZE otherwise); up to 32
+   columns it runs on the device solver through the real symmetric form of
+   the pencil (order 2n, each eigenvalue twice; the operators still see
+   complex blocks, HipDenseMatMultiVec being the dense one), and blap_fn is
+   not used.  Wider blocks and other backends run the generic algorithm with
    the complex FortranMatrix routines (zutilities_*) and blap_fn.zpotrf /
-   zhegv (required: -1 without them).  lambda and its history are komplex
-   with zero imaginary parts.  Blocks wrapped with the HIP interpreter must
-   be HIPMV_C64 (WRONG_BLOCK_SIZE otherwise); HipDenseMatMultiVec is their
-   dense operator. */
+   zhegv (required there: -1 without them).  lambda and its history are
+   komplex with zero imaginary parts. */
 int lobpcg_solve_complex(mv_MultiVectorPtr blockVectorX, void *operatorAData,
                          void (*operatorA)(void *, void *, void *), void *operatorBData,
                          void (*operatorB)(void *, void *, void *), void *operatorTData,

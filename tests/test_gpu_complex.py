@@ -12,6 +12,12 @@ blocks) through the C ABI.
   initial Rayleigh-Ritz values, residuals at the tolerance, orthonormal X.
 * The driver's random mode (A = Hermitian part of a drand48 fill, X0 the
   same stream) against the reference run in the test on the same inputs.
+* The device path (the real form of the pencil on the real device solver)
+  beyond the driver's cases, against built-in spectra: a generalized pencil
+  with constraints, doubly degenerate eigenvalues (the returned vectors are
+  B-orthonormal and span the eigenspaces), a 4000-row problem, MIXED mode;
+  and a block wider than the device solver takes, which runs the generic
+  algorithm with the caller's zpotrf/zhegv.
 """
 import ctypes
 import sys
@@ -138,6 +144,7 @@ def test_complex_solve_matches_reference(golden, name):
     ref_it = int(golden[name + "_iters"])
     say(f"  flag {r.flag} iterations {r.iterations} (reference {ref_it})")
     assert r.flag == int(golden[name + "_flag"]) == 0
+    assert r.stats["iterations_fp64"] == r.iterations  # the device solver ran
     assert abs(r.iterations - ref_it) <= max(2, ref_it // 20)
     assert np.all(r.lam.imag == 0.0)
     ref_lam = golden[name + "_lam"]
@@ -173,3 +180,101 @@ def test_complex_solve_random_hermitian_mode():
     ev = np.linalg.eigvalsh(A)[:k]
     assert np.max(np.abs(r.lam.real - ev) / np.abs(ev)) < 1e-10
     assert np.max(np.abs(r.lam.real - o.lam.real) / np.abs(ev)) < 1e-10
+
+
+def _hermitian_with_spectrum(rng, d):
+    n = len(d)
+    Q, _ = np.linalg.qr(_crand(rng, n, n))
+    return np.asfortranarray((Q * d) @ Q.conj().T), Q
+
+
+def test_complex_generalized_with_constraints():
+    """(A, B) with B dense Hermitian positive definite and, as constraints,
+    its two lowest eigenvectors (the residual lobpcg.c measures is that of
+    the unprojected pencil, so the constraints are eigenvectors, as in the
+    reference's test_diag.m): the next k eigenpairs, built in."""
+    say("complex generalized pencil with constraints")
+    rng = np.random.default_rng(21)
+    n, k, m = 400, 6, 2
+    d = np.concatenate([np.arange(1.0, 13.0), np.linspace(40.0, 400.0, n - 12)])
+    Cb = _crand(rng, n, n) * (0.2 / n)
+    B = np.asfortranarray(np.eye(n) + (Cb + Cb.conj().T) / 2)
+    s, U = np.linalg.eigh(B)
+    Q, _ = np.linalg.qr(_crand(rng, n, n))
+    Z = (U / np.sqrt(s)) @ U.conj().T @ Q  # Z^H B Z = I
+    A = np.asfortranarray(B @ (Z * d) @ Z.conj().T @ B)
+    A = (A + A.conj().T) / 2
+    Y = np.asfortranarray(Z[:, :m])
+    r = bl.lobpcg_complex(_crand(rng, n, k), A, tol=1e-9, maxit=500, B=B, Y=Y)
+    say(f"  flag {r.flag} iterations {r.iterations}")
+    assert r.flag == 0 and r.stats["iterations_fp64"] == r.iterations
+    np.testing.assert_allclose(r.lam.real, d[m:m + k], rtol=1e-9)
+    X = r.X
+    assert np.max(np.abs(X.conj().T @ B @ X - np.eye(k))) < 1e-10
+    assert np.max(np.abs(Y.conj().T @ B @ X)) < 1e-10
+    res = np.linalg.norm(A @ X - B @ X * r.lam.real, axis=0)
+    assert np.all(res <= 1.5e-9 * np.abs(A).max())
+
+
+def test_complex_degenerate_eigenvalues():
+    """Eigenvalues 1, 1, 2, 2, 3, 3: each pair's real form has four Ritz
+    vectors; the returned complex vectors are orthonormal and span each
+    two-dimensional eigenspace."""
+    say("complex doubly degenerate eigenvalues")
+    rng = np.random.default_rng(22)
+    n, k = 200, 6
+    d = np.concatenate([[1.0, 1.0, 2.0, 2.0, 3.0, 3.0], np.linspace(20.0, 200.0, n - 6)])
+    A, Q = _hermitian_with_spectrum(rng, d)
+    r = bl.lobpcg_complex(_crand(rng, n, k), A, tol=1e-10, maxit=400)
+    say(f"  flag {r.flag} iterations {r.iterations}")
+    assert r.flag == 0
+    np.testing.assert_allclose(r.lam.real, d[:k], rtol=1e-12)
+    X = r.X
+    assert np.max(np.abs(X.conj().T @ X - np.eye(k))) < 1e-12
+    for j in range(0, k, 2):  # the projector onto each eigenspace from X and from Q agree
+        Px, Pq = X[:, j:j + 2] @ X[:, j:j + 2].conj().T, Q[:, j:j + 2] @ Q[:, j:j + 2].conj().T
+        assert np.max(np.abs(Px - Pq)) < 1e-8
+
+
+def test_complex_larger_problem():
+    """n = 4000, k = 16: eigenvalues against the known spectrum, residuals
+    recomputed on the host."""
+    say("complex n=4000 k=16")
+    rng = np.random.default_rng(23)
+    n, k = 4000, 16
+    d = np.concatenate([np.linspace(1.0, 4.0, k), np.linspace(30.0, 3000.0, n - k)])
+    A, _ = _hermitian_with_spectrum(rng, d)
+    r = bl.lobpcg_complex(_crand(rng, n, k), A, tol=1e-8, maxit=300)
+    say(f"  flag {r.flag} iterations {r.iterations} in {r.stats['seconds']:.3f} s")
+    assert r.flag == 0 and r.stats["iterations_fp64"] == r.iterations
+    np.testing.assert_allclose(r.lam.real, d[:k], rtol=1e-10)
+    res = np.linalg.norm(A @ r.X - r.X * r.lam.real, axis=0)
+    assert np.all(res <= 1.5e-8)
+
+
+def test_complex_mixed_mode(golden):
+    """MIXED mode on the driver's second case: FP32 iterations first, then
+    the reference's eigenvalues to 1e-8 (the bar of the real MIXED parity
+    tests at the same 1e-6 residual tolerance) after the FP64 finish."""
+    say("complex MIXED mode (ztest2)")
+    A, X0 = golden["ztest2_A"], golden["ztest2_X0"]
+    r = bl.lobpcg_complex(X0, A, tol=1e-6, maxit=1000, mode="mixed")
+    say(f"  flag {r.flag} iterations {r.iterations} (fp32 {r.stats['iterations_fp32']})")
+    assert r.flag == 0 and r.stats["iterations_fp32"] > 0
+    ref_lam = golden["ztest2_lam"]
+    assert np.max(np.abs(r.lam.real - ref_lam) / np.abs(ref_lam)) < 1e-8
+    assert np.all(r.res <= 1e-6 + 1e-15)
+
+
+def test_complex_wide_block_runs_generic_path():
+    """33 columns (66 in real form, above the device solver's 64): the
+    generic algorithm with scipy's zpotrf/zhegv, against the spectrum."""
+    say("complex k=33 (generic path)")
+    rng = np.random.default_rng(24)
+    n, k = 300, 33
+    d = np.concatenate([np.linspace(1.0, 3.0, k + 3), np.linspace(20.0, 300.0, n - k - 3)])
+    A, _ = _hermitian_with_spectrum(rng, d)
+    r = bl.lobpcg_complex(_crand(rng, n, k), A, tol=1e-8, maxit=300)
+    say(f"  flag {r.flag} iterations {r.iterations}")
+    assert r.flag == 0 and r.stats["iterations_fp64"] == 0
+    np.testing.assert_allclose(r.lam.real, d[:k], rtol=1e-10)
