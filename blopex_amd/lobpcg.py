@@ -290,9 +290,10 @@ def _zlapack() -> tuple[int, int]:
 
 def lobpcg_complex(X0: np.ndarray, A: np.ndarray, tol: float = 1e-6, maxit: int = 1000,
                    verbosity: int = 0, rtol: float = 1e-50, B: np.ndarray | None = None,
-                   Y: np.ndarray | None = None, mode: str = "fp64") -> Result:
+                   Y: np.ndarray | None = None, T: np.ndarray | None = None,
+                   mode: str = "fp64") -> Result:
     """Smallest eigenpairs of a dense complex Hermitian A (B: dense Hermitian
-    positive definite, Y: constraints) through lobpcg_solve_complex on
+    positive definite, Y: constraints, T: dense preconditioner) through lobpcg_solve_complex on
     HIPMV_C64 blocks (the reference's complex driver,
     blopex_serial_complex/driver/serial_driver.c, on the device).  Blocks of
     up to 32 columns run on the device solver; wider ones on the generic
@@ -304,10 +305,13 @@ def lobpcg_complex(X0: np.ndarray, A: np.ndarray, tol: float = 1e-6, maxit: int 
     try:
         dA = DeviceBlock.from_numpy(np.asarray(A, dtype=np.complex128))
         res_objs.append(dA)
-        dB = None
+        dB = dT = None
         if B is not None:
             dB = DeviceBlock.from_numpy(np.asarray(B, dtype=np.complex128))
             res_objs.append(dB)
+        if T is not None:
+            dT = DeviceBlock.from_numpy(np.asarray(T, dtype=np.complex128))
+            res_objs.append(dT)
         X = DeviceBlock.from_numpy(np.asarray(X0, dtype=np.complex128))
         res_objs.append(X)
         s = L.Settings()
@@ -332,7 +336,9 @@ def lobpcg_complex(X0: np.ndarray, A: np.ndarray, tol: float = 1e-6, maxit: int 
         flag = lib.lobpcg_solve_complex(
             xx, ctypes.cast(dA.ptr, ctypes.c_void_p), dense,
             None if dB is None else ctypes.cast(dB.ptr, ctypes.c_void_p),
-            None if dB is None else dense, None, None, yy, blap, L.Tolerance(tol, rtol), maxit,
+            None if dB is None else dense,
+            None if dT is None else ctypes.cast(dT.ptr, ctypes.c_void_p),
+            None if dT is None else dense, yy, blap, L.Tolerance(tol, rtol), maxit,
             verbosity, ctypes.byref(it), lam.ctypes.data, lh.ctypes.data, k, res.ctypes.data,
             rh.ctypes.data, k)
         st = L.Stats()
