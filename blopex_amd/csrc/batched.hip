@@ -12,8 +12,8 @@
 //   k_bresid     R = T (AX - X diag(lambda)) and per-chunk partial norms
 //   k_bspmm      AR = A R over the active chunks of the stacked CSR matrix
 //   k_bgram      per-chunk partial Gram blocks [X R P]' [AX AR AP] and
-//                [X R P]' [X R P] (upper triangles, the problem's columns only);
-//                FP32 blocks take k_bgram_mfma (FP32 MFMA tiles)
+//                [X R P]' [X R P] (upper triangles, the problem's columns only)
+//                of FP64 blocks; FP32 blocks take k_bgram_mfma (FP32 MFMA tiles)
 //   k_bgram_sum  per-problem fixed-order sums of the partials
 //   k_rr_tri_batched  one workgroup per problem: the single-problem path's
 //                tridiagonal Rayleigh-Ritz solver (rr_fast.hip) on the
@@ -121,13 +121,12 @@ __global__ void k_bres_sum(const int *act_prob, const int *nact, const int *pc_b
 constexpr int kHM = kM / 2;                               // block rows / columns
 constexpr int kBlk = kHM * (kHM + 1) / 2;                 // upper 2 x 2 blocks
 constexpr int kBlkPerThread = (kBlk + kT - 1) / kT;
-// FP32 blocks are staged in FP32 and each kSub-row step is summed in FP32
-// before it is added to the FP64 accumulators (FP32 rounding never spans
-// more than kSub rows); FP64 blocks accumulate in FP64 throughout.
-template <typename T>
-__global__ void __launch_bounds__(kT) k_bgram(Chunks ch, Probs pr, long ld, int nb, const T *X,
-                                              const T *R, const T *P, const T *AX, const T *AR,
-                                              const T *AP, double *part) {
+// FP64 blocks (FP32 blocks take k_bgram_mfma).
+__global__ void __launch_bounds__(kT) k_bgram(Chunks ch, Probs pr, long ld, int nb,
+                                              const double *X, const double *R, const double *P,
+                                              const double *AX, const double *AR,
+                                              const double *AP, double *part) {
+    using T = double;
     __shared__ T Ls[kSub][kM + 2];
     __shared__ T As[kSub][kM + 2];
     const int c = blockIdx.x;
@@ -176,12 +175,8 @@ __global__ void __launch_bounds__(kT) k_bgram(Chunks ch, Probs pr, long ld, int 
             const int o = threadIdx.x + s * kT;
             if (o < nblk) {
                 const int i = bi[s], j = bj[s];
-                // FP64: the accumulators themselves; FP32: this step's sums
-                constexpr bool f64 = std::is_same<T, double>::value;
-                T a0 = f64 ? (T)ga[s][0] : T(0), a1 = f64 ? (T)ga[s][1] : T(0);
-                T a2 = f64 ? (T)ga[s][2] : T(0), a3 = f64 ? (T)ga[s][3] : T(0);
-                T g0 = f64 ? (T)gb[s][0] : T(0), g1 = f64 ? (T)gb[s][1] : T(0);
-                T g2 = f64 ? (T)gb[s][2] : T(0), g3 = f64 ? (T)gb[s][3] : T(0);
+                T a0 = ga[s][0], a1 = ga[s][1], a2 = ga[s][2], a3 = ga[s][3];
+                T g0 = gb[s][0], g1 = gb[s][1], g2 = gb[s][2], g3 = gb[s][3];
                 for (int rr = 0; rr < rows; rr++) {
                     const T l0 = Ls[rr][i], l1 = Ls[rr][i + 1];
                     const T x0 = As[rr][j], x1 = As[rr][j + 1];
@@ -195,13 +190,8 @@ __global__ void __launch_bounds__(kT) k_bgram(Chunks ch, Probs pr, long ld, int 
                     g2 = fma(l0, y1, g2);
                     g3 = fma(l1, y1, g3);
                 }
-                if (f64) {
-                    ga[s][0] = a0; ga[s][1] = a1; ga[s][2] = a2; ga[s][3] = a3;
-                    gb[s][0] = g0; gb[s][1] = g1; gb[s][2] = g2; gb[s][3] = g3;
-                } else {
-                    ga[s][0] += a0; ga[s][1] += a1; ga[s][2] += a2; ga[s][3] += a3;
-                    gb[s][0] += g0; gb[s][1] += g1; gb[s][2] += g2; gb[s][3] += g3;
-                }
+                ga[s][0] = a0; ga[s][1] = a1; ga[s][2] = a2; ga[s][3] = a3;
+                gb[s][0] = g0; gb[s][1] = g1; gb[s][2] = g2; gb[s][3] = g3;
             }
         }
     }
@@ -961,7 +951,7 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
             hipLaunchKernelGGL(k_bgram_mfma, dim3(nch), dim3(kT), 0, stream(), ch, pr, N, nb,
                                bk.X, bk.R, bk.P, bk.AX, bk.AR, bk.AP, part_g);
         else
-            hipLaunchKernelGGL(k_bgram<T>, dim3(nch), dim3(kT), 0, stream(), ch, pr, N, nb,
+            hipLaunchKernelGGL(k_bgram, dim3(nch), dim3(kT), 0, stream(), ch, pr, N, nb,
                                bk.X, bk.R, bk.P, bk.AX, bk.AR, bk.AP, part_g);
     };
     // grids cover every chunk / problem; workgroups past the device counts
