@@ -238,7 +238,7 @@ int device_lobpcg_complex(mv_MultiVectorPtr X, void *Ad, Op opA, void *Bd, Op op
             printf("%d constraint%s\n\n", ny, ny > 1 ? "s" : "");
     }
     std::vector<double> lr(m), rr(m);
-    const size_t hist = (size_t)m * (maxit + 1);
+    const size_t hist = (size_t)m * (size_t)(maxit > 0 ? maxit + 1 : 1);
     std::vector<double> lhr(lamh ? hist : 0), rhr(resh ? hist : 0);
     const int flag = device_lobpcg(xr.mv, &za, zop_apply, opB ? &zb : nullptr,
                                    opB ? zop_apply : nullptr, opT ? &zt : nullptr,
