@@ -1,12 +1,13 @@
 // Complex (komplex: interleaved FP64 real, imaginary) block kernels for the
 // hip_Multi_Vector slots that lobpcg_solve_complex drives: the Gram product
 // X^H Y and its diagonal, block times small complex matrix, column scaling by
-// complex diagonals, and a dense complex operator (the reference's
-// MatMultiVec over a dense multivector, blopex_serial_complex/multivector/
-// multi_vector.c:567-610).  Copies, clears, real axpy and the drand48 fill
+// complex diagonals, and a dense complex operator on FP64 MFMA (the
+// reference's MatMultiVec over a dense multivector,
+// blopex_serial_complex/multivector/multi_vector.c:567-610).  Copies, clears, real axpy and the drand48 fill
 // treat a complex column as 2n doubles and use the FP64 kernels.
 // Reductions are two-stage with a fixed summation order, as in blas.hip.
 #include "kernels.h"
+#include "mfma.h"
 
 namespace bx {
 
@@ -134,20 +135,100 @@ __global__ void __launch_bounds__(kThreads) k_zscale(long n, const double2 *d, c
 }
 
 // y(:, ym[j]) = A x(:, xm[j]), A dense n x n (leading dim lda)
-__global__ void __launch_bounds__(kThreads) k_zdense(long n, const double2 *A, long lda,
-                                                     const double2 *x, long ldx, ColMap xm,
-                                                     double2 *y, long ldy, ColMap ym) {
-    const int j = blockIdx.y;
-    const double2 *xc = x + (size_t)xm.idx[j] * ldx;
-    double2 *yc = y + (size_t)ym.idx[j] * ldy;
+// Y(:, ym) = A X(:, xm), A dense n x n, on FP64 MFMA 16x16x4: a wave owns 16
+// rows of Y and a slice of the inner index, with Y_r = A_r X_r - A_i X_i and
+// Y_i = A_r X_i + A_i X_r as four real products per step of 4, and CT tiles of
+// 16 X columns (columns c0 .. c0 + w of the maps).  Lane (li, lk) supplies A's
+// row r0 + li and X's column 16 c + li at inner index k + lk; entries past n
+// or w are read from a clamped place and zeroed.  One slice writes Y itself;
+// several write partials part[s][col][row] that k_zdense_sum adds in order.
+template <int CT>
+__global__ void __launch_bounds__(64) k_zdense_mfma(long n, const double2 *__restrict__ A,
+                                                    long lda, const double2 *__restrict__ x,
+                                                    long ldx, ColMap xm, int c0, int w,
+                                                    long kslice, double2 *__restrict__ y,
+                                                    long ldy, ColMap ym, double2 *part) {
+    const int lane = threadIdx.x, li = lane & 15, lk = lane >> 4;
+    const long r0 = (long)blockIdx.x * 16, row = r0 + li;
+    const long kb = (long)blockIdx.y * kslice, ke = kb + kslice < n ? kb + kslice : n;
+    const bool rok = row < n;
+    const double2 *ar = A + (rok ? row : n - 1);
+    const double2 *xc[CT];
+    bool cok[CT];
+#pragma unroll
+    for (int c = 0; c < CT; c++) {
+        const int col = 16 * c + li;
+        cok[c] = col < w;
+        xc[c] = x + (size_t)xm.idx[c0 + (cok[c] ? col : w - 1)] * ldx;
+    }
+    f64x4 yr[CT], yi[CT];
+#pragma unroll
+    for (int c = 0; c < CT; c++) yr[c] = yi[c] = f64x4{0.0, 0.0, 0.0, 0.0};
+    auto load = [&](long k, double2 &a, double2 (&b)[CT]) {
+        const long kk = k + lk;
+        const bool kok = kk < ke;
+        const long kc = kok ? kk : ke - 1;
+        const double2 av = ar[(size_t)kc * lda];
+        a = (kok && rok) ? av : make_double2(0.0, 0.0);
+#pragma unroll
+        for (int c = 0; c < CT; c++) {
+            const double2 bv = xc[c][kc];
+            b[c] = (kok && cok[c]) ? bv : make_double2(0.0, 0.0);
+        }
+    };
+    // kD steps of loads in flight: a ring refilled as each step is used
+    // (steps past the slice read clamped places and come out zero)
+    constexpr int kD = 4;
+    double2 a[kD], b[kD][CT];
+#pragma unroll
+    for (int d = 0; d < kD; d++) load(kb + 4 * d, a[d], b[d]);
+    for (long k = kb; k < ke; k += 4 * kD) {
+#pragma unroll
+        for (int d = 0; d < kD; d++) {
+#pragma unroll
+            for (int c = 0; c < CT; c++) {
+                yr[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[d].x, b[d][c].x, yr[c], 0, 0, 0);
+                yi[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[d].x, b[d][c].y, yi[c], 0, 0, 0);
+            }
+#pragma unroll
+            for (int c = 0; c < CT; c++) {
+                yr[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[d].y, b[d][c].y, yr[c], 0, 0, 0);
+                yi[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[d].y, b[d][c].x, yi[c], 0, 0, 0);
+            }
+            load(k + 4 * (d + kD), a[d], b[d]);
+        }
+    }
+    // register q of lane (li, lk): row r0 + lk + 4 q, column 16 c + li
+#pragma unroll
+    for (int c = 0; c < CT; c++) {
+        const int col = 16 * c + li;
+        if (col >= w) continue;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const long r = r0 + lk + 4 * q;
+            if (r >= n) continue;
+            const double2 v = make_double2(yr[c][q], yi[c][q]);
+            if (gridDim.y == 1)
+                y[(size_t)ym.idx[c0 + col] * ldy + r] = v;
+            else
+                part[((size_t)blockIdx.y * w + col) * n + r] = v;
+        }
+    }
+}
+
+// y(:, ym[c0 + col]) = sum over the slices of part[s][col][:], in slice order
+__global__ void __launch_bounds__(kThreads) k_zdense_sum(long n, int w, int nslice,
+                                                         const double2 *part, double2 *y,
+                                                         long ldy, ColMap ym, int c0) {
+    const int col = blockIdx.y;
     for (long r = (long)blockIdx.x * kThreads + threadIdx.x; r < n; r += (long)gridDim.x * kThreads) {
         double2 s = make_double2(0.0, 0.0);
-        for (long i = 0; i < n; i++) {
-            const double2 p = cmul(A[(size_t)i * lda + r], xc[i]);
-            s.x += p.x;
-            s.y += p.y;
+        for (int q = 0; q < nslice; q++) {
+            const double2 v = part[((size_t)q * w + col) * n + r];
+            s.x += v.x;
+            s.y += v.y;
         }
-        yc[r] = s;
+        y[(size_t)ym.idx[c0 + col] * ldy + r] = s;
     }
 }
 
@@ -202,10 +283,34 @@ void zdense_apply(long n, const void *A, long lda, const void *x, long ldx, cons
                   void *y, long ldy, const ColMap &ym) {
     BX_REQUIRE(xm.n == ym.n, "dense operator: active column counts differ (%d vs %d)", xm.n, ym.n);
     if (xm.n == 0 || n == 0) return;
-    hipLaunchKernelGGL(k_zdense, dim3(zrow_grid(n), xm.n), dim3(kThreads), 0, stream(), n,
-                       (const double2 *)A, lda, (const double2 *)x, ldx, xm, (double2 *)y, ldy,
-                       ym);
-    BX_HIP(hipGetLastError());
+    // slices of the inner index so that about 2048 waves run (two per SIMD)
+    const long tiles = (n + 15) / 16;
+    long ns = (2048 + tiles - 1) / tiles;
+    if (ns > (n + 3) / 4) ns = (n + 3) / 4;
+    const long kslice = ((n + ns - 1) / ns + 3) / 4 * 4;
+    ns = (n + kslice - 1) / kslice;
+    BX_REQUIRE(tiles <= 0x7fffffffL && ns <= 65535, "dense operator: %ld rows", n);
+    for (int c0 = 0; c0 < xm.n; c0 += 64) {
+        const int w = xm.n - c0 < 64 ? xm.n - c0 : 64;
+        double2 *part = ns > 1 ? (double2 *)scratch(15, (size_t)ns * w * n * sizeof(double2))
+                               : nullptr;
+        const dim3 g((unsigned)tiles, (unsigned)ns);
+        const int ct = (w + 15) / 16;
+#define BX_ZD(CT)                                                                            \
+    hipLaunchKernelGGL(k_zdense_mfma<CT>, g, dim3(64), 0, stream(), n, (const double2 *)A, lda, \
+                       (const double2 *)x, ldx, xm, c0, w, kslice, (double2 *)y, ldy, ym, part)
+        if (ct == 1) BX_ZD(1);
+        else if (ct == 2) BX_ZD(2);
+        else if (ct == 3) BX_ZD(3);
+        else BX_ZD(4);
+#undef BX_ZD
+        BX_HIP(hipGetLastError());
+        if (ns > 1) {
+            hipLaunchKernelGGL(k_zdense_sum, dim3(zrow_grid(n), w), dim3(kThreads), 0, stream(), n,
+                               w, (int)ns, part, (double2 *)y, ldy, ym, c0);
+            BX_HIP(hipGetLastError());
+        }
+    }
 }
 
 void z_to_real(long n, int w, const void *z, long ldz, void *r, long ldr, int rprec, bool jpair) {
