@@ -570,15 +570,30 @@ void HipDiagMatMultiVec(void *D_, void *X_, void *Y_) {
 }
 
 void HipDenseMatMultiVec(void *A_, void *X_, void *Y_) {
-    const hip_Multi_Vector *A = H(A_), *X = H(X_), *Y = H(Y_);
-    BX_REQUIRE(A->precision == HIPMV_C64 && X->precision == HIPMV_C64 &&
-                   Y->precision == HIPMV_C64,
-               "HipDenseMatMultiVec: A, X and Y must be complex blocks");
+    const hip_Multi_Vector *X = H(X_), *Y = H(Y_);
+    BX_REQUIRE(X->size == Y->size, "HipDenseMatMultiVec: X has %ld rows, Y %ld", X->size, Y->size);
+    BX_REQUIRE(X->precision == Y->precision, "HipDenseMatMultiVec: X and Y differ in precision");
+    if (!A_) {
+        BX_REQUIRE(X->precision != HIPMV_C64, "HipDenseMatMultiVec: the 1D Laplacian takes real blocks");
+        lapl1d_apply(X->precision, X->size, X->data, X->ld, colmap_of(X), Y->data, Y->ld,
+                     colmap_of(Y));
+        return;
+    }
+    const hip_Multi_Vector *A = H(A_);
     BX_REQUIRE(A->size == A->num_vectors && A->size == X->size && X->size == Y->size,
                "HipDenseMatMultiVec: A is %ld x %d, X has %ld rows, Y %ld", A->size,
                (int)A->num_vectors, X->size, Y->size);
-    zdense_apply(A->size, A->data, A->ld, X->data, X->ld, colmap_of(X), Y->data, Y->ld,
-                 colmap_of(Y));
+    if (A->precision == HIPMV_C64) {
+        BX_REQUIRE(X->precision == HIPMV_C64,
+                   "HipDenseMatMultiVec: a complex A takes complex blocks");
+        zdense_apply(A->size, A->data, A->ld, X->data, X->ld, colmap_of(X), Y->data, Y->ld,
+                     colmap_of(Y));
+        return;
+    }
+    BX_REQUIRE(A->precision == HIPMV_FP64 && X->precision != HIPMV_C64,
+               "HipDenseMatMultiVec: a real A must be FP64 and take real blocks");
+    dense_apply(X->precision, A->size, (const double *)A->data, A->ld, X->data, X->ld,
+                colmap_of(X), Y->data, Y->ld, colmap_of(Y));
 }
 
 }  // extern "C"

@@ -154,6 +154,13 @@ void zscale_cols(long n, const void *d, const void *x, long ldx, const ColMap &x
 void z_to_real(long n, int w, const void *z, long ldz, void *r, long ldr, int rprec, bool jpair);
 // z_j = r_j[0:n] + i r_j[n:2n] for the w columns of r
 void real_to_z(long n, int w, const void *r, long ldr, int rprec, void *z, long ldz);
+// y(:, ym) = A x(:, xm), A dense FP64 n x n (leading dim lda), x and y FP64 or
+// FP32 (prec), FP64 arithmetic
+void dense_apply(int prec, long n, const double *A, long lda, const void *x, long ldx,
+                 const ColMap &xm, void *y, long ldy, const ColMap &ym);
+// y(:, ym) = L x(:, xm), L the n x n 1D Laplacian [-1 2 -1] with Dirichlet ends
+void lapl1d_apply(int prec, long n, const void *x, long ldx, const ColMap &xm, void *y, long ldy,
+                  const ColMap &ym);
 // y(:, ym) = A x(:, xm), A dense komplex n x n
 void zdense_apply(long n, const void *A, long lda, const void *x, long ldx, const ColMap &xm,
                   void *y, long ldy, const ColMap &ym);

@@ -235,6 +235,8 @@ class Phase {
             bytes = A->ndiag > 0 ? (double)A->ndiag * A->diag_ld * es_ + 2.0 * n_ * w * es_
                                  : (double)A->nnz * (es_ + 4) + (double)(A->n + 1) * 4 +
                                        2.0 * n_ * w * es_;
+        } else if (isA && op == HipDenseMatMultiVec) {
+            bytes = d ? (double)n_ * n_ * 8 + 2.0 * n_ * w * es_ : 2.0 * n_ * w * es_;
         }
         tm_->begin(isA ? kSpmm : kOther);
         op(d, &vx, &vy);

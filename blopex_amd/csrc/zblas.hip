@@ -1,10 +1,13 @@
 // Complex (komplex: interleaved FP64 real, imaginary) block kernels for the
 // hip_Multi_Vector slots that lobpcg_solve_complex drives: the Gram product
 // X^H Y and its diagonal, block times small complex matrix, column scaling by
-// complex diagonals, and a dense complex operator on FP64 MFMA (the
-// reference's MatMultiVec over a dense multivector,
-// blopex_serial_complex/multivector/multi_vector.c:567-610).  Copies, clears, real axpy and the drand48 fill
-// treat a complex column as 2n doubles and use the FP64 kernels.
+// complex diagonals, and the reference's MatMultiVec operators: a dense
+// operator on FP64 MFMA for complex and real blocks
+// (blopex_serial_complex/multivector/multi_vector.c:567-610,
+// blopex_serial_double/pcg_multi/pcg_multi.c:167-193) and the 1D Laplacian it
+// applies when given no matrix (blopex_serial_double/matmultivec/
+// matmultivec.c:11-48).  Copies, clears, real axpy and the drand48 fill treat a
+// complex column as 2n doubles and use the FP64 kernels.
 // Reductions are two-stage with a fixed summation order, as in blas.hip.
 #include "kernels.h"
 #include "mfma.h"
@@ -232,6 +235,135 @@ __global__ void __launch_bounds__(kThreads) k_zdense_sum(long n, int w, int nsli
     }
 }
 
+// The real form of the same product, Y(:, ym) = A X(:, xm) with A dense FP64
+// and X, Y in FP64 or FP32 (FP64 arithmetic): one 16x16x4 product per step
+// and column tile, the same slices, ring of loads and partial layout.
+template <typename T, int CT>
+__global__ void __launch_bounds__(64) k_dense_mfma(long n, const double *__restrict__ A, long lda,
+                                                   const T *__restrict__ x, long ldx, ColMap xm,
+                                                   int c0, int w, long kslice, T *__restrict__ y,
+                                                   long ldy, ColMap ym, double *part) {
+    const int lane = threadIdx.x, li = lane & 15, lk = lane >> 4;
+    const long r0 = (long)blockIdx.x * 16, row = r0 + li;
+    const long kb = (long)blockIdx.y * kslice, ke = kb + kslice < n ? kb + kslice : n;
+    const bool rok = row < n;
+    const double *ar = A + (rok ? row : n - 1);
+    const T *xc[CT];
+    bool cok[CT];
+#pragma unroll
+    for (int c = 0; c < CT; c++) {
+        const int col = 16 * c + li;
+        cok[c] = col < w;
+        xc[c] = x + (size_t)xm.idx[c0 + (cok[c] ? col : w - 1)] * ldx;
+    }
+    f64x4 acc[CT];
+#pragma unroll
+    for (int c = 0; c < CT; c++) acc[c] = f64x4{0.0, 0.0, 0.0, 0.0};
+    auto load = [&](long k, double &a, double (&b)[CT]) {
+        const long kk = k + lk;
+        const bool kok = kk < ke;
+        const long kc = kok ? kk : ke - 1;
+        const double av = ar[(size_t)kc * lda];
+        a = (kok && rok) ? av : 0.0;
+#pragma unroll
+        for (int c = 0; c < CT; c++) {
+            const double bv = (double)xc[c][kc];
+            b[c] = (kok && cok[c]) ? bv : 0.0;
+        }
+    };
+    constexpr int kD = 4;
+    double a[kD], b[kD][CT];
+#pragma unroll
+    for (int d = 0; d < kD; d++) load(kb + 4 * d, a[d], b[d]);
+    for (long k = kb; k < ke; k += 4 * kD) {
+#pragma unroll
+        for (int d = 0; d < kD; d++) {
+#pragma unroll
+            for (int c = 0; c < CT; c++)
+                acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[d], b[d][c], acc[c], 0, 0, 0);
+            load(k + 4 * (d + kD), a[d], b[d]);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < CT; c++) {
+        const int col = 16 * c + li;
+        if (col >= w) continue;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const long r = r0 + lk + 4 * q;
+            if (r >= n) continue;
+            if (gridDim.y == 1)
+                y[(size_t)ym.idx[c0 + col] * ldy + r] = (T)acc[c][q];
+            else
+                part[((size_t)blockIdx.y * w + col) * n + r] = acc[c][q];
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_dense_sum(long n, int w, int nslice,
+                                                        const double *part, T *y, long ldy,
+                                                        ColMap ym, int c0) {
+    const int col = blockIdx.y;
+    for (long r = (long)blockIdx.x * kThreads + threadIdx.x; r < n; r += (long)gridDim.x * kThreads) {
+        double s = 0.0;
+        for (int q = 0; q < nslice; q++) s += part[((size_t)q * w + col) * n + r];
+        y[(size_t)ym.idx[c0 + col] * ldy + r] = (T)s;
+    }
+}
+
+// 1D Laplacian with Dirichlet ends, y_j = -x_{j-1} + 2 x_j - x_{j+1} on each
+// active column (the stencil of matmultivec.c:36-45), FP64 arithmetic
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_lapl1d(long n, const T *x, long ldx, ColMap xm,
+                                                     T *y, long ldy, ColMap ym) {
+    const T *xc = x + (size_t)xm.idx[blockIdx.y] * ldx;
+    T *yc = y + (size_t)ym.idx[blockIdx.y] * ldy;
+    for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < n; i += (long)gridDim.x * kThreads) {
+        double v = 2.0 * (double)xc[i];
+        if (i > 0) v -= (double)xc[i - 1];
+        if (i + 1 < n) v -= (double)xc[i + 1];
+        yc[i] = (T)v;
+    }
+}
+
+// slices of the inner index so that about 2048 waves run (two per SIMD)
+void dense_slices(long n, long &tiles, long &ns, long &kslice) {
+    tiles = (n + 15) / 16;
+    ns = (2048 + tiles - 1) / tiles;
+    if (ns > (n + 3) / 4) ns = (n + 3) / 4;
+    kslice = ((n + ns - 1) / ns + 3) / 4 * 4;
+    ns = (n + kslice - 1) / kslice;
+    BX_REQUIRE(tiles <= 0x7fffffffL && ns <= 65535, "dense operator: %ld rows", n);
+}
+
+template <typename T>
+void dense_apply_t(long n, const double *A, long lda, const T *x, long ldx, const ColMap &xm,
+                   T *y, long ldy, const ColMap &ym) {
+    long tiles, ns, kslice;
+    dense_slices(n, tiles, ns, kslice);
+    for (int c0 = 0; c0 < xm.n; c0 += 64) {
+        const int w = xm.n - c0 < 64 ? xm.n - c0 : 64;
+        double *part = ns > 1 ? (double *)scratch(15, (size_t)ns * w * n * sizeof(double)) : nullptr;
+        const dim3 g((unsigned)tiles, (unsigned)ns);
+        const int ct = (w + 15) / 16;
+#define BX_DD(CT)                                                                             \
+    hipLaunchKernelGGL((k_dense_mfma<T, CT>), g, dim3(64), 0, stream(), n, A, lda, x, ldx, xm, c0, \
+                       w, kslice, y, ldy, ym, part)
+        if (ct == 1) BX_DD(1);
+        else if (ct == 2) BX_DD(2);
+        else if (ct == 3) BX_DD(3);
+        else BX_DD(4);
+#undef BX_DD
+        BX_HIP(hipGetLastError());
+        if (ns > 1) {
+            hipLaunchKernelGGL(k_dense_sum<T>, dim3(zrow_grid(n), w), dim3(kThreads), 0, stream(), n,
+                               w, (int)ns, part, y, ldy, ym, c0);
+            BX_HIP(hipGetLastError());
+        }
+    }
+}
+
 void zgram_launch(long n, const void *x, long ldx, const ColMap &xm, const void *y, long ldy,
                   const ColMap &ym, int diag, void *g, long ldg) {
     const int h = xm.n, npairs = diag ? xm.n : xm.n * ym.n;
@@ -283,13 +415,8 @@ void zdense_apply(long n, const void *A, long lda, const void *x, long ldx, cons
                   void *y, long ldy, const ColMap &ym) {
     BX_REQUIRE(xm.n == ym.n, "dense operator: active column counts differ (%d vs %d)", xm.n, ym.n);
     if (xm.n == 0 || n == 0) return;
-    // slices of the inner index so that about 2048 waves run (two per SIMD)
-    const long tiles = (n + 15) / 16;
-    long ns = (2048 + tiles - 1) / tiles;
-    if (ns > (n + 3) / 4) ns = (n + 3) / 4;
-    const long kslice = ((n + ns - 1) / ns + 3) / 4 * 4;
-    ns = (n + kslice - 1) / kslice;
-    BX_REQUIRE(tiles <= 0x7fffffffL && ns <= 65535, "dense operator: %ld rows", n);
+    long tiles, ns, kslice;
+    dense_slices(n, tiles, ns, kslice);
     for (int c0 = 0; c0 < xm.n; c0 += 64) {
         const int w = xm.n - c0 < 64 ? xm.n - c0 : 64;
         double2 *part = ns > 1 ? (double2 *)scratch(15, (size_t)ns * w * n * sizeof(double2))
@@ -311,6 +438,31 @@ void zdense_apply(long n, const void *A, long lda, const void *x, long ldx, cons
             BX_HIP(hipGetLastError());
         }
     }
+}
+
+void dense_apply(int prec, long n, const double *A, long lda, const void *x, long ldx,
+                 const ColMap &xm, void *y, long ldy, const ColMap &ym) {
+    BX_REQUIRE(xm.n == ym.n, "dense operator: active column counts differ (%d vs %d)", xm.n, ym.n);
+    if (xm.n == 0 || n == 0) return;
+    if (prec == HIPMV_FP64)
+        dense_apply_t<double>(n, A, lda, (const double *)x, ldx, xm, (double *)y, ldy, ym);
+    else
+        dense_apply_t<float>(n, A, lda, (const float *)x, ldx, xm, (float *)y, ldy, ym);
+}
+
+void lapl1d_apply(int prec, long n, const void *x, long ldx, const ColMap &xm, void *y, long ldy,
+                  const ColMap &ym) {
+    BX_REQUIRE(xm.n == ym.n, "1D Laplacian: active column counts differ (%d vs %d)", xm.n, ym.n);
+    BX_REQUIRE(n > 1, "1D Laplacian: %ld rows (at least 2 needed)", n);
+    if (xm.n == 0) return;
+    const dim3 g(zrow_grid(n), xm.n);
+    if (prec == HIPMV_FP64)
+        hipLaunchKernelGGL(k_lapl1d<double>, g, dim3(kThreads), 0, stream(), n, (const double *)x,
+                           ldx, xm, (double *)y, ldy, ym);
+    else
+        hipLaunchKernelGGL(k_lapl1d<float>, g, dim3(kThreads), 0, stream(), n, (const float *)x,
+                           ldx, xm, (float *)y, ldy, ym);
+    BX_HIP(hipGetLastError());
 }
 
 void z_to_real(long n, int w, const void *z, long ldz, void *r, long ldr, int rprec, bool jpair) {

@@ -95,13 +95,32 @@ class DeviceBlock:
             self.ptr = None
 
 
+class DeviceDense:
+    """operatorA data for HipDenseMatMultiVec, the reference's MatMultiVec
+    (blopex_serial_double/pcg_multi/pcg_multi.c:167-193): a dense n x n FP64
+    (or complex) matrix on the device, or, with A None, no matrix, which
+    makes the callback apply the 1D Laplacian (matmultivec.c:11-48)."""
+
+    def __init__(self, A: np.ndarray | None, n: int | None = None):
+        self.lib = L.load()
+        self.block = None if A is None else DeviceBlock.from_numpy(np.asarray(A))
+        self.n = n if A is None else A.shape[0]
+        self.ptr = None if self.block is None else self.block.ptr
+        self.op = L.symbol_address("HipDenseMatMultiVec")
+
+    def close(self):
+        if self.block is not None:
+            self.block.close()
+            self.block = self.ptr = None
+
+
 def interpreter() -> L.Interpreter:
     ii = L.Interpreter()
     L.load().HipSetupInterpreter(ctypes.byref(ii))
     return ii
 
 
-def solve_device(A: DeviceCSR, X: DeviceBlock, B=None, T=None, Y: DeviceBlock | None = None,
+def solve_device(A: DeviceCSR | DeviceDense, X: DeviceBlock, B=None, T=None, Y: DeviceBlock | None = None,
                  tol: float = 1e-6, rtol: float = 1e-50, maxit: int = 200, verbosity: int = 0,
                  mode: str = "fp64", switch_factor: float = 1.5, stall_iterations: int = 50,
                  refine_iterations: int = 0, histories: bool = True, profile: bool = False,

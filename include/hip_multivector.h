@@ -211,10 +211,13 @@ void hip_DiagMatrixDestroy(hip_DiagMatrix *D);
 /* operatorB / operatorT callback: Y(active) = diag(d) * X(active) */
 void HipDiagMatMultiVec(void *D, void *X, void *Y);
 
-/* operatorA callback for a dense complex matrix: A is a hip_Multi_Vector of
-   n x n HIPMV_C64 entries (all columns), Y(active) = A * X(active) -- the
-   reference's MatMultiVec over a dense multivector (blopex_serial_complex/
-   pcg_multi/pcg_multi.c, multi_vector.c:567-610) */
+/* operatorA callback MatMultiVec (blopex_serial_double/pcg_multi/
+   pcg_multi.c:167-193, blopex_serial_complex/pcg_multi/pcg_multi.c,
+   multi_vector.c:567-610): A is a hip_Multi_Vector of n x n entries (all
+   columns), Y(active) = A * X(active).  A complex (HIPMV_C64) A takes complex
+   blocks; a real (HIPMV_FP64) A takes FP64 or FP32 blocks (FP64 arithmetic).
+   A == NULL applies the 1D Laplacian [-1 2 -1] with Dirichlet ends to real
+   blocks, as serial_Lapl1DMatMultiVec does (matmultivec.c:11-48). */
 void HipDenseMatMultiVec(void *A, void *X, void *Y);
 
 #ifdef __cplusplus
