@@ -236,52 +236,73 @@ __global__ void __launch_bounds__(kThreads) k_zdense_sum(long n, int w, int nsli
 }
 
 // The real form of the same product, Y(:, ym) = A X(:, xm) with A dense FP64
-// and X, Y in FP64 or FP32 (FP64 arithmetic): one 16x16x4 product per step
-// and column tile, the same slices, ring of loads and partial layout.
-template <typename T, int CT>
-__global__ void __launch_bounds__(64) k_dense_mfma(long n, const double *__restrict__ A, long lda,
-                                                   const T *__restrict__ x, long ldx, ColMap xm,
-                                                   int c0, int w, long kslice, T *__restrict__ y,
-                                                   long ldy, ColMap ym, double *part) {
-    const int lane = threadIdx.x, li = lane & 15, lk = lane >> 4;
-    const long r0 = (long)blockIdx.x * 16, row = r0 + li;
+// and X, Y in FP64 or FP32 (FP64 arithmetic).  A wave owns RT tiles of 16
+// rows (rows r0 + 16 t + li), so each X operand it loads feeds RT products;
+// per step of 4 it issues RT x CT 16x16x4 products, with KD steps of loads in
+// flight.  Loads read clamped places (rows past n are never stored); the main
+// loop runs whole rounds of KD steps inside the slice, and a last partial
+// round zeroes the steps past it.  Slices and the partial layout are those of
+// the complex kernel.
+template <typename T, int CT, int RT, int KD, int WV>
+__global__ void __launch_bounds__(64 * WV) k_dense_mfma(long n, const double *__restrict__ A,
+                                                        long lda, const T *__restrict__ x, long ldx,
+                                                        ColMap xm, int c0, int w, long kslice,
+                                                        T *__restrict__ y, long ldy, ColMap ym,
+                                                        double *part) {
+    const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+    const long r0 = ((long)blockIdx.x * WV + (threadIdx.x >> 6)) * 16 * RT;
     const long kb = (long)blockIdx.y * kslice, ke = kb + kslice < n ? kb + kslice : n;
-    const bool rok = row < n;
-    const double *ar = A + (rok ? row : n - 1);
+    const double *ar[RT];
+#pragma unroll
+    for (int t = 0; t < RT; t++) {
+        const long row = r0 + 16 * t + li;
+        ar[t] = A + (row < n ? row : n - 1);
+    }
     const T *xc[CT];
-    bool cok[CT];
 #pragma unroll
     for (int c = 0; c < CT; c++) {
         const int col = 16 * c + li;
-        cok[c] = col < w;
-        xc[c] = x + (size_t)xm.idx[c0 + (cok[c] ? col : w - 1)] * ldx;
+        xc[c] = x + (size_t)xm.idx[c0 + (col < w ? col : w - 1)] * ldx;
     }
-    f64x4 acc[CT];
+    f64x4 acc[RT][CT];
 #pragma unroll
-    for (int c = 0; c < CT; c++) acc[c] = f64x4{0.0, 0.0, 0.0, 0.0};
-    auto load = [&](long k, double &a, double (&b)[CT]) {
-        const long kk = k + lk;
-        const bool kok = kk < ke;
-        const long kc = kok ? kk : ke - 1;
-        const double av = ar[(size_t)kc * lda];
-        a = (kok && rok) ? av : 0.0;
+    for (int t = 0; t < RT; t++)
 #pragma unroll
-        for (int c = 0; c < CT; c++) {
-            const double bv = (double)xc[c][kc];
-            b[c] = (kok && cok[c]) ? bv : 0.0;
-        }
+        for (int c = 0; c < CT; c++) acc[t][c] = f64x4{0.0, 0.0, 0.0, 0.0};
+    auto load = [&](long k, double (&a)[RT], double (&b)[CT]) {
+        const long kc = k + lk < n ? k + lk : n - 1;
+#pragma unroll
+        for (int t = 0; t < RT; t++) a[t] = ar[t][(size_t)kc * lda];
+#pragma unroll
+        for (int c = 0; c < CT; c++) b[c] = (double)xc[c][kc];
     };
-    constexpr int kD = 4;
-    double a[kD], b[kD][CT];
+    auto mma = [&](const double (&a)[RT], const double (&b)[CT]) {
 #pragma unroll
-    for (int d = 0; d < kD; d++) load(kb + 4 * d, a[d], b[d]);
-    for (long k = kb; k < ke; k += 4 * kD) {
-#pragma unroll
-        for (int d = 0; d < kD; d++) {
+        for (int t = 0; t < RT; t++)
 #pragma unroll
             for (int c = 0; c < CT; c++)
-                acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[d], b[d][c], acc[c], 0, 0, 0);
-            load(k + 4 * (d + kD), a[d], b[d]);
+                acc[t][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], b[c], acc[t][c], 0, 0, 0);
+    };
+    double a[KD][RT], b[KD][CT];
+#pragma unroll
+    for (int d = 0; d < KD; d++) load(kb + 4 * d, a[d], b[d]);
+    long k = kb;
+    for (; k + 4 * KD <= ke; k += 4 * KD) {
+#pragma unroll
+        for (int d = 0; d < KD; d++) {
+            mma(a[d], b[d]);
+            load(k + 4 * (d + KD), a[d], b[d]);
+        }
+    }
+    if (k < ke) {
+#pragma unroll
+        for (int d = 0; d < KD; d++) {
+            const bool kok = k + 4 * d + lk < ke;
+#pragma unroll
+            for (int t = 0; t < RT; t++) a[d][t] = kok ? a[d][t] : 0.0;
+#pragma unroll
+            for (int c = 0; c < CT; c++) b[d][c] = kok ? b[d][c] : 0.0;
+            mma(a[d], b[d]);
         }
     }
 #pragma unroll
@@ -289,14 +310,16 @@ __global__ void __launch_bounds__(64) k_dense_mfma(long n, const double *__restr
         const int col = 16 * c + li;
         if (col >= w) continue;
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const long r = r0 + lk + 4 * q;
-            if (r >= n) continue;
-            if (gridDim.y == 1)
-                y[(size_t)ym.idx[c0 + col] * ldy + r] = (T)acc[c][q];
-            else
-                part[((size_t)blockIdx.y * w + col) * n + r] = acc[c][q];
-        }
+        for (int t = 0; t < RT; t++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const long r = r0 + 16 * t + lk + 4 * q;
+                if (r >= n) continue;
+                if (gridDim.y == 1)
+                    y[(size_t)ym.idx[c0 + col] * ldy + r] = (T)acc[t][c][q];
+                else
+                    part[((size_t)blockIdx.y * w + col) * n + r] = acc[t][c][q];
+            }
     }
 }
 
@@ -305,10 +328,17 @@ __global__ void __launch_bounds__(kThreads) k_dense_sum(long n, int w, int nslic
                                                         const double *part, T *y, long ldy,
                                                         ColMap ym, int c0) {
     const int col = blockIdx.y;
+    const size_t qs = (size_t)w * n;
     for (long r = (long)blockIdx.x * kThreads + threadIdx.x; r < n; r += (long)gridDim.x * kThreads) {
-        double s = 0.0;
-        for (int q = 0; q < nslice; q++) s += part[((size_t)q * w + col) * n + r];
-        y[(size_t)ym.idx[c0 + col] * ldy + r] = (T)s;
+        // four running sums over the slices q = j (mod 4), then added in a fixed order
+        const double *p = part + (size_t)col * n + r;
+        double s[4] = {0.0, 0.0, 0.0, 0.0};
+        int q = 0;
+        for (; q + 4 <= nslice; q += 4)
+#pragma unroll
+            for (int j = 0; j < 4; j++) s[j] += p[(q + j) * qs];
+        for (; q < nslice; q++) s[q & 3] += p[q * qs];
+        y[(size_t)ym.idx[c0 + col] * ldy + r] = (T)((s[0] + s[1]) + (s[2] + s[3]));
     }
 }
 
@@ -327,12 +357,21 @@ __global__ void __launch_bounds__(kThreads) k_lapl1d(long n, const T *x, long ld
     }
 }
 
-// slices of the inner index so that about 2048 waves run (two per SIMD)
-void dense_slices(long n, long &tiles, long &ns, long &kslice) {
-    tiles = (n + 15) / 16;
-    ns = (2048 + tiles - 1) / tiles;
+// Shape of the real dense product: 4 row tiles of 16 per wave, 4 steps of
+// loads in flight, 4 waves per workgroup (256 rows), one wave per SIMD.
+// Measured at n = 16000, 32 columns: 532 us against 366 us with the loads
+// cut out and 412 us with the products cut out; 2 tiles per wave, 8 steps in
+// flight, 8 waves per workgroup or 2 waves per SIMD were no faster.
+constexpr int kDenseRT = 4, kDenseKD = 4, kDenseWV = 4, kDenseWaves = 1024;
+
+// slices of the inner index (lengths a multiple of kmult) so that about
+// `units` workgroups run, each owning tile_rows rows
+void dense_slices(long n, long tile_rows, long units, long kmult, long &tiles, long &ns,
+                  long &kslice) {
+    tiles = (n + tile_rows - 1) / tile_rows;
+    ns = (units + tiles - 1) / tiles;
     if (ns > (n + 3) / 4) ns = (n + 3) / 4;
-    kslice = ((n + ns - 1) / ns + 3) / 4 * 4;
+    kslice = ((n + ns - 1) / ns + kmult - 1) / kmult * kmult;
     ns = (n + kslice - 1) / kslice;
     BX_REQUIRE(tiles <= 0x7fffffffL && ns <= 65535, "dense operator: %ld rows", n);
 }
@@ -340,16 +379,25 @@ void dense_slices(long n, long &tiles, long &ns, long &kslice) {
 template <typename T>
 void dense_apply_t(long n, const double *A, long lda, const T *x, long ldx, const ColMap &xm,
                    T *y, long ldy, const ColMap &ym) {
+    constexpr int RT = kDenseRT, WV = kDenseWV;
     long tiles, ns, kslice;
-    dense_slices(n, tiles, ns, kslice);
+    // a whole number of workgroups per CU where it can (kDenseWaves waves on
+    // 256 CUs), so that the SIMDs hold equal numbers of waves, and slices of
+    // at least 128 inner indices (32 steps per wave)
+    long wgs = (long)kDenseWaves * num_cus() / (256L * WV);
+    tiles = (n + 16 * RT * WV - 1) / (16 * RT * WV);
+    long per = wgs / tiles;
+    if (per > n / 128) per = n / 128;
+    if (per < 1) per = 1;
+    dense_slices(n, 16 * RT * WV, per * tiles, 4 * kDenseKD, tiles, ns, kslice);
     for (int c0 = 0; c0 < xm.n; c0 += 64) {
         const int w = xm.n - c0 < 64 ? xm.n - c0 : 64;
         double *part = ns > 1 ? (double *)scratch(15, (size_t)ns * w * n * sizeof(double)) : nullptr;
         const dim3 g((unsigned)tiles, (unsigned)ns);
         const int ct = (w + 15) / 16;
 #define BX_DD(CT)                                                                             \
-    hipLaunchKernelGGL((k_dense_mfma<T, CT>), g, dim3(64), 0, stream(), n, A, lda, x, ldx, xm, c0, \
-                       w, kslice, y, ldy, ym, part)
+    hipLaunchKernelGGL((k_dense_mfma<T, CT, RT, kDenseKD, WV>), g, dim3(64 * WV), 0, stream(), n,    \
+                       A, lda, x, ldx, xm, c0, w, kslice, y, ldy, ym, part)
         if (ct == 1) BX_DD(1);
         else if (ct == 2) BX_DD(2);
         else if (ct == 3) BX_DD(3);
@@ -416,7 +464,7 @@ void zdense_apply(long n, const void *A, long lda, const void *x, long ldx, cons
     BX_REQUIRE(xm.n == ym.n, "dense operator: active column counts differ (%d vs %d)", xm.n, ym.n);
     if (xm.n == 0 || n == 0) return;
     long tiles, ns, kslice;
-    dense_slices(n, tiles, ns, kslice);
+    dense_slices(n, 16, 2048, 4, tiles, ns, kslice);
     for (int c0 = 0; c0 < xm.n; c0 += 64) {
         const int w = xm.n - c0 < 64 ? xm.n - c0 : 64;
         double2 *part = ns > 1 ? (double2 *)scratch(15, (size_t)ns * w * n * sizeof(double2))
