@@ -7,9 +7,10 @@ Bars: products in FP64 agree with numpy to 1e-13 of |A| |X| (summation order
 differs); FP32 blocks (FP64 arithmetic, one rounding at the store) to 2^-23 of
 |A| |X|; the stencil to one rounding (exact on FP32 blocks).  Operand
 mismatches abort the process, as the reference's assert does, so they are not
-exercised here.  The stock serial driver (40 x 40 diag(1..40) stored dense, k = 5, drand48 seed 1, tol 1e-6, maxit 100) reproduces the
-compiled reference's flag, eigenvalues (1e-8 relative), first Ritz values
-(1e-12) and iteration count (within max(2, 5%)).
+exercised here.  The stock serial driver (40 x 40 diag(1..40) stored dense,
+k = 5, drand48 seed 1, tol 1e-6, maxit 100) reproduces the compiled
+reference's flag, eigenvalues (1e-8 relative), first Ritz values (1e-12) and
+iteration count (within max(2, 5%)).
 """
 import ctypes
 import sys
