@@ -50,8 +50,11 @@ constexpr double kEps = 2.220446049250313e-16;
 // where the two pencil matrices live
 constexpr int kBothLds = 0, kALdsB = 1, kBothGlobal = 2;  // kALdsB: A global, B in LDS
 
-// static LDS of the body (the small arrays), for sizing the dynamic part
-constexpr size_t static_lds_bytes(int nm, int km) { return (size_t)(11 * nm + 2 * km + 4) * 8; }
+// static LDS of the body (the small arrays: eleven of NM doubles, two of KM,
+// sc[4] and cpiv[1]), for sizing the dynamic part; rounded up to 16 bytes
+constexpr size_t static_lds_bytes(int nm, int km) {
+    return ((size_t)(11 * nm + 2 * km + 5) * 8 + 15) / 16 * 16;
+}
 // leading dimension of the pencil arrays: 3 (mod 32), which spreads rows,
 // columns and 8 x 8 blocks of doubles over the LDS banks
 __host__ __device__ inline int pencil_ld(int n0) { return n0 + ((3 - n0) & 31); }
