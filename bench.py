@@ -170,7 +170,8 @@ def run_batched(count, graph=True, maxit=2000, mode="mixed", compare_fp64=True):
            "wall_seconds": wall, "iterations_max": int(r.iterations.max()),
            "iterations_mean": float(r.iterations.mean()),
            "iterations_fp32": r.stats["iterations_fp32"],
-           "graph_replays": r.stats["graph_replays"], "chunk_rows": r.stats["chunk_rows"]}
+           "graph_replays": r.stats["graph_replays"], "chunk_rows": r.stats["chunk_rows"],
+           "gram_rows": r.stats["gram_rows"]}
     if compare_fp64 and mode != "fp64":
         f = lobpcg_batched(problems, tol=1e-6, maxit=maxit, graph=graph)
         out["fp64"] = {"seconds": f.stats["seconds"], "converged": int(np.sum(f.flags == 0)),

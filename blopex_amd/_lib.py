@@ -65,7 +65,7 @@ class BatchSettings(ctypes.Structure):
 class BatchStats(ctypes.Structure):
     _fields_ = [("seconds", c_double), ("iterations", c_int), ("graph_captures", c_int),
                 ("graph_replays", c_int), ("total_rows", c_long), ("iterations_fp32", c_int),
-                ("chunk_rows", c_int)]
+                ("chunk_rows", c_int), ("gram_rows", c_int)]
 
 
 class HipMultiVector(ctypes.Structure):

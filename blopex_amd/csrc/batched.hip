@@ -214,7 +214,7 @@ __global__ void __launch_bounds__(kT) k_bgram(Chunks ch, Probs pr, long ld, int 
     }
 }
 
-// FP32 blocks on MFMA: the chunk's rows in 32-row steps staged in LDS (the
+// FP32 blocks on MFMA: the chunk's rows in 16-, 32- or 64-row steps staged in LDS (the
 // problem's m = nb kp columns of L = [X R P] and of [AX AR AP], m padded to
 // whole 16-column tiles; the next step is loaded into registers while this
 // one is multiplied), every upper tile product (ti <= tj) of GA = L' A L and
@@ -227,8 +227,9 @@ constexpr int kMP = kMT * 16;                          // padded columns
 constexpr int kTilePairs = kMT * (kMT + 1) / 2;        // upper tile pairs
 constexpr int kMProd = 2 * kTilePairs;                 // GA and GB
 constexpr int kMPerWave = (kMProd + kT / 64 - 1) / (kT / 64);
-constexpr int kMRows = 32;                             // rows per step
 constexpr int kMS = kMP + 4;                           // LDS row stride (floats)
+// kMRows: rows per staged step (16, 32 or 64; the autotuner picks one)
+template <int kMRows>
 __global__ void __launch_bounds__(kT) k_bgram_mfma(Chunks ch, Probs pr, long ld, int nb,
                                                    const float *X, const float *R,
                                                    const float *P, const float *AX,
@@ -944,12 +945,17 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
             BX_HIP(hipMemsetAsync(*q, 0, nk * 4, stream()));
         }
     }
-    // the Gram partials: FP32 blocks on MFMA, FP64 blocks on FP64 FMA
+    // the Gram partials: FP32 blocks on MFMA (gram_rows rows per staged step,
+    // the autotuner's choice), FP64 blocks on FP64 FMA
+    int gram_rows = 32;
     auto launch_gram = [&](auto &bk, int nb) {
         using T = typename std::remove_reference<decltype(bk)>::type::type;
         if constexpr (std::is_same<T, float>::value)
-            hipLaunchKernelGGL(k_bgram_mfma, dim3(nch), dim3(kT), 0, stream(), ch, pr, N, nb,
-                               bk.X, bk.R, bk.P, bk.AX, bk.AR, bk.AP, part_g);
+            hipLaunchKernelGGL(gram_rows == 16   ? k_bgram_mfma<16>
+                               : gram_rows == 64 ? k_bgram_mfma<64>
+                                                 : k_bgram_mfma<32>,
+                               dim3(nch), dim3(kT), 0, stream(), ch, pr, N, nb, bk.X, bk.R, bk.P,
+                               bk.AX, bk.AR, bk.AP, part_g);
         else
             hipLaunchKernelGGL(k_bgram, dim3(nch), dim3(kT), 0, stream(), ch, pr, N, nb,
                                bk.X, bk.R, bk.P, bk.AX, bk.AR, bk.AP, part_g);
@@ -1050,20 +1056,19 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
         }
         return pit;
     };
-    // Autotuner: the rows per chunk (the work of one workgroup in the grouped
-    // kernels) that make the SpMM and the three-block Gram pass fastest on
-    // this batch, timed with HIP events on the blocks of the first phase
-    // (still zero: the kernels' cost does not depend on the values)
+    // Autotuner: the launch shapes that make the SpMM and the three-block
+    // Gram pass fastest on this batch, timed with HIP events on the blocks of
+    // the first phase (still zero: the kernels' cost does not depend on the
+    // values).  Coordinate search: the rows per chunk (the work of one
+    // workgroup in the grouped kernels), then for FP32 blocks the Gram's
+    // staged rows per step (16, 32, 64).
     auto tune = [&](auto &bk) {
         using T = typename std::remove_reference<decltype(bk)>::type::type;
         const T *vals = std::is_same<T, float>::value ? (const T *)A->val32 : (const T *)A->val;
         hipEvent_t e0, e1;
         BX_HIP(hipEventCreate(&e0));
         BX_HIP(hipEventCreate(&e1));
-        float best = 1e30f;
-        for (int rows : kChunkCand) {
-            nch = make_chunks(rows);
-            rebuild();
+        auto time_pass = [&]() {
             float ms = 0.f;
             for (int rep = 0; rep < 3; rep++) {
                 BX_HIP(hipEventRecord(e0, stream()));
@@ -1076,15 +1081,34 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
                 BX_HIP(hipEventElapsedTime(&t, e0, e1));
                 if (rep > 0) ms += t;  // the first launch warms up
             }
+            return ms;
+        };
+        float best = 1e30f;
+        for (int rows : kChunkCand) {
+            nch = make_chunks(rows);
+            rebuild();
+            const float ms = time_pass();
             if (ms < best) {
                 best = ms;
                 chunk_rows = rows;
             }
         }
-        BX_HIP(hipEventDestroy(e0));
-        BX_HIP(hipEventDestroy(e1));
         nch = make_chunks(chunk_rows);
         rebuild();
+        if (std::is_same<T, float>::value) {
+            int bestg = gram_rows;
+            for (int gr : {16, 64}) {
+                gram_rows = gr;
+                const float ms = time_pass();
+                if (ms < best) {
+                    best = ms;
+                    bestg = gr;
+                }
+            }
+            gram_rows = bestg;
+        }
+        BX_HIP(hipEventDestroy(e0));
+        BX_HIP(hipEventDestroy(e1));
     };
     if (s.autotune) {
         if (mixed)
@@ -1189,6 +1213,7 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
         st->iterations = it;
         st->iterations_fp32 = it32;
         st->chunk_rows = chunk_rows;
+        st->gram_rows = gram_rows;
         st->graph_captures = captures;
         st->graph_replays = replays;
         st->total_rows = N;

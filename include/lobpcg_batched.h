@@ -50,9 +50,11 @@ typedef struct hip_BatchSettings {
     double switch_factor; /* MIXED: see mode (default 100: on the 2D 512-problem workload
                              30, 100, 300, 1000 and 3000 took 3.15, 3.07, 3.08, 3.21 and
                              3.36 s) */
-    int autotune;       /* 1: pick the rows per chunk (the work of one workgroup: 512, 1024,
-                           2048 or 4096) by timing the SpMM and Gram kernels on this batch
-                           before the solve (default 1); 0: 2048 */
+    int autotune;       /* 1: pick the launch shapes by timing the SpMM and Gram kernels on
+                           this batch before the solve: the rows per chunk (the work of one
+                           workgroup: 512, 1024, 2048 or 4096), then for MIXED the FP32
+                           Gram's staged rows per step (16, 32 or 64) (default 1);
+                           0: 2048 rows per chunk, 32 rows per step */
 } hip_BatchSettings;
 
 typedef struct hip_BatchStats {
@@ -64,6 +66,7 @@ typedef struct hip_BatchStats {
     long total_rows;    /* sum of n */
     int iterations_fp32; /* MIXED: batch iterations of the FP32 phase */
     int chunk_rows;     /* rows per chunk used */
+    int gram_rows;      /* rows per staged step of the FP32 Gram pass used */
 } hip_BatchStats;
 
 void hip_BatchSettingsDefault(hip_BatchSettings *s);

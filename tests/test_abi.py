@@ -67,10 +67,11 @@ def test_struct_layouts():
     assert ctypes.sizeof(L.AdjointSettings) == 24 and L.AdjointSettings.maxit.offset == 16
     assert ctypes.sizeof(L.AdjointStats) == 24 and L.AdjointStats.seconds.offset == 16
     # hip_BatchProblem {long, long, 3 pointers, int}, hip_BatchSettings {double, 5 ints},
-    # hip_BatchStats {double, 3 ints, long}
+    # hip_BatchStats {double, 3 ints, long, 3 ints}
     assert ctypes.sizeof(L.BatchProblem) == 48 and L.BatchProblem.k.offset == 40
     assert ctypes.sizeof(L.BatchSettings) == 48 and L.BatchSettings.autotune.offset == 40
-    assert ctypes.sizeof(L.BatchStats) == 40 and L.BatchStats.chunk_rows.offset == 36
+    assert ctypes.sizeof(L.BatchStats) == 48 and L.BatchStats.chunk_rows.offset == 36
+    assert L.BatchStats.gram_rows.offset == 40
 
 
 def test_adjoint_defaults_and_argument_checks(lib):

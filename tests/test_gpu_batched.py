@@ -77,7 +77,19 @@ def test_batched_workload_generator_reduced():
     r = bl.lobpcg_batched(problems, tol=1e-6, maxit=1000, want_X=True)
     say(f"  max iterations {r.iterations.max()} stats {r.stats}")
     assert r.stats["chunk_rows"] in (512, 1024, 2048, 4096)
+    assert r.stats["gram_rows"] == 32  # FP64 blocks: the VALU Gram, not tuned
     _check(problems, r, 1e-6, 1e-8)
+
+
+def test_batched_mixed_tunes_the_gram_stage():
+    """MIXED: the autotuner also picks the FP32 Gram's staged rows per step;
+    every stage height gives the same eigenpairs to the tolerance."""
+    say("batched solve, 24 problems, mixed, autotuned launch shapes")
+    problems = pb.batched_problems(24, smin=16, smax=64)
+    r = bl.lobpcg_batched(problems, tol=1e-6, maxit=1000, mode="mixed", want_X=True)
+    say(f"  stats {r.stats}")
+    assert r.stats["gram_rows"] in (16, 32, 64)
+    _check(problems, r, 1e-6, 1e-7)
 
 
 def test_batched_argument_errors():
