@@ -27,7 +27,7 @@ event time.  Its `traffic` (HBM bytes per launch) is read from the
 committed rocprofv3 counter summary of `bench.py --roofline-only`
 (FETCH_SIZE and WRITE_SIZE passes scaled by a 512 MiB copy).  cpu_baseline
 is the reference's own lobpcg_solve_double (oracle/_ref) on this host and
-this operator, one thread, five iterations.
+this operator, one thread, four iterations (three with the P block).
 
     python bench.py [--steps K] [--warmup W] [--n 128] [--k 32] [--guard G] [--no-cpu]
 """
@@ -81,8 +81,9 @@ class HipEvents:
         return ms.value
 
 
-def spmm_roofline(lib, dA, A, k, prec, reps=30, force_csr=False):
-    """Average duration of the SpMM kernel (HipCSRMatMultiVec) over `reps` back-to-back launches;
+def spmm_roofline(lib, dA, A, k, prec, reps=100, force_csr=False):
+    """Average duration of the SpMM kernel (HipCSRMatMultiVec) over `reps` back-to-back launches
+    after ten untimed ones;
     algorithmic bytes of the format the kernel reads (the z-marching and diagonal kernels read
     the diagonal copy, the CSR kernel values, columns and row pointers)."""
     from blopex_amd import _lib as L
@@ -99,7 +100,7 @@ def spmm_roofline(lib, dA, A, k, prec, reps=30, force_csr=False):
     call = lambda: lib.HipCSRMatMultiVec(ctypes.cast(dA.ptr, ctypes.c_void_p),
                                          ctypes.cast(X.ptr, ctypes.c_void_p),
                                          ctypes.cast(Y.ptr, ctypes.c_void_p))
-    for _ in range(3):
+    for _ in range(10):
         call()
     lib.blopex_amd_synchronize()
     ev = HipEvents()
@@ -121,13 +122,15 @@ def spmm_roofline(lib, dA, A, k, prec, reps=30, force_csr=False):
     return ms, algo
 
 
-def cpu_baseline(A, k, seed_x=1, iterations=5):
+def cpu_baseline(A, k, seed_x=1, iterations=4):
     """The reference's own lobpcg_solve_double (oracle/_ref) on this workload
     (the same CSR operator, Jacobi T, k, X0), one thread, on a bounded sample:
-    five iterations.  The harness stamps CLOCK_MONOTONIC at every operator-A
-    call (the initial AX, then one per iteration), so the times between the
-    calls of iterations 1..5 are four full-width iterations (no column
-    converges that early); their mean is the reported s/iteration."""
+    four iterations.  The harness stamps CLOCK_MONOTONIC at every operator-A
+    call (the initial AX, then one per iteration).  Iteration 1 has no P
+    block (its Rayleigh-Ritz basis is [X R], 2k columns); from iteration 2 on
+    the basis is [X R P] with every column active (none converges that early),
+    so the intervals between the calls of iterations 2..4 are full-width
+    iterations, and their mean is the reported s/iteration."""
     from threadpoolctl import threadpool_limits
 
     from oracle import pyoracle as po
@@ -143,12 +146,16 @@ def cpu_baseline(A, k, seed_x=1, iterations=5):
     if len(st) < 3:
         return None
     per = [float(st[i + 1] - st[i]) for i in range(1, len(st) - 1)]
-    return {"s_per_iteration": float(np.mean(per)), "s_per_iteration_each": per,
+    if len(per) < 2:
+        return None
+    return {"s_per_iteration": float(np.mean(per[1:])), "s_per_iteration_each": per[1:],
+            "s_first_iteration_without_p": per[0],
             "setup_s": float(st[1] - st[0]), "sample_wall_s": wall,
             "sample": f"reference lobpcg_solve_double on the same 128^3 operator, Jacobi, k={k}, "
                       f"X0 = drand48 seed {seed_x}, maxit {iterations}, 1 thread ({wall:.1f} s "
                       f"of CPU work); s/iteration = mean time between the operator-A calls of "
-                      f"iterations 1..{iterations}"}
+                      f"iterations 2..{iterations} (basis [X R P], all columns active; "
+                      f"iteration 1, without P, reported apart)"}
 
 
 def run_batched(count, graph=True, maxit=2000, mode="mixed", compare_fp64=True):
@@ -542,6 +549,7 @@ def main():
                 "value": cb["s_per_iteration"], "unit": "s per full-width iteration (k=32)",
                 "iterations_per_s": 1.0 / cb["s_per_iteration"], "cores": 1, "kind": "reference",
                 "s_per_iteration_each": cb["s_per_iteration_each"],
+                "s_first_iteration_without_p": cb["s_first_iteration_without_p"],
                 "setup_s": cb["setup_s"], "sample": cb["sample"],
                 "gpu_fp64_ms_per_iteration": gpu_it,
                 "extrapolated_time_to_tolerance_s": cb["s_per_iteration"] * its,

@@ -20,7 +20,7 @@ namespace {
 hip_LobpcgSettings default_settings() {
     hip_LobpcgSettings s;
     s.mode = HIPLOBPCG_FP64;
-    s.switch_factor = 1.5;
+    s.switch_factor = 8.0;
     s.stall_iterations = 50;
     s.refine_iterations = 0;
     s.profile = 0;

@@ -122,7 +122,7 @@ def interpreter() -> L.Interpreter:
 
 def solve_device(A: DeviceCSR | DeviceDense, X: DeviceBlock, B=None, T=None, Y: DeviceBlock | None = None,
                  tol: float = 1e-6, rtol: float = 1e-50, maxit: int = 200, verbosity: int = 0,
-                 mode: str = "fp64", switch_factor: float = 1.5, stall_iterations: int = 50,
+                 mode: str = "fp64", switch_factor: float = 8.0, stall_iterations: int = 50,
                  refine_iterations: int = 0, histories: bool = True, profile: bool = False,
                  refresh_interval: int = 20, final_rr: bool = False, guard_columns: int = 0,
                  eig_rtol: float = 0.0, cond_restart: bool = False, size_guard: bool = False,
