@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "dense.h"
+#include "dpp.h"
 #include "kernels.h"
 #include "lobpcg_batched.h"
 #include "mfma.h"
@@ -68,37 +69,78 @@ __device__ __forceinline__ double blk_get(const T *b, long ld, long row, int col
 }
 
 // ---- residual, Jacobi, per-chunk squared norms ----
+// A thread owns rows t, t + kT, ... of the chunk and keeps the squared norms
+// of all the problem's columns (KB of them: kp rounded up to 8, 16 or 24)
+// in registers; the loads of eight columns of a row are issued together.
+// One reduction per chunk at the end (DPP sums in each wave, then the four
+// waves in order).
+template <int KB, typename T>
+__device__ __forceinline__ void bresid_rows(int kp, long ld, long r0, int nr, const T *X,
+                                            const T *AX, const double *lj, const double *dinv,
+                                            T *R, double (&red)[kT / 64][kK], double *out) {
+    double acc[KB];
+#pragma unroll
+    for (int j = 0; j < KB; j++) acc[j] = 0.0;
+    for (int t = threadIdx.x; t < nr; t += kT) {
+        const long r = r0 + t;
+        const double di = dinv ? dinv[r] : 1.0;
+#pragma unroll
+        for (int g = 0; g < KB; g += 8) {
+            if (g < kp) {  // uniform
+                T ax[8], xv[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const bool ok = g + u < kp;
+                    ax[u] = ok ? AX[(size_t)(g + u) * ld + r] : T(0);
+                    xv[u] = ok ? X[(size_t)(g + u) * ld + r] : T(0);
+                }
+#pragma unroll
+                for (int u = 0; u < 8; u++)
+                    if (g + u < kp) {
+                        double v = (double)ax[u] - lj[g + u] * (double)xv[u];
+                        acc[g + u] = fma(v, v, acc[g + u]);
+                        if (dinv) v *= di;
+                        R[(size_t)(g + u) * ld + r] = (T)v;
+                    }
+            }
+        }
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int j = 0; j < KB; j++)
+        if (j < kp) {
+            const double sj = wave_sum(acc[j]);
+            if (lane == 0) red[wave][j] = sj;
+        }
+    __syncthreads();
+    if ((int)threadIdx.x < kp)
+        out[threadIdx.x] = (red[0][threadIdx.x] + red[1][threadIdx.x]) +
+                           (red[2][threadIdx.x] + red[3][threadIdx.x]);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kT) k_bresid(Chunks ch, Probs pr, long ld, const T *X,
                                                const T *AX, const double *lam,
                                                const double *dinv, T *R, double *part) {
-    __shared__ double red[kT];
+    static_assert(kT == 256, "four waves");
+    __shared__ double red[kT / 64][kK];
+    __shared__ double lj[kK];
     const int c = blockIdx.x;
     if (c >= *ch.count) return;
     const int p = ch.prob[c];
     const int kp = pr.k[p];
     const long r0 = ch.r0[c];
     const int nr = ch.nr[c];
+    if ((int)threadIdx.x < kK) lj[threadIdx.x] = lam[p * kK + threadIdx.x];
+    __syncthreads();
     // the problem's columns only: R's other columns are never written and stay zero
-    for (int j = 0; j < kp; j++) {
-        double acc = 0.0;
-        const double l = lam[p * kK + j];
-        for (int t = threadIdx.x; t < nr; t += kT) {
-            const long r = r0 + t;
-            double v = blk_get(AX, ld, r, j) - l * blk_get(X, ld, r, j);
-            acc = fma(v, v, acc);
-            if (dinv) v *= dinv[r];
-            R[(size_t)j * ld + r] = (T)v;
-        }
-        red[threadIdx.x] = acc;
-        __syncthreads();
-        for (int w = kT / 2; w > 0; w >>= 1) {
-            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) part[(size_t)c * kK + j] = red[0];
-        __syncthreads();
-    }
+    double *out = part + (size_t)c * kK;
+    if (kp <= 8)
+        bresid_rows<8, T>(kp, ld, r0, nr, X, AX, lj, dinv, R, red, out);
+    else if (kp <= 16)
+        bresid_rows<16, T>(kp, ld, r0, nr, X, AX, lj, dinv, R, red, out);
+    else
+        bresid_rows<kK, T>(kp, ld, r0, nr, X, AX, lj, dinv, R, red, out);
 }
 
 // res2[p K + j] = sum of problem p's chunk partials (fixed order)
@@ -610,7 +652,7 @@ __global__ void __launch_bounds__(kRT) k_brr(const int *act_prob, const int *nac
 // them: the problem's column count rounded up to 8, 16 or 24, chosen per
 // workgroup), the coefficients sit in LDS in that precision (each read, the
 // same address across the wave, feeds the FMAs of a whole wave), and the
-// loads of four input columns are issued before their FMAs.
+// loads of all of a block's columns are issued before their FMAs.
 template <int KB, typename T>
 __device__ __forceinline__ void bupd_rows(const T *cs, int kp, int nb, long ld, long r0, int nr,
                                           T *b0, const T *b1, T *b2) {
@@ -622,15 +664,13 @@ __device__ __forceinline__ void bupd_rows(const T *cs, int kp, int nb, long ld, 
         for (int j = 0; j < KB; j++) xa[j] = pa[j] = T(0);
         for (int b = 0; b < nb; b++) {
             const T *src = bl[b] + r;
-            for (int c0 = 0; c0 < kp; c0 += 4) {
-                T u[4];
+            T u[KB];
 #pragma unroll
-                for (int d = 0; d < 4; d++)
-                    u[d] = c0 + d < kp ? src[(size_t)(c0 + d) * ld] : T(0);
+            for (int d = 0; d < KB; d++) u[d] = d < kp ? src[(size_t)d * ld] : T(0);
 #pragma unroll
-                for (int d = 0; d < 4; d++) {
-                    if (c0 + d >= kp) break;
-                    const T *cq = cs + (b * kp + c0 + d);
+            for (int d = 0; d < KB; d++) {
+                if (d < kp) {  // uniform
+                    const T *cq = cs + (b * kp + d);
                     if (b == 0) {
 #pragma unroll
                         for (int j = 0; j < KB; j++)
