@@ -69,24 +69,24 @@ __device__ __forceinline__ double blk_get(const T *b, long ld, long row, int col
 }
 
 // ---- residual, Jacobi, per-chunk squared norms ----
-// A thread owns rows t, t + kT, ... of the chunk and keeps the squared norms
-// of all the problem's columns (KB of them: kp rounded up to 8, 16 or 24)
-// in registers; the loads of eight columns of a row are issued together.
-// One reduction per chunk at the end (DPP sums in each wave, then the four
-// waves in order).
+// The problem's columns in groups of eight (KB: kp rounded up to 8, 16 or
+// 24): a thread owns rows t, t + kT, ... of the chunk, issues the loads of a
+// row's eight columns together and keeps their squared norms in registers;
+// each group ends with DPP sums in every wave, and the four waves' sums are
+// added in order once per chunk.
 template <int KB, typename T>
 __device__ __forceinline__ void bresid_rows(int kp, long ld, long r0, int nr, const T *X,
                                             const T *AX, const double *lj, const double *dinv,
                                             T *R, double (&red)[kT / 64][kK], double *out) {
-    double acc[KB];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
-    for (int j = 0; j < KB; j++) acc[j] = 0.0;
-    for (int t = threadIdx.x; t < nr; t += kT) {
-        const long r = r0 + t;
-        const double di = dinv ? dinv[r] : 1.0;
+    for (int g = 0; g < KB; g += 8) {
+        if (g < kp) {  // uniform
+            double acc[8];
 #pragma unroll
-        for (int g = 0; g < KB; g += 8) {
-            if (g < kp) {  // uniform
+            for (int u = 0; u < 8; u++) acc[u] = 0.0;
+            for (int t = threadIdx.x; t < nr; t += kT) {
+                const long r = r0 + t;
                 T ax[8], xv[8];
 #pragma unroll
                 for (int u = 0; u < 8; u++) {
@@ -94,24 +94,24 @@ __device__ __forceinline__ void bresid_rows(int kp, long ld, long r0, int nr, co
                     ax[u] = ok ? AX[(size_t)(g + u) * ld + r] : T(0);
                     xv[u] = ok ? X[(size_t)(g + u) * ld + r] : T(0);
                 }
+                const double di = dinv ? dinv[r] : 1.0;
 #pragma unroll
                 for (int u = 0; u < 8; u++)
                     if (g + u < kp) {
                         double v = (double)ax[u] - lj[g + u] * (double)xv[u];
-                        acc[g + u] = fma(v, v, acc[g + u]);
+                        acc[u] = fma(v, v, acc[u]);
                         if (dinv) v *= di;
                         R[(size_t)(g + u) * ld + r] = (T)v;
                     }
             }
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (g + u < kp) {
+                    const double sj = wave_sum(acc[u]);
+                    if (lane == 0) red[wave][g + u] = sj;
+                }
         }
     }
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-#pragma unroll
-    for (int j = 0; j < KB; j++)
-        if (j < kp) {
-            const double sj = wave_sum(acc[j]);
-            if (lane == 0) red[wave][j] = sj;
-        }
     __syncthreads();
     if ((int)threadIdx.x < kp)
         out[threadIdx.x] = (red[0][threadIdx.x] + red[1][threadIdx.x]) +
