@@ -150,8 +150,18 @@ __global__ void k_bres_sum(const int *act_prob, const int *nact, const int *pc_b
     const int p = act_prob[blockIdx.x];
     const int j = threadIdx.x;
     if (j >= kK) return;
+    // chunk by chunk in order; eight chunks' loads in flight at a time
     double s = 0.0;
-    for (int c = pc_begin[p], e = pc_begin[p] + pc_count[p]; c < e; c++) s += part[(size_t)c * kK + j];
+    int c = pc_begin[p];
+    const int e = pc_begin[p] + pc_count[p];
+    for (; c + 8 <= e; c += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = part[(size_t)(c + u) * kK + j];
+#pragma unroll
+        for (int u = 0; u < 8; u++) s += v[u];
+    }
+    for (; c < e; c++) s += part[(size_t)c * kK + j];
     res2[p * kK + j] = s;
 }
 
@@ -386,8 +396,24 @@ __global__ void __launch_bounds__(kT) k_bgram_sum(const int *act_prob, const int
         while ((j + 1) * (j + 2) / 2 <= o) j++;
         while (j * (j + 1) / 2 > o) j--;
         const int i = o - j * (j + 1) / 2;
+        // chunk by chunk in order; eight chunks' loads in flight at a time
         double a = 0.0, g = 0.0;
-        for (int c = pc_begin[p], e = pc_begin[p] + pc_count[p]; c < e; c++) {
+        int c = pc_begin[p];
+        const int e = pc_begin[p] + pc_count[p];
+        for (; c + 8 <= e; c += 8) {
+            double va[8], vg[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                va[u] = part[((size_t)(c + u) * 2 + 0) * kPairs + o];
+                vg[u] = part[((size_t)(c + u) * 2 + 1) * kPairs + o];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                a += va[u];
+                g += vg[u];
+            }
+        }
+        for (; c < e; c++) {
             a += part[((size_t)c * 2 + 0) * kPairs + o];
             g += part[((size_t)c * 2 + 1) * kPairs + o];
         }
