@@ -205,23 +205,41 @@ __global__ void __launch_bounds__(kT) k_bgram(Chunks ch, Probs pr, long ld, int 
 #pragma unroll
         for (int q = 0; q < 4; q++) ga[s][q] = gb[s][q] = 0.0;
     }
+    // staged columns 0 .. 2 hm (an odd m gets a zero column); element
+    // e = threadIdx.x + kT u of a step is row e % kSub of column e / kSub.
+    // The next step's elements are loaded into registers while this step's
+    // products run.
+    constexpr int kPer = (kSub * (kM + 2) + kT - 1) / kT;
+    const int nst = kSub * 2 * hm;
+    T lreg[kPer], areg[kPer];
+    auto fetch = [&](int s0) {
+        const int rows = min(kSub, nr - s0);
+#pragma unroll
+        for (int u = 0; u < kPer; u++) {
+            const int e = threadIdx.x + kT * u, rr = e % kSub, q = e / kSub;
+            lreg[u] = areg[u] = T(0);
+            if (e < nst && rr < rows && q < m) {
+                const int b = q / kp, col = q % kp;
+                const size_t o = (size_t)col * ld + (size_t)(r0 + s0 + rr);
+                lreg[u] = Lb[b][o];
+                areg[u] = Ab[b][o];
+            }
+        }
+    };
+    fetch(0);
     for (int s0 = 0; s0 < nr; s0 += kSub) {
         const int rows = min(kSub, nr - s0);
         __syncthreads();
-        // staged columns 0 .. 2 hm (an odd m gets a zero column)
-        for (int e = threadIdx.x; e < kSub * 2 * hm; e += kT) {
-            const int rr = e % kSub, q = e / kSub;
-            T lv = T(0), av = T(0);
-            if (rr < rows && q < m) {
-                const int b = q / kp, col = q % kp;
-                const long r = r0 + s0 + rr;
-                lv = Lb[b][(size_t)col * ld + r];
-                av = Ab[b][(size_t)col * ld + r];
+#pragma unroll
+        for (int u = 0; u < kPer; u++) {
+            const int e = threadIdx.x + kT * u, rr = e % kSub, q = e / kSub;
+            if (e < nst) {
+                Ls[rr][q] = lreg[u];
+                As[rr][q] = areg[u];
             }
-            Ls[rr][q] = lv;
-            As[rr][q] = av;
         }
         __syncthreads();
+        if (s0 + kSub < nr) fetch(s0 + kSub);
 #pragma unroll
         for (int s = 0; s < kBlkPerThread; s++) {
             const int o = threadIdx.x + s * kT;
