@@ -168,7 +168,8 @@ void launch_update(const UpdKArgs &a, int ngroups) {
 struct UpdRArgs {
     UpdKArgs u;         // g[0] = X group, g[1] = AX group
     const double *lam;  // new Ritz values, by output column
-    const float *tdiag; // diagonal preconditioner or null
+    const float *tdiag; // diagonal preconditioner (n entries) or null
+    long n;             // rows of the problem (the rows from n to u.ld are padding)
     float *r;           // residual block (tail columns), leading dimension u.ld
     double *part;       // [tail column][nparts]
     int nparts;         // workgroups in the grid
@@ -228,8 +229,18 @@ __global__ void __launch_bounds__(kUT) k_update_resid(UpdRArgs a) {
         upd_block<float, NOT>((const float *)gx.in[0], gx.h[0], gx.crow[0], Cl, u.ld, r0, li, lk, xx);
         upd_store<float, NOT>((float *)gx.out_full, u.ld, r0, pfull, xx);
         // R = T (X' lam - AX') for the tail columns, as the residual kernel forms it
+        // (the diagonal has n entries: padding rows, whose X' and AX' are
+        // zero, keep the factor 1)
         float td[4] = {1.f, 1.f, 1.f, 1.f};
-        if (a.tdiag) load4(a.tdiag + r0, td);
+        if (a.tdiag) {
+            if (r0 + 3 < a.n) {
+                load4(a.tdiag + r0, td);
+            } else {
+#pragma unroll
+                for (int t = 0; t < 4; t++)
+                    if (r0 + t < a.n) td[t] = a.tdiag[r0 + t];
+            }
+        }
 #pragma unroll
         for (int o = 0; o < NOT; o++)
 #pragma unroll
@@ -296,10 +307,11 @@ void mfma_layout_probe(int prec, int *rows, int *cols) {
     BX_HIP(hipFree(d));
 }
 
-void block_update_resid(long ld, const UpdGroup *groups, const double *C, int ldc, int crows,
-                        int w, const DevStatus *st, const int *tail_pos, int ntail,
+void block_update_resid(long n, long ld, const UpdGroup *groups, const double *C, int ldc,
+                        int crows, int w, const DevStatus *st, const int *tail_pos, int ntail,
                         const double *lam, const float *tdiag, float *r, double *rn2) {
     BX_REQUIRE(ld % 64 == 0, "block_update_resid: leading dimension %ld not a multiple of 64", ld);
+    BX_REQUIRE(n >= 1 && n <= ld, "block_update_resid: %ld rows with leading dimension %ld", n, ld);
     BX_REQUIRE(w >= 1 && w <= 32, "block_update_resid: %d output columns (max 32)", w);
     UpdRArgs a;
     for (int i = 0; i < 2; i++) {
@@ -325,6 +337,7 @@ void block_update_resid(long ld, const UpdGroup *groups, const double *C, int ld
     }
     a.lam = lam;
     a.tdiag = tdiag;
+    a.n = n;
     a.r = r;
     const size_t lds = (size_t)crows * 32 * sizeof(float);
     BX_REQUIRE(crows <= kMaxRR, "block_update_resid: %d coefficient rows", crows);

@@ -549,6 +549,30 @@ def test_mixed_precision_block_sizes(k):
     assert relerr(r.lam, o.lam) < 1e-8
 
 
+@pytest.mark.parametrize("shape", [(13, 11, 9), (41, 41, 41)])
+def test_mixed_precision_padded_rows(shape):
+    """MIXED with Jacobi where the solver's blocks carry padding rows: n = 1287
+    (leading dimension 1344) and n = 68921 (blocks of 2^16 rows or more are
+    padded to a multiple of 256 plus 256 rows: 69376).  The fused FP32 update
+    reads the preconditioner's n entries only; the padding rows stay zero, so
+    the eigenpairs are those of the unpadded problem."""
+    say(f"mixed precision with padding rows, {shape}, k=6, jacobi")
+    A = pb.laplacian_7pt(*shape, sigma=1.0, seed=11)
+    X0 = po.drand48_block(A.n, 6, 1)
+    r = bl.lobpcg(X0, A, T="jacobi", maxit=800, mode="mixed")
+    assert r.flag == 0
+    assert r.stats["iterations_fp32"] > 0
+    assert np.all(r.res <= 1e-6 + 1e-15)
+    S = A.to_scipy()
+    if A.n <= 2000:
+        ref = np.linalg.eigvalsh(S.toarray())[:6]
+    else:
+        import scipy.sparse.linalg as sla
+
+        ref = np.sort(sla.eigsh(S, k=6, sigma=0.0, which="LM", return_eigenvectors=False))
+    assert relerr(np.sort(r.lam), ref) < 1e-8
+
+
 @pytest.mark.parametrize("B", [None, "diag"])
 def test_mixed_precision_without_preconditioner(B):
     """MIXED mode without T: with B = I the FP32 update forms the residual
