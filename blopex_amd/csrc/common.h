@@ -53,6 +53,11 @@ void *pinned(int slot, size_t bytes);
 void upload_async(void *dst, const void *src, size_t bytes);
 // Device->host copy on the stream, then waits for it.
 void download_sync(void *dst, const void *src, size_t bytes);
+// The same copy in two halves: download_start enqueues it, download_finish
+// waits for the copy alone (work enqueued in between keeps running) and
+// copies the bytes out.  One such copy at a time; no download_sync between.
+void download_start(const void *src, size_t bytes);
+void download_finish(void *dst, size_t bytes);
 
 inline size_t elem_size(int precision) {
     return precision == HIPMV_FP32 ? 4 : precision == HIPMV_C64 ? 16 : 8;

@@ -80,6 +80,7 @@ const double kExplicitGramRes = sqrt(DBL_EPSILON);
 class KTimer {
   public:
     explicit KTimer(bool on) : on_(on) {}
+    bool on() const { return on_; }
     ~KTimer() {
         for (hipEvent_t e : pool_) hipEventDestroy(e);
     }
@@ -273,6 +274,19 @@ class Phase {
         download_sync(buf.data(), tail_, tail_bytes_);
         memcpy(hs, (const char *)buf.data() + 4 * (size_t)k_ * sizeof(double), sizeof(DevStatus));
         tm_->resolve();
+    }
+    // read_tail with the products A R (and B R) of the w columns of R
+    // enqueued behind the copy, so that the device computes them while the
+    // host reads the tail and decides the next iteration (which uses them if
+    // its active set is unchanged).  Not with kernel timing, whose events
+    // are resolved after the copy
+    void read_tail_ahead(std::vector<double> &buf, DevStatus *hs, int w) {
+        buf.resize(tail_bytes_ / sizeof(double) + 1);
+        download_start(tail_, tail_bytes_);
+        if (!noB_) apply(opB_, Bd_, R_, BR_, w);
+        apply(opA_, Ad_, R_, AR_, w);
+        download_finish(buf.data(), tail_bytes_);
+        memcpy(hs, (const char *)buf.data() + 4 * (size_t)k_ * sizeof(double), sizeof(DevStatus));
     }
     void read_status(DevStatus *h) {
         download_sync(h, st_, sizeof(DevStatus));
@@ -509,6 +523,10 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
     // condition estimates of the pencils (lobpcg.m:488, 518, 912); the first
     // entry is lobpcg.m's start value, large enough to avoid early restarts
     std::vector<double> condh(1, -log10(DBL_EPSILON) / 2.0);
+    // A R and B R of the columns of R as the last iteration left them are
+    // already enqueued (read_tail_ahead); valid while R is not rewritten
+    bool ahead = false;
+    const bool may_ahead = !tm_->on() && (noT_ || fuseT_) && ny_ == 0;
     for (; flag == 0 && it <= maxit; it++) {
         act.clear();
         // wanted columns: residual above the threshold (lobpcg_checkResiduals)
@@ -575,6 +593,7 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
                 tm_->end();
                 prev_act = act;
                 local_it = 0;
+                ahead = false;
             }
             refine_left--;
             refine_done_++;
@@ -585,6 +604,7 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
         // for the columns active when it ran); compact them when columns
         // have converged since (the active set only shrinks)
         if (act != prev_act) {
+            ahead = false;
             tm_->begin(kOther);
             std::vector<int> src;
             size_t q = 0;
@@ -616,8 +636,11 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
         }
         if (ny_ > 0) project_constraints(R_, nr);
         const int r0 = k, p0 = k + nr;
-        if (!noB_) apply(opB_, Bd_, R_, BR_, nr);
-        apply(opA_, Ad_, R_, AR_, nr);
+        if (!ahead) {
+            if (!noB_) apply(opB_, Bd_, R_, BR_, nr);
+            apply(opA_, Ad_, R_, AR_, nr);
+        }
+        ahead = false;
         const int preq = local_it > 0 ? nr : 0;
         // explicit X'AX, X'BX: always for FP32 iterates; in FP64 once an
         // active column's residual is below kExplicitGramRes, where the
@@ -728,7 +751,12 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
             tm_->end();
         }
         BXp = noB_ ? X_ : BX_;
-        read_tail(tail, &hs);
+        if (may_ahead && it < maxit) {
+            read_tail_ahead(tail, &hs, nr);
+            ahead = true;
+        } else {
+            read_tail(tail, &hs);
+        }
         if (hs.abort && fp32_phase) {
             // an FP32 breakdown ends the FP32 phase; X is still the last update
             *switched = true;

@@ -140,6 +140,23 @@ void download_sync(void *dst, const void *src, size_t bytes) {
     memcpy(dst, h, bytes);
 }
 
+namespace {
+hipEvent_t g_download_ev = nullptr;
+}  // namespace
+
+void download_start(const void *src, size_t bytes) {
+    hipStream_t s = stream();
+    void *h = pinned(kSlots - 1 - kStage, bytes ? bytes : 1);
+    if (!g_download_ev) BX_HIP(hipEventCreateWithFlags(&g_download_ev, hipEventDisableTiming));
+    if (bytes) BX_HIP(hipMemcpyAsync(h, src, bytes, hipMemcpyDeviceToHost, s));
+    BX_HIP(hipEventRecord(g_download_ev, s));
+}
+
+void download_finish(void *dst, size_t bytes) {
+    BX_HIP(hipEventSynchronize(g_download_ev));
+    if (bytes) memcpy(dst, pinned(kSlots - 1 - kStage, bytes), bytes);
+}
+
 ColMap colmap_of(const hip_Multi_Vector *x) {
     ColMap m;
     BX_REQUIRE(x->num_active_vectors <= kMaxCols,
