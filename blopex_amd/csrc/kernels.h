@@ -73,13 +73,14 @@ struct UpdGroup {
 // as it is written).
 void block_update(int prec, long ld, const UpdGroup *groups, int ngroups, const double *C, int ldc,
                   int crows, int w, const DevStatus *st, const int *tail_pos = nullptr);
-// FP32, B = I, at most 32 output columns: the X group (groups[0]) and AX group
-// (groups[1]) of block_update, then R = T (X' diag(lam) - AX') into the ntail
-// tail columns of r (tail_pos as above) and their squared norms (before T)
-// into rn2[0..ntail) -- the update and the residual pass in one.
-void block_update_resid(long n, long ld, const UpdGroup *groups, const double *C, int ldc,
-                        int crows, int w, const DevStatus *st, const int *tail_pos, int ntail,
-                        const double *lam, const float *tdiag, float *r, double *rn2);
+// FP32 or FP64 blocks, B = I, at most 32 output columns: the X group
+// (groups[0]) and AX group (groups[1]) of block_update, then
+// R = T (X' diag(lam) - AX') into the ntail tail columns of r (tail_pos as
+// above; tdiag and r in the blocks' precision) and their squared norms
+// (before T) into rn2[0..ntail) -- the update and the residual pass in one.
+void block_update_resid(int prec, long n, long ld, const UpdGroup *groups, const double *C,
+                        int ldc, int crows, int w, const DevStatus *st, const int *tail_pos,
+                        int ntail, const double *lam, const void *tdiag, void *r, double *rn2);
 // out[j] = sum of part[j * nparts + 0 .. nparts) in a fixed order, j < ncols
 void row_sums(const double *part, int nparts, int ncols, double *out);
 

@@ -733,13 +733,17 @@ int Phase::run(void *Xuser, int *it_io, int maxit, bool fp32_phase, const hip_Lo
         }
         std::vector<int> tail_pos(k, -1);  // P' kept for the active columns only
         for (int t = 0; t < nr; t++) tail_pos[act[t]] = t;
-        if (noB_ && prec_ == HIPMV_FP32 && k <= 32) {
-            // FP32, B = I: the residual of the new block is formed as it is
-            // updated (R = T (X' lam - AX') and its norms, for the active columns)
-            const float *t32 = fuseT_ ? ((const hip_DiagMatrix *)Td_)->d32 : nullptr;
+        if (noB_ && k <= 32) {
+            // B = I: the residual of the new block is formed as it is updated
+            // (R = T (X' lam - AX') and its norms, for the active columns)
+            const void *t = nullptr;
+            if (fuseT_) {
+                const hip_DiagMatrix *T = (const hip_DiagMatrix *)Td_;
+                t = prec_ == HIPMV_FP32 ? (const void *)T->d32 : (const void *)T->d;
+            }
             tm_->begin(kUpdate);
-            block_update_resid(n_, ld_, g, coef_, ld3_, k + nr + preq, k, st_, tail_pos.data(), nr,
-                               lam_new_, t32, (float *)R_, rn2_);
+            block_update_resid(prec_, n_, ld_, g, coef_, ld3_, k + nr + preq, k, st_,
+                               tail_pos.data(), nr, lam_new_, t, R_, rn2_);
             tm_->end();
         } else {
             update(g, ng, coef_, k + nr + preq, k, tail_pos.data());

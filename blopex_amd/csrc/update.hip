@@ -158,53 +158,59 @@ void launch_update(const UpdKArgs &a, int ngroups) {
     BX_HIP(hipGetLastError());
 }
 
-// The FP32 block update of the X and AX groups (B = I) with the residual of
-// the new block folded in: one wave computes AX' and then X' for its 64 rows
+// The block update of the X and AX groups (B = I) with the residual of the
+// new block folded in: one wave computes AX' and then X' for its 64 rows
 // (both groups' accumulators live at once), stores them and P', AP' as
 // k_update does, and forms R = T (X' diag(lam) - AX') for the columns that
 // keep a tail position, with the squared norms of X' lam - AX' summed per
 // workgroup (FP64) into part[tail column][workgroup].  Saves the residual
-// pass's reads of X' and AX'.
+// pass's reads of X' and AX'.  T is the block's precision (FP32 or FP64).
 struct UpdRArgs {
     UpdKArgs u;         // g[0] = X group, g[1] = AX group
     const double *lam;  // new Ritz values, by output column
-    const float *tdiag; // diagonal preconditioner (n entries) or null
+    const void *tdiag;  // diagonal preconditioner (n entries, precision T) or null
     long n;             // rows of the problem (the rows from n to u.ld are padding)
-    float *r;           // residual block (tail columns), leading dimension u.ld
+    void *r;            // residual block (tail columns, precision T), leading dimension u.ld
     double *part;       // [tail column][nparts]
     int nparts;         // workgroups in the grid
 };
 
-__global__ void __launch_bounds__(kUT) k_update_resid(UpdRArgs a) {
+// MINW: waves per SIMD the register allocation must allow (FP64 takes 2,
+// else its registers allow one; FP32 is faster at its own choice of 2 than
+// held to the 3 that a bound of 2 gives it)
+template <typename T, int MINW>
+__global__ void __launch_bounds__(kUT, MINW) k_update_resid(UpdRArgs a) {
     constexpr int NOT = 2;
     extern __shared__ unsigned char smem_raw[];
-    float *Cl = reinterpret_cast<float *>(smem_raw);
+    T *Cl = reinterpret_cast<T *>(smem_raw);
     constexpr int WT = NOT * 16;
     const UpdKArgs &u = a.u;
     if (u.st && u.st->abort) return;
     for (int e = threadIdx.x; e < u.crows * WT; e += kUT) {
         const int r = e / WT, c = e % WT;
-        Cl[e] = (c < u.w) ? (float)u.C[r + (size_t)c * u.ldc] : 0.f;
+        Cl[e] = (c < u.w) ? (T)u.C[r + (size_t)c * u.ldc] : T(0);
     }
     __syncthreads();
     // tail positions and Ritz values of the output columns, read where used
     __shared__ int tposl[WT];
-    __shared__ float laml[WT];
+    __shared__ T laml[WT];
     for (int c = threadIdx.x; c < WT; c += kUT) {
         tposl[c] = c < u.w ? (int)u.tail_pos[c] : -1;
-        laml[c] = c < u.w ? (float)a.lam[c] : 0.f;
+        laml[c] = c < u.w ? (T)a.lam[c] : T(0);
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const long nchunks = u.ld / 64;
+    const T *tdiag = (const T *)a.tdiag;
+    T *rout = (T *)a.r;
     int pfull[NOT][4], ptail[NOT][4];
     double ss[NOT][4];
 #pragma unroll
     for (int o = 0; o < NOT; o++)
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const int c = o * 16 + Mfma<float>::drow(lk, r);
+            const int c = o * 16 + Mfma<T>::drow(lk, r);
             pfull[o][r] = c < u.w ? c : -1;
             ptail[o][r] = tposl[c];
             ss[o][r] = 0.0;
@@ -212,33 +218,33 @@ __global__ void __launch_bounds__(kUT) k_update_resid(UpdRArgs a) {
     const UpdGroup &gx = u.g[0], &ga = u.g[1];
     for (long ch = (long)blockIdx.x * 4 + wave; ch < nchunks; ch += (long)gridDim.x * 4) {
         const long r0 = ch * 64 + 4 * li;
-        typename Mfma<float>::acc ax[NOT][4], xx[NOT][4];
+        typename Mfma<T>::acc ax[NOT][4], xx[NOT][4];
 #pragma unroll
         for (int o = 0; o < NOT; o++)
 #pragma unroll
-            for (int t = 0; t < 4; t++) ax[o][t] = xx[o][t] = typename Mfma<float>::acc{};
+            for (int t = 0; t < 4; t++) ax[o][t] = xx[o][t] = typename Mfma<T>::acc{};
         // AX' (with AP' on the way), then X' (with P')
-        if (ga.in[1]) upd_block<float, NOT>((const float *)ga.in[1], ga.h[1], ga.crow[1], Cl, u.ld, r0, li, lk, ax);
-        if (ga.in[2]) upd_block<float, NOT>((const float *)ga.in[2], ga.h[2], ga.crow[2], Cl, u.ld, r0, li, lk, ax);
-        if (ga.out_tail) upd_store<float, NOT>((float *)ga.out_tail, u.ld, r0, ptail, ax);
-        upd_block<float, NOT>((const float *)ga.in[0], ga.h[0], ga.crow[0], Cl, u.ld, r0, li, lk, ax);
-        upd_store<float, NOT>((float *)ga.out_full, u.ld, r0, pfull, ax);
-        if (gx.in[1]) upd_block<float, NOT>((const float *)gx.in[1], gx.h[1], gx.crow[1], Cl, u.ld, r0, li, lk, xx);
-        if (gx.in[2]) upd_block<float, NOT>((const float *)gx.in[2], gx.h[2], gx.crow[2], Cl, u.ld, r0, li, lk, xx);
-        if (gx.out_tail) upd_store<float, NOT>((float *)gx.out_tail, u.ld, r0, ptail, xx);
-        upd_block<float, NOT>((const float *)gx.in[0], gx.h[0], gx.crow[0], Cl, u.ld, r0, li, lk, xx);
-        upd_store<float, NOT>((float *)gx.out_full, u.ld, r0, pfull, xx);
+        if (ga.in[1]) upd_block<T, NOT>((const T *)ga.in[1], ga.h[1], ga.crow[1], Cl, u.ld, r0, li, lk, ax);
+        if (ga.in[2]) upd_block<T, NOT>((const T *)ga.in[2], ga.h[2], ga.crow[2], Cl, u.ld, r0, li, lk, ax);
+        if (ga.out_tail) upd_store<T, NOT>((T *)ga.out_tail, u.ld, r0, ptail, ax);
+        upd_block<T, NOT>((const T *)ga.in[0], ga.h[0], ga.crow[0], Cl, u.ld, r0, li, lk, ax);
+        upd_store<T, NOT>((T *)ga.out_full, u.ld, r0, pfull, ax);
+        if (gx.in[1]) upd_block<T, NOT>((const T *)gx.in[1], gx.h[1], gx.crow[1], Cl, u.ld, r0, li, lk, xx);
+        if (gx.in[2]) upd_block<T, NOT>((const T *)gx.in[2], gx.h[2], gx.crow[2], Cl, u.ld, r0, li, lk, xx);
+        if (gx.out_tail) upd_store<T, NOT>((T *)gx.out_tail, u.ld, r0, ptail, xx);
+        upd_block<T, NOT>((const T *)gx.in[0], gx.h[0], gx.crow[0], Cl, u.ld, r0, li, lk, xx);
+        upd_store<T, NOT>((T *)gx.out_full, u.ld, r0, pfull, xx);
         // R = T (X' lam - AX') for the tail columns, as the residual kernel forms it
         // (the diagonal has n entries: padding rows, whose X' and AX' are
         // zero, keep the factor 1)
-        float td[4] = {1.f, 1.f, 1.f, 1.f};
-        if (a.tdiag) {
+        T td[4] = {T(1), T(1), T(1), T(1)};
+        if (tdiag) {
             if (r0 + 3 < a.n) {
-                load4(a.tdiag + r0, td);
+                load4(tdiag + r0, td);
             } else {
 #pragma unroll
                 for (int t = 0; t < 4; t++)
-                    if (r0 + t < a.n) td[t] = a.tdiag[r0 + t];
+                    if (r0 + t < a.n) td[t] = tdiag[r0 + t];
             }
         }
 #pragma unroll
@@ -247,17 +253,17 @@ __global__ void __launch_bounds__(kUT) k_update_resid(UpdRArgs a) {
             for (int r = 0; r < 4; r++) {
                 const int q = ptail[o][r];
                 if (q < 0) continue;
-                const float lc = laml[o * 16 + Mfma<float>::drow(lk, r)];
-                float v[4];
+                const T lc = laml[o * 16 + Mfma<T>::drow(lk, r)];
+                T v[4];
 #pragma unroll
                 for (int t = 0; t < 4; t++) {
                     v[t] = lc * xx[o][t][r] - ax[o][t][r];
                     ss[o][r] = fma((double)v[t], (double)v[t], ss[o][r]);
                 }
-                if (a.tdiag)
-                    store4(a.r + (size_t)q * u.ld + r0, td[0] * v[0], td[1] * v[1], td[2] * v[2], td[3] * v[3]);
+                if (tdiag)
+                    store4(rout + (size_t)q * u.ld + r0, td[0] * v[0], td[1] * v[1], td[2] * v[2], td[3] * v[3]);
                 else
-                    store4(a.r + (size_t)q * u.ld + r0, v[0], v[1], v[2], v[3]);
+                    store4(rout + (size_t)q * u.ld + r0, v[0], v[1], v[2], v[3]);
             }
     }
     // norms: sum over the 16 lanes that share the output columns, then over
@@ -268,7 +274,7 @@ __global__ void __launch_bounds__(kUT) k_update_resid(UpdRArgs a) {
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const double t = group_sum16(ss[o][r]);
-            if (li == 0) wsum[wave][o * 16 + Mfma<float>::drow(lk, r)] = t;
+            if (li == 0) wsum[wave][o * 16 + Mfma<T>::drow(lk, r)] = t;
         }
     __syncthreads();
     for (int c = threadIdx.x; c < WT; c += kUT) {
@@ -276,6 +282,22 @@ __global__ void __launch_bounds__(kUT) k_update_resid(UpdRArgs a) {
         if (q >= 0)
             a.part[(size_t)q * a.nparts + blockIdx.x] = (wsum[0][c] + wsum[1][c]) + (wsum[2][c] + wsum[3][c]);
     }
+}
+
+// resident workgroups of k_update_resid<T> with coefficient rows for the
+// largest pencil (LDS never limits it); sets the dynamic LDS limit once
+template <typename T, int MINW>
+int update_resid_occupancy() {
+    static int per_cu = 0;
+    if (per_cu == 0) {
+        const size_t lds = (size_t)kMaxRR * 32 * sizeof(T);
+        BX_HIP(hipFuncSetAttribute((const void *)k_update_resid<T, MINW>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        BX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_update_resid<T, MINW>,
+                                                            kUT, lds));
+        if (per_cu < 1) per_cu = 1;
+    }
+    return per_cu;
 }
 
 // D = A B with A[i][0] = i (other k-slots 0), B[0][j] = 1: D[i][j] = i;
@@ -307,9 +329,10 @@ void mfma_layout_probe(int prec, int *rows, int *cols) {
     BX_HIP(hipFree(d));
 }
 
-void block_update_resid(long n, long ld, const UpdGroup *groups, const double *C, int ldc,
-                        int crows, int w, const DevStatus *st, const int *tail_pos, int ntail,
-                        const double *lam, const float *tdiag, float *r, double *rn2) {
+void block_update_resid(int prec, long n, long ld, const UpdGroup *groups, const double *C,
+                        int ldc, int crows, int w, const DevStatus *st, const int *tail_pos,
+                        int ntail, const double *lam, const void *tdiag, void *r, double *rn2) {
+    BX_REQUIRE(prec == HIPMV_FP32 || prec == HIPMV_FP64, "block_update_resid: precision %d", prec);
     BX_REQUIRE(ld % 64 == 0, "block_update_resid: leading dimension %ld not a multiple of 64", ld);
     BX_REQUIRE(n >= 1 && n <= ld, "block_update_resid: %ld rows with leading dimension %ld", n, ld);
     BX_REQUIRE(w >= 1 && w <= 32, "block_update_resid: %d output columns (max 32)", w);
@@ -339,23 +362,10 @@ void block_update_resid(long n, long ld, const UpdGroup *groups, const double *C
     a.tdiag = tdiag;
     a.n = n;
     a.r = r;
-    const size_t lds = (size_t)crows * 32 * sizeof(float);
     BX_REQUIRE(crows <= kMaxRR, "block_update_resid: %d coefficient rows", crows);
-    static bool attr = false;
-    static int per_cu = 0;
-    if (!attr) {
-        // the coefficients (at most kMaxRR rows of 32 floats); static LDS comes on top
-        BX_HIP(hipFuncSetAttribute((const void *)k_update_resid,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)((size_t)kMaxRR * 32 * sizeof(float))));
-        attr = true;
-    }
-    if (per_cu == 0) {
-        // the resident workgroups for the largest pencil (LDS never limits it)
-        BX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_update_resid,
-                                                            kUT, (size_t)kMaxRR * 32 * sizeof(float)));
-        if (per_cu < 1) per_cu = 1;
-    }
+    const bool f32 = prec == HIPMV_FP32;
+    const size_t lds = (size_t)crows * 32 * (f32 ? sizeof(float) : sizeof(double));
+    const int per_cu = f32 ? update_resid_occupancy<float, 1>() : update_resid_occupancy<double, 2>();
     const long nchunks = ld / 64;
     long blocks = (nchunks + 3) / 4;
     const long cap = (long)num_cus() * per_cu;
@@ -363,7 +373,10 @@ void block_update_resid(long n, long ld, const UpdGroup *groups, const double *C
     if (blocks < 1) blocks = 1;
     a.nparts = (int)blocks;
     a.part = (double *)scratch(1, (size_t)a.nparts * ntail * sizeof(double));
-    hipLaunchKernelGGL(k_update_resid, dim3((unsigned)blocks), dim3(kUT), lds, stream(), a);
+    if (f32)
+        hipLaunchKernelGGL((k_update_resid<float, 1>), dim3((unsigned)blocks), dim3(kUT), lds, stream(), a);
+    else
+        hipLaunchKernelGGL((k_update_resid<double, 2>), dim3((unsigned)blocks), dim3(kUT), lds, stream(), a);
     BX_HIP(hipGetLastError());
     row_sums(a.part, a.nparts, ntail, rn2);
 }
