@@ -47,6 +47,9 @@ sys.path.insert(0, ROOT)
 
 COUNTERS = os.path.join(ROOT, "profiles", "spmm_roofline_128cube_k32_counters.txt")
 TRACE = os.path.join(ROOT, "profiles", "spmm_roofline_128cube_k32_trace.txt")
+# the general CSR SpMM kernel as rocprofv3 names it (FP32, up to 8 entries
+# a row in registers, two block columns per step)
+CSR_KERNEL = "k_spmm<float, 8, 2>"
 EIG_RTOL = 1e-9  # eigenvalue error bound of the headline solve, relative
 MFMA_SUMMARY = os.path.join(ROOT, "profiles", "lobpcg_mixed_128cube_k32_mfma.txt")
 GRAM_KERNEL = "k_gram<float, 6, 3, 64>"  # the Gram plan of the FP32 iterations (up to 192 columns)
@@ -484,7 +487,7 @@ def main():
     nd = dA.ptr.contents.ndiag
     path = lib.hip_CSRMatrixProductKernel(dA.ptr, L.HIPMV_FP32, k, A.n)
     ksym = {2: f"k_zm<float, 32, {dA.ptr.contents.grid[0]}>", 1: f"k_dia<float, {nd}>"}.get(
-        path, "k_spmm<float, 8>")
+        path, CSR_KERNEL)
     kname = ksym + {2: f" (z-marching 7-point stencil SpMM on the diagonal copy, k={k}, FP32)",
                     1: f" (SpMM on the diagonal copy of the CSR matrix, k={k}, FP32)"}.get(
         path, f" (CSR SpMM, k={k}, FP32)")
@@ -517,18 +520,18 @@ def main():
         out["roofline"]["trace_source"] = os.path.relpath(TRACE, ROOT)
     out["spmm_fp64"] = {"achieved": algo64 / (ms64 * 1e-3) / 1e9, "avg_launch_us": ms64 * 1e3,
                         "algorithmic_bytes": algo64}
-    # the general CSR kernel (k_spmm<float, 8>: matrix slices staged through
+    # the general CSR kernel (CSR_KERNEL: matrix slices staged through
     # LDS, thread per row) on the same matrix, for matrices without a
     # diagonal copy
     msc, algoc = spmm_roofline(lib, dA, A, k, L.HIPMV_FP32, force_csr=True)
-    out["csr_general"] = {"kernel": "k_spmm<float, 8>", "avg_launch_us": msc * 1e3,
+    out["csr_general"] = {"kernel": CSR_KERNEL, "avg_launch_us": msc * 1e3,
                           "algorithmic_bytes": algoc,
                           "achieved": algoc / (msc * 1e-3) / 1e9,
                           "frac": algoc / (msc * 1e-3) / 1e9 / HBM_PEAK_GBS}
-    trc = read_trace_avg_us(TRACE, "k_spmm<float, 8>") if (nside, k) == (128, 32) else None
+    trc = read_trace_avg_us(TRACE, CSR_KERNEL) if (nside, k) == (128, 32) else None
     if trc:
         out["csr_general"]["trace_avg_us"] = trc
-    tfc = read_traffic(COUNTERS, "k_spmm<float, 8>") if (nside, k) == (128, 32) else None
+    tfc = read_traffic(COUNTERS, CSR_KERNEL) if (nside, k) == (128, 32) else None
     if tfc:
         out["csr_general"]["traffic"] = tfc
     # MFMA rate of the FP32 Gram kernel in the mixed solve, from the committed
