@@ -48,8 +48,8 @@ sys.path.insert(0, ROOT)
 COUNTERS = os.path.join(ROOT, "profiles", "spmm_roofline_128cube_k32_counters.txt")
 TRACE = os.path.join(ROOT, "profiles", "spmm_roofline_128cube_k32_trace.txt")
 # the general CSR SpMM kernel as rocprofv3 names it (FP32, up to 8 entries
-# a row in registers, two block columns per step)
-CSR_KERNEL = "k_spmm<float, 8, 2>"
+# a row in registers)
+CSR_KERNEL = "k_spmm<float, 8>"
 EIG_RTOL = 1e-9  # eigenvalue error bound of the headline solve, relative
 MFMA_SUMMARY = os.path.join(ROOT, "profiles", "lobpcg_mixed_128cube_k32_mfma.txt")
 GRAM_KERNEL = "k_gram<float, 6, 3, 64>"  # the Gram plan of the FP32 iterations (up to 192 columns)

@@ -53,7 +53,7 @@ __device__ __forceinline__ long xcd_block(unsigned b, unsigned nb) {
 // reuses them for every block column.  Y streams out non-temporally.
 constexpr int kSpSeg = kSpThreads * 10;  // staged pairs per row group (rows of <= 10 on average)
 
-template <typename T, int MAXE, int JU>
+template <typename T, int MAXE>
 __global__ void __launch_bounds__(kSpThreads) k_spmm(long n, const int *__restrict__ rowptr,
                                                      const int *__restrict__ colind,
                                                      const T *__restrict__ val, const T *x, long ldx,
@@ -88,28 +88,9 @@ __global__ void __launch_bounds__(kSpThreads) k_spmm(long n, const int *__restri
             ci[e] = ok ? col(e0 + e) : (int)r;
             v[e] = ok ? value(e0 + e) : T(0);
         }
-        // JU block columns at a time, their gathers in flight together (two:
-        // 10% faster than one on the 128^3 stencil in CSR form, FP32 and
-        // FP64 alike; four: no better); the gathers of a wave touch one
-        // column's lines per load, which the neighbouring rows reuse from L1
-        int j = 0;
-        for (; j + JU <= xm.n; j += JU) {
-            T xv[JU][MAXE];
-#pragma unroll
-            for (int u = 0; u < JU; u++) {
-                const T *xa = x + (size_t)xm.idx[j + u] * ldx;
-#pragma unroll
-                for (int e = 0; e < MAXE; e++) xv[u][e] = xa[ci[e]];
-            }
-#pragma unroll
-            for (int u = 0; u < JU; u++) {
-                T sa = T(0);
-#pragma unroll
-                for (int e = 0; e < MAXE; e++) sa = fma(v[e], xv[u][e], sa);
-                __builtin_nontemporal_store(sa, y + (size_t)ym.idx[j + u] * ldy + r);
-            }
-        }
-        for (; j < xm.n; j++) {
+        // one block column at a time: the gathers of a wave then touch one
+        // column's lines, which the neighbouring rows reuse from L1
+        for (int j = 0; j < xm.n; j++) {
             const T *xa = x + (size_t)xm.idx[j] * ldx;
             T sa = T(0);
 #pragma unroll
@@ -429,10 +410,10 @@ void spmm_csr(int prec, const hip_CSRMatrix *A, const void *x, long ldx, const C
     }
     dim3 g(xcd_grid(A->n));
     if (prec == HIPMV_FP32)
-        hipLaunchKernelGGL((k_spmm<float, 8, 2>), g, dim3(kSpThreads), 0, stream(), A->n, A->rowptr,
+        hipLaunchKernelGGL((k_spmm<float, 8>), g, dim3(kSpThreads), 0, stream(), A->n, A->rowptr,
                            A->colind, A->val32, (const float *)x, ldx, xm, (float *)y, ldy, ym);
     else
-        hipLaunchKernelGGL((k_spmm<double, 8, 2>), g, dim3(kSpThreads), 0, stream(), A->n, A->rowptr,
+        hipLaunchKernelGGL((k_spmm<double, 8>), g, dim3(kSpThreads), 0, stream(), A->n, A->rowptr,
                            A->colind, A->val, (const double *)x, ldx, xm, (double *)y, ldy, ym);
     BX_HIP(hipGetLastError());
 }
