@@ -12,6 +12,10 @@ of the 128^3 workload itself.
   block is nearly dependent on X is solved again without P; in a solve to a
   tight tolerance the residuals recomputed on the host meet it.
 * The size guard of lobpcg.m:480 (settings.size_guard).
+* The look-ahead product: the next iteration's A R is enqueued behind the
+  read of the residual norms and used only if the active set is unchanged.
+  With the class timers on (settings.profile), which turn it off, a solve in
+  which columns leave the active set one by one gives the same bits.
 * The 128^3 workload's generator at 20^3 (k = 32, Jacobi) against the
   reference's fixture, FP64 and MIXED; and the 128^3 workload itself with
   the bench's settings: residuals recomputed on the host from the returned
@@ -99,6 +103,28 @@ def test_eigenvalue_error_bound_holds(mode):
     assert r.iterations > loose.iterations
     assert 0.0 < bound <= 1e-12
     assert err <= bound * 1.01 + 1e-15
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_lookahead_product_leaves_the_results_unchanged(mode):
+    say(f"look-ahead product, 16^3 lognormal, k=12, {mode}")
+    A = pb.laplacian_7pt(16, 16, 16, sigma=1.0, seed=3)
+    X0 = po.drand48_block(A.n, 12, 1)
+    a = bl.lobpcg(X0, A, T="jacobi", tol=1e-7, maxit=800, mode=mode)
+    b = bl.lobpcg(X0, A, T="jacobi", tol=1e-7, maxit=800, mode=mode, profile=True)
+    # the iteration at which each column first met the tolerance: the
+    # active set shrank at several iterations, each one discarding the
+    # product enqueued ahead
+    first = [int(np.argmax(a.res_hist[i, 1:] < 1e-7)) for i in range(12)]
+    say(f"  flag {a.flag}/{b.flag} it {a.iterations}/{b.iterations} "
+        f"active-set changes at {sorted(set(first))}")
+    assert a.flag == 0 and b.flag == 0
+    assert len(set(first)) >= 3
+    assert a.iterations == b.iterations
+    np.testing.assert_array_equal(a.lam, b.lam)
+    np.testing.assert_array_equal(a.res, b.res)
+    np.testing.assert_array_equal(a.X, b.X)
+    np.testing.assert_array_equal(a.res_hist, b.res_hist)
 
 
 def test_condition_restart_on_the_rayleigh_ritz_kernel():
