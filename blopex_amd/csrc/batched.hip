@@ -967,16 +967,24 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
     long *c_r0 = dev_alloc<long>(nch_max);
     int *d_actp = dev_alloc<int>(nprob), *d_pcb = dev_alloc<int>(nprob),
         *d_pcc = dev_alloc<int>(nprob), *d_counts = dev_alloc<int>(2);
-    double *lam = dev_alloc<double>((size_t)nprob * kK), *res2 = dev_alloc<double>((size_t)nprob * kK);
+    double *lam = dev_alloc<double>((size_t)nprob * kK);
+    // what the host reads after every iteration, in one allocation so that one
+    // copy brings it: the residual norms, the small pencils' status, the
+    // Rayleigh-Ritz status
+    const size_t rb_res = (size_t)nprob * kK * sizeof(double), rb_ds = (size_t)nprob * sizeof(DevStatus),
+                 rb_st = (size_t)nprob * sizeof(int);
+    char *rb = dev_alloc<char>(rb_res + rb_ds + rb_st);
+    double *res2 = (double *)rb;
+    DevStatus *dstat = (DevStatus *)(rb + rb_res);
+    int *status = (int *)(rb + rb_res + rb_ds);
     double *GA = dev_alloc<double>((size_t)nprob * kM * kM), *GB = dev_alloc<double>((size_t)nprob * kM * kM);
     double *Cm = dev_alloc<double>((size_t)nprob * kM * kK);
-    int *nbe = dev_alloc<int>(nprob), *status = dev_alloc<int>(nprob);
+    int *nbe = dev_alloc<int>(nprob);
     // the one-workgroup tridiagonal Rayleigh-Ritz solver of the single-problem
     // path, launched over the batch (explicit X blocks, P dropped when the
     // pencil is not positive definite, status per problem)
     const size_t rr_words = rr_fast_work_words(kM);
     double *rr_work = dev_alloc<double>((size_t)nprob * rr_words);
-    DevStatus *dstat = dev_alloc<DevStatus>(nprob);
     BX_HIP(hipMemsetAsync(dstat, 0, sizeof(DevStatus) * nprob, stream()));
     double *part_r = dev_alloc<double>((size_t)nch_max * kK);
     double *part_g = dev_alloc<double>((size_t)nch_max * 2 * kPairs);
@@ -1076,6 +1084,7 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
     std::vector<double> res_h((size_t)nprob * kK), lam_h((size_t)nprob * kK);
     std::vector<int> st_h(nprob);
     std::vector<DevStatus> ds_h(nprob);
+    std::vector<char> rb_h(rb_res + rb_ds + rb_st);
     int captures = 0, replays = 0, it = 0, it32 = 0;
     // One phase: from the Rayleigh-Ritz step in span(X) (AX current), iterate
     // until every active problem's wanted residuals are below thr, its pencil
@@ -1087,9 +1096,10 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
         hipGraphExec_t exec = nullptr;
         int pit = 0;
         for (;;) {
-            download_sync(res_h.data(), res2, res_h.size() * 8);
-            download_sync(st_h.data(), status, nprob * 4);
-            download_sync(ds_h.data(), dstat, sizeof(DevStatus) * nprob);
+            download_sync(rb_h.data(), rb, rb_h.size());
+            memcpy(res_h.data(), rb_h.data(), rb_res);
+            memcpy(ds_h.data(), rb_h.data() + rb_res, rb_ds);
+            memcpy(st_h.data(), rb_h.data() + rb_res + rb_ds, rb_st);
             bool changed = false;
             for (int p = 0; p < nprob; p++) {
                 if (!active[p]) continue;
@@ -1286,9 +1296,8 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
     for (void *b : {(void *)X, (void *)AX, (void *)R, (void *)AR, (void *)P, (void *)AP,
                     (void *)dv, (void *)d_off, (void *)d_sbase, (void *)d_n, (void *)d_k,
                     (void *)c_prob, (void *)c_nr, (void *)c_r0, (void *)d_actp, (void *)d_pcb,
-                    (void *)d_pcc, (void *)lam, (void *)res2, (void *)GA, (void *)GB, (void *)Cm,
-                    (void *)nbe, (void *)status, (void *)part_r, (void *)part_g,
-                    (void *)rr_work, (void *)dstat})
+                    (void *)d_pcc, (void *)lam, (void *)rb, (void *)GA, (void *)GB, (void *)Cm,
+                    (void *)nbe, (void *)part_r, (void *)part_g, (void *)rr_work})
         BX_HIP(hipFree(b));
     hip_CSRMatrixDestroy(A);
     BX_HIP(hipFree(d_counts));
