@@ -3,6 +3,7 @@
     python3 -m blopex_amd.profsum stats  <trace-dir> [--title T] [--command C]...
     python3 -m blopex_amd.profsum counters <fetch-dir> <write-dir> [--title T] [--command C]...
     python3 -m blopex_amd.profsum mfma <counter-dir> <trace-dir> [--title T] [--command C]...
+    python3 -m blopex_amd.profsum sq <counter-dir> [--min-calls N] [--title T] [--command C]...
 
 `stats` reads `*_kernel_stats.csv` (rocprofv3 --kernel-trace --stats) and
 prints the per-kernel table.  `counters` reads the `*_counter_collection.csv`
@@ -11,7 +12,14 @@ program must also run a device-to-device copy of 512 MiB
 (`bench.py --roofline-only` does): its counter readings give the factor that
 turns each counter's KiB into bytes moved (counter reads x factor =
 512 MiB).  For every kernel the summary prints the scaled bytes per launch and
-a line `traffic <kernel> <bytes>` that bench.py reads back.
+a line `traffic <kernel> <bytes>` that bench.py reads back.  `sq` reads one
+pass of SQ counters and prints, for every kernel dispatched at least N times,
+the per-dispatch average of each counter and, where the counters are there:
+the MFMA pipes' busy share (SQ_VALU_MFMA_BUSY_CYCLES over the SIMD cycles:
+SQ_BUSY_CYCLES is summed over the shader arrays, so the kernel's cycles are
+SQ_BUSY_CYCLES / arrays, times the SIMD count from the agent info), the share
+of wave cycles spent waiting (SQ_WAIT_ANY, SQ_WAIT_INST_LDS over
+SQ_WAVE_CYCLES), and bank-conflict cycles per LDS-active cycle.
 """
 from __future__ import annotations
 
@@ -136,6 +144,45 @@ def mfma(counter_dir: str, trace_dir: str) -> tuple[list[str], dict[str, tuple[f
     return out, res
 
 
+def sq(counter_dir: str, min_calls: int = 10) -> list[str]:
+    path = _one(os.path.join(counter_dir, "**", "*counter_collection.csv"))
+    agent = _one(os.path.join(counter_dir, "**", "*agent_info.csv"))
+    simds = arrays = 0
+    for r in csv.DictReader(open(agent)):
+        if int(r.get("Simd_Count", 0) or 0) > 0:
+            simds, arrays = int(r["Simd_Count"]), int(r["Array_Count"])
+    sums: dict[str, dict[str, float]] = collections.defaultdict(lambda: collections.defaultdict(float))
+    disp: dict[str, set] = collections.defaultdict(set)
+    names: list[str] = []
+    for r in csv.DictReader(open(path)):
+        k = short_name(r["Kernel_Name"])
+        sums[k][r["Counter_Name"]] += float(r["Counter_Value"])
+        disp[k].add(r["Dispatch_Id"])
+        if r["Counter_Name"] not in names:
+            names.append(r["Counter_Name"])
+    out = [f"agent: {simds} SIMDs, {arrays} shader arrays; counters: {' '.join(names)}", ""]
+    for k in sorted(sums, key=lambda x: -sums[x].get("SQ_BUSY_CYCLES", 0.0)):
+        n = len(disp[k])
+        if n < min_calls:
+            continue
+        a = sums[k]
+        out.append(f"{k}  ({n} dispatches; per dispatch)")
+        for c in names:
+            out.append(f"    {c:<28} {a[c] / n:>14.4g}")
+        if a.get("SQ_BUSY_CYCLES") and "SQ_VALU_MFMA_BUSY_CYCLES" in a and simds and arrays:
+            share = a["SQ_VALU_MFMA_BUSY_CYCLES"] / (simds * a["SQ_BUSY_CYCLES"] / arrays)
+            out.append(f"    MFMA busy share of SIMD cycles  {share:.3f}")
+        if a.get("SQ_WAVE_CYCLES"):
+            if "SQ_WAIT_ANY" in a:
+                out.append(f"    waiting share of wave cycles    {a['SQ_WAIT_ANY'] / a['SQ_WAVE_CYCLES']:.3f}")
+            if "SQ_WAIT_INST_LDS" in a:
+                out.append(f"    LDS-wait share of wave cycles   {a['SQ_WAIT_INST_LDS'] / a['SQ_WAVE_CYCLES']:.3f}")
+        if a.get("SQ_ACTIVE_INST_LDS") and "SQ_LDS_BANK_CONFLICT" in a:
+            out.append(f"    bank conflict per LDS-active    {a['SQ_LDS_BANK_CONFLICT'] / a['SQ_ACTIVE_INST_LDS']:.3f}")
+        out.append("")
+    return out
+
+
 def read_mfma(path: str, kernel: str) -> tuple[float, float] | None:
     """The `mfma <kernel> <TFLOP/s> <peak>` figures of a committed summary, or None."""
     if not os.path.exists(path):
@@ -174,8 +221,9 @@ def read_traffic(path: str, kernel: str) -> float | None:
 
 def main(argv=None) -> None:
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["stats", "counters", "mfma"])
+    ap.add_argument("what", choices=["stats", "counters", "mfma", "sq"])
     ap.add_argument("dirs", nargs="+")
+    ap.add_argument("--min-calls", type=int, default=10)
     ap.add_argument("--title", default="")
     ap.add_argument("--command", action="append", default=[])
     a = ap.parse_args(argv)
@@ -183,10 +231,13 @@ def main(argv=None) -> None:
     if a.command:
         head.append("commands (from the repository root, TMPDIR=/tmp):")
         head += [f"  {c}" for c in a.command]
-        head.append(f"summary: python3 -m blopex_amd.profsum {a.what} {' '.join(a.dirs)}")
+        extra = f" --min-calls {a.min_calls}" if a.what == "sq" and a.min_calls != 10 else ""
+        head.append(f"summary: python3 -m blopex_amd.profsum {a.what} {' '.join(a.dirs)}{extra}")
         head.append("")
     if a.what == "stats":
         body = stats(a.dirs[0])
+    elif a.what == "sq":
+        body = sq(a.dirs[0], a.min_calls)
     elif a.what == "mfma":
         if len(a.dirs) != 2:
             raise SystemExit("mfma needs <counter-dir> <trace-dir>")
