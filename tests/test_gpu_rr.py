@@ -169,6 +169,19 @@ def test_rr_explicit_x_blocks(jacobi):
     np.testing.assert_allclose(Xg.T @ B @ Xg, np.eye(k), atol=1e-11)
 
 
+def test_rr_explicit_x_not_positive_definite():
+    """explicit_x with a zero X column: X'BX fails at pivot 3, reported as
+    dsygv does for the pencil (n + 3); with retry_without_p, after P is
+    dropped (p_info = -1, n = k + a)."""
+    say("rr explicit_x with X'BX not positive definite")
+    A, B, X, R, P, lam = make_case(6, 6, 6, seed=5)
+    X[:, 2] = 0.0
+    _, _, flags, _ = run_kernel(A, B, X, R, P, lam, 0, explicit_x=1)
+    assert flags[0] == 1 and flags[1] == 0 and flags[3] == 18 + 3
+    _, _, flags, _ = run_kernel(A, B, X, R, P, lam, 0, explicit_x=1, retry=1)
+    assert flags[0] == 1 and flags[2] == -1 and flags[3] == 12 + 3
+
+
 @pytest.mark.parametrize("jacobi", [0, 1])
 def test_rr_drops_rank_deficient_p(jacobi):
     say(f"rr drop P jacobi={jacobi}")
