@@ -57,7 +57,9 @@ typedef struct hip_LobpcgSettings {
     int refine_iterations;    /* FP64 iterations with all columns active after convergence
                                  (default 0) */
     int profile;              /* 1: time kernel classes with HIP events (adds event records
-                                 to the stream; default 0) */
+                                 to the stream, and the next iteration's A R is no longer
+                                 enqueued ahead of the host's read of the residual norms;
+                                 the results are the same bits; default 0) */
     int refresh_interval;     /* MIXED: FP32 iterations between recomputations of AX (and BX)
                                  from X, which keep the FP32 residuals those of X; 0: never
                                  (default 20) */
