@@ -170,6 +170,8 @@ _SIGS = {
     "zutilities_FortranMatrixAbs": (c_double, [P(FortranMatrix), c_long, c_long]),
     "blopex_amd_kernel_update": (c_int, [c_int, c_long, c_int, c_int, c_int, c_void_p, c_void_p,
                                          c_void_p, c_void_p, c_void_p, c_void_p]),
+    "blopex_amd_kernel_update_resid": (c_int, [c_int, c_long, c_int, c_int, c_int] + [c_void_p] * 8
+                                       + [c_int] + [c_void_p] * 8),
     "blopex_amd_kernel_gram": (c_int, [c_int, c_long, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "blopex_amd_kernel_mfma_probe": (c_int, [c_int, c_void_p, c_void_p]),
     "blopex_amd_kernel_rr": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p,

@@ -73,6 +73,54 @@ int blopex_amd_kernel_update(int prec, long n, int k, int a, int p, const double
     return 0;
 }
 
+int blopex_amd_kernel_update_resid(int prec, long n, int k, int a, int p, const double *X,
+                                   const double *AX, const double *R, const double *AR,
+                                   const double *P, const double *AP, const double *C,
+                                   const int *tail_pos, int ntail, const double *lam,
+                                   const double *tdiag, double *Xo, double *AXo, double *Po,
+                                   double *APo, double *Ro, double *rn2) {
+    BX_REQUIRE(ntail >= 1 && ntail <= k, "kernel_update_resid: %d tail columns for k = %d", ntail,
+               k);
+    DevBlock dX(n, k, prec, X), dAX(n, k, prec, AX), dR(n, a, prec, R), dAR(n, a, prec, AR),
+        dP(n, p, prec, P), dAP(n, p, prec, AP), dPo(n, ntail, prec, nullptr),
+        dAPo(n, ntail, prec, nullptr), dRo(n, ntail, prec, nullptr),
+        dT(n, tdiag ? 1 : 0, prec, tdiag);
+    const int crows = k + a + p;
+    double *dC = (double *)scratch(8, (size_t)crows * k * sizeof(double));
+    upload_async(dC, C, (size_t)crows * k * sizeof(double));
+    double *dlam = (double *)scratch(9, (size_t)(k + ntail) * sizeof(double)), *drn2 = dlam + k;
+    upload_async(dlam, lam, (size_t)k * sizeof(double));
+    UpdGroup g[2] = {};
+    void *in[2][3] = {{dX.p, dR.p, dP.p}, {dAX.p, dAR.p, dAP.p}};
+    void *tails[2] = {dPo.p, dAPo.p};
+    for (int q = 0; q < 2; q++) {
+        g[q].in[0] = in[q][0];
+        g[q].h[0] = k;
+        g[q].crow[0] = 0;
+        if (a) {
+            g[q].in[1] = in[q][1];
+            g[q].h[1] = a;
+            g[q].crow[1] = k;
+        }
+        if (p) {
+            g[q].in[2] = in[q][2];
+            g[q].h[2] = p;
+            g[q].crow[2] = k + a;
+        }
+        g[q].out_tail = tails[q];
+        g[q].out_full = in[q][0];  // in place, as the solver uses it
+    }
+    block_update_resid(prec, n, dX.ld, g, dC, crows, crows, k, nullptr, tail_pos, ntail, dlam,
+                       tdiag ? dT.p : nullptr, dRo.p, drn2);
+    dX.download(n, prec, Xo);
+    dAX.download(n, prec, AXo);
+    dPo.download(n, prec, Po);
+    dAPo.download(n, prec, APo);
+    dRo.download(n, prec, Ro);
+    download_sync(rn2, drn2, (size_t)ntail * sizeof(double));
+    return 0;
+}
+
 int blopex_amd_kernel_gram(int prec, long n, int lw, int qw, const double *L, const double *Q,
                            double *G) {
     DevBlock dL(n, lw, prec, L), dQ(n, qw, prec, Q);

@@ -20,6 +20,20 @@ int blopex_amd_kernel_update(int precision, long n, int k, int a, int p, const d
                              const double *R, const double *P, const double *C, double *full,
                              double *tail);
 
+/* The update above for the X group (X, R, P) and the AX group (AX, AR, AP)
+   together, with the residual of the new block formed from the registers
+   that hold X' and AX' (B = I, k <= 32): Xo = X', AXo = AX'; output column
+   c with tail_pos[c] = q >= 0 (q < ntail; -1: none) also gives Po(:, q) and
+   APo(:, q) (its P' and AP' columns), Ro(:, q) = t .* (X'(:, c) lam[c] -
+   AX'(:, c)) with t = tdiag (n entries) or 1 when tdiag is NULL, and
+   rn2[q] = ||X'(:, c) lam[c] - AX'(:, c)||^2.  Po, APo, Ro are n x ntail. */
+int blopex_amd_kernel_update_resid(int precision, long n, int k, int a, int p, const double *X,
+                                   const double *AX, const double *R, const double *AR,
+                                   const double *P, const double *AP, const double *C,
+                                   const int *tail_pos, int ntail, const double *lam,
+                                   const double *tdiag, double *Xo, double *AXo, double *Po,
+                                   double *APo, double *Ro, double *rn2);
+
 /* G (lw x qw) = L' * Q for L n x lw, Q n x qw. */
 int blopex_amd_kernel_gram(int precision, long n, int lw, int qw, const double *L, const double *Q,
                            double *G);

@@ -63,6 +63,58 @@ def test_block_update(prec, n, k, a, p):
 
 
 @pytest.mark.parametrize("prec", [L.HIPMV_FP64, L.HIPMV_FP32])
+@pytest.mark.parametrize("n,k,a,p,active,tdiag", [
+    (1000, 20, 5, 13, "some", True), (4096, 32, 32, 32, "all", True), (777, 8, 8, 0, "all", False),
+    (130, 32, 3, 3, "some", True), (64, 3, 2, 1, "all", False), (70000, 32, 16, 16, "some", True)])
+def test_block_update_with_residual(prec, n, k, a, p, active, tdiag):
+    """The fused update and residual against numpy: X', AX' (in place), P'
+    and AP' at the compact tail positions of the active columns, and
+    R = t .* (X' lam - AX') with its squared norms.  Row counts that are not
+    a multiple of 64 leave padding rows, where the diagonal t is not read."""
+    say(f"block update + residual prec={prec} n={n} k={k} a={a} p={p} {active} tdiag={tdiag}")
+    rng = np.random.default_rng(n + 3 * k + a)
+    X, AX, R, AR, P, AP = (f(rng.standard_normal((n, w))) for w in (k, k, a, a, p, p))
+    C = f(rng.standard_normal((k + a + p, k)))
+    lam = np.sort(rng.uniform(0.5, 2.0, k))
+    t = rng.uniform(0.5, 1.5, n) if tdiag else None
+    if prec == L.HIPMV_FP32:
+        X, AX, R, AR, P, AP = (f(v.astype(np.float32)) for v in (X, AX, R, AR, P, AP))
+        if t is not None:
+            t = t.astype(np.float32).astype(np.float64)
+    cols = list(range(k)) if active == "all" else [c for c in range(k) if c % 3 != 1]
+    tail_pos = np.full(k, -1, dtype=np.int32)
+    tail_pos[cols] = np.arange(len(cols), dtype=np.int32)
+    nt = len(cols)
+    Xo, AXo = (np.zeros((n, k), order="F") for _ in range(2))
+    Po, APo, Ro = (np.zeros((n, nt), order="F") for _ in range(3))
+    rn2 = np.zeros(nt)
+    rc = L.load().blopex_amd_kernel_update_resid(
+        prec, n, k, a, p, X.ctypes.data, AX.ctypes.data, R.ctypes.data, AR.ctypes.data,
+        P.ctypes.data, AP.ctypes.data, C.ctypes.data, tail_pos.ctypes.data, nt, lam.ctypes.data,
+        None if t is None else t.ctypes.data, Xo.ctypes.data, AXo.ctypes.data, Po.ctypes.data,
+        APo.ctypes.data, Ro.ctypes.data, rn2.ctypes.data)
+    assert rc == 0
+    tol = 1e-13 if prec == L.HIPMV_FP64 else 8e-6
+    for In, Rin, Pin, full, tail in ((X, R, P, Xo, Po), (AX, AR, AP, AXo, APo)):
+        ref_tail = Rin @ C[k:k + a] + Pin @ C[k + a:]
+        ref_full = ref_tail + In @ C[:k]
+        scale = (np.abs(In) @ np.abs(C[:k]) + np.abs(Rin) @ np.abs(C[k:k + a])
+                 + np.abs(Pin) @ np.abs(C[k + a:]))
+        assert np.max(np.abs(full - ref_full) / (scale + 1e-300)) < tol
+        if a + p:
+            st = scale[:, cols]
+            assert np.max(np.abs(tail - ref_tail[:, cols]) / (st + 1e-300)) < tol
+    # the residual from the kernel's own X' and AX' (exact in either precision)
+    v = Xo[:, cols] * lam[cols] - AXo[:, cols]
+    vs = np.abs(Xo[:, cols] * lam[cols]) + np.abs(AXo[:, cols])
+    rtol = 1e-15 if prec == L.HIPMV_FP64 else 2e-7
+    ref_r = v * (t[:, None] if t is not None else 1.0)
+    rs = vs * (t[:, None] if t is not None else 1.0)
+    assert np.max(np.abs(Ro - ref_r) / (rs + 1e-300)) < 4 * rtol
+    np.testing.assert_allclose(rn2, np.sum(v * v, axis=0), rtol=1e-12 if prec == L.HIPMV_FP64 else 1e-6)
+
+
+@pytest.mark.parametrize("prec", [L.HIPMV_FP64, L.HIPMV_FP32])
 @pytest.mark.parametrize("n,lw,qw", [(1000, 20, 13), (70000, 32, 32), (257, 64, 96), (64, 1, 1),
                                      (5000, 96, 64)])
 def test_gram(prec, n, lw, qw):
