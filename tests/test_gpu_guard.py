@@ -105,13 +105,19 @@ def test_eigenvalue_error_bound_holds(mode):
     assert err <= bound * 1.01 + 1e-15
 
 
-@pytest.mark.parametrize("mode", ["fp64", "mixed"])
-def test_lookahead_product_leaves_the_results_unchanged(mode):
-    say(f"look-ahead product, 16^3 lognormal, k=12, {mode}")
+@pytest.mark.parametrize("mode,case", [("fp64", "plain"), ("mixed", "plain"), ("fp64", "B"),
+                                       ("mixed", "B"), ("fp64", "refine")])
+def test_lookahead_product_leaves_the_results_unchanged(mode, case):
+    say(f"look-ahead product, 16^3 lognormal, k=12, {mode}, {case}")
     A = pb.laplacian_7pt(16, 16, 16, sigma=1.0, seed=3)
     X0 = po.drand48_block(A.n, 12, 1)
-    a = bl.lobpcg(X0, A, T="jacobi", tol=1e-7, maxit=800, mode=mode)
-    b = bl.lobpcg(X0, A, T="jacobi", tol=1e-7, maxit=800, mode=mode, profile=True)
+    extra = {}
+    if case == "B":  # B x = lambda ... with a diagonal B: A R and B R both enqueued ahead
+        extra["B"] = np.random.default_rng(4).uniform(0.5, 1.5, A.n)
+    if case == "refine":  # every column active again after convergence
+        extra["refine_iterations"] = 6
+    a = bl.lobpcg(X0, A, T="jacobi", tol=1e-7, maxit=800, mode=mode, **extra)
+    b = bl.lobpcg(X0, A, T="jacobi", tol=1e-7, maxit=800, mode=mode, profile=True, **extra)
     # the iteration at which each column first met the tolerance: the
     # active set shrank at several iterations, each one discarding the
     # product enqueued ahead
