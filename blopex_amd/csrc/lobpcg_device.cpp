@@ -4,7 +4,8 @@
 // keeps only what the reference keeps on the host anyway: the soft-locking
 // mask, the eigenvalue / residual histories and the exit bookkeeping; per
 // iteration it reads back k Ritz values, the active residual norms and a
-// status word.
+// status word (with the next iteration's A R already enqueued behind that
+// read, used when the active set is unchanged).
 //
 // Block layout: X, AX, BX and P, AP, BP are n x k column-major; the residual
 // blocks R, AR, BR hold only the active columns, compacted in ascending
@@ -15,9 +16,11 @@
 //
 // Precision: FP64 runs everything in FP64.  MIXED iterates with FP32 blocks
 // (FP64 Gram accumulation, FP64 small dense; AX and BX recomputed from X
-// every refresh_interval iterations) until the FP32 residuals stop improving
-// or fall below switch_factor * atol (or an FP32 Rayleigh-Ritz step breaks
-// down), then restarts from the last FP32 X in FP64 (re-orthonormalisation
+// every refresh_interval iterations) until every wanted residual is below
+// switch_factor times its threshold (|lambda| rtol + atol; with eig_rtol,
+// every eigenvalue bound within switch_factor^2 of its own), the FP32
+// residuals stop improving, or an FP32 Rayleigh-Ritz step breaks down; then
+// it restarts from the last FP32 X in FP64 (re-orthonormalisation
 // and Rayleigh-Ritz in FP64) and finishes there, so the final residuals and
 // Ritz values are FP64 ones.  refine_iterations adds FP64 iterations with
 // every column active after convergence.
