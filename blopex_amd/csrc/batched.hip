@@ -45,6 +45,7 @@ constexpr int kM = 3 * kK;               // largest pencil
 constexpr int kT = 256;                  // threads per workgroup
 constexpr int kChunk = 2048;             // rows per chunk
 constexpr int kRefresh = 20;             // FP32 iterations between recomputations of AX
+constexpr int kHostEvery = 4;            // graph-driven iterations between host reads
 constexpr int kSub = 32;                 // rows staged per Gram step
 constexpr int kPairs = kM * (kM + 1) / 2;
 constexpr int kPairsPerThread = (kPairs + kT - 1) / kT;
@@ -61,6 +62,7 @@ struct Probs {
     const long *off;
     const int *n;
     const int *k;
+    const int *run;  // 0: the problem is skipped (it left after the last list rebuild); or null
 };
 
 template <typename T>
@@ -128,6 +130,7 @@ __global__ void __launch_bounds__(kT) k_bresid(Chunks ch, Probs pr, long ld, con
     const int c = blockIdx.x;
     if (c >= *ch.count) return;
     const int p = ch.prob[c];
+    if (pr.run && !pr.run[p]) return;
     const int kp = pr.k[p];
     const long r0 = ch.r0[c];
     const int nr = ch.nr[c];
@@ -145,9 +148,10 @@ __global__ void __launch_bounds__(kT) k_bresid(Chunks ch, Probs pr, long ld, con
 
 // res2[p K + j] = sum of problem p's chunk partials (fixed order)
 __global__ void k_bres_sum(const int *act_prob, const int *nact, const int *pc_begin,
-                           const int *pc_count, const double *part, double *res2) {
+                           const int *pc_count, const int *run, const double *part, double *res2) {
     if ((int)blockIdx.x >= *nact) return;
     const int p = act_prob[blockIdx.x];
+    if (run && !run[p]) return;
     const int j = threadIdx.x;
     if (j >= kK) return;
     // chunk by chunk in order; eight chunks' loads in flight at a time
@@ -184,6 +188,7 @@ __global__ void __launch_bounds__(kT) k_bgram(Chunks ch, Probs pr, long ld, int 
     const int c = blockIdx.x;
     if (c >= *ch.count) return;
     const int p = ch.prob[c];
+    if (pr.run && !pr.run[p]) return;
     const int kp = pr.k[p];
     const int m = nb * kp;
     const int hm = (m + 1) / 2;
@@ -310,6 +315,7 @@ __global__ void __launch_bounds__(kT) k_bgram_mfma(Chunks ch, Probs pr, long ld,
     const int c = blockIdx.x;
     if (c >= *ch.count) return;
     const int p = ch.prob[c];
+    if (pr.run && !pr.run[p]) return;
     const int kp = pr.k[p];
     const int m = nb * kp;
     const int tm = (m + 15) / 16;
@@ -406,6 +412,7 @@ __global__ void __launch_bounds__(kT) k_bgram_sum(const int *act_prob, const int
                                                   double *GA, double *GB) {
     if ((int)blockIdx.x >= *nact) return;
     const int p = act_prob[blockIdx.x];
+    if (pr.run && !pr.run[p]) return;
     const int m = nb * pr.k[p];
     const int npairs = m * (m + 1) / 2;
     double *ga = GA + (size_t)p * kM * kM, *gb = GB + (size_t)p * kM * kM;
@@ -744,6 +751,7 @@ __global__ void __launch_bounds__(kT) k_bupd(Chunks ch, Probs pr, long ld, const
     const int c = blockIdx.x;
     if (c >= *ch.count) return;
     const int p = ch.prob[c];
+    if (pr.run && !pr.run[p]) return;
     const int kp = pr.k[p];
     const int nb = nbe ? nbe[p] : nb_all;
     const long r0 = ch.r0[c];
@@ -790,6 +798,7 @@ __global__ void __launch_bounds__(kT) k_bspmm(Chunks ch, Probs pr, long ld, cons
     const int c = blockIdx.x;
     if (c >= *ch.count) return;
     const int p = ch.prob[c];
+    if (pr.run && !pr.run[p]) return;
     const int kp = pr.k[p];
     const long r0 = ch.r0[c];
     const int nr = ch.nr[c];
@@ -834,6 +843,55 @@ struct Blocks {
     using type = T;
     T *X, *AX, *R, *AR, *P, *AP;
 };
+
+// End of an iteration of a graph-driven phase (one workgroup): problem p
+// leaves when all its wanted residuals are at most thr or its pencil broke
+// down (the host loop's test, on the same values), recording the iteration
+// and the outcome; when none, or fewer than min_active, are left the phase
+// stops and every iteration launched after that does nothing.  ctl[0]:
+// completed iterations of the phase, ctl[1]: stopped, ctl[2]: iteration of
+// the stop.
+__global__ void __launch_bounds__(1024) k_bcheck(int nprob, const int *kw, double thr,
+                                                 int min_active, const double *res2,
+                                                 const int *status, const DevStatus *dstat,
+                                                 int *active, int *run, int *leave_it,
+                                                 int *leave_conv, int *ctl) {
+    __shared__ int left_s;
+    const int tid = threadIdx.x;
+    if (tid == 0) left_s = 0;
+    __syncthreads();
+    if (ctl[1]) return;  // uniform
+    const int pit = ctl[0];
+    int left = 0;
+    for (int p = tid; p < nprob; p += blockDim.x) {
+        if (!active[p]) continue;
+        bool conv = true;
+        for (int j = 0; j < kw[p]; j++) conv = conv && sqrt(fabs(res2[(size_t)p * kK + j])) <= thr;
+        const bool broke = status[p] != 0 || dstat[p].abort != 0;
+        if (conv || broke) {
+            active[p] = 0;
+            run[p] = 0;
+            leave_it[p] = pit;
+            leave_conv[p] = conv ? 1 : 0;
+        } else {
+            left++;
+        }
+    }
+    if (left) atomicAdd(&left_s, left);
+    __syncthreads();
+    const int lt = left_s;
+    const bool stop = lt == 0 || lt < min_active;
+    if (stop)
+        for (int p = tid; p < nprob; p += blockDim.x) run[p] = 0;
+    if (tid == 0) {
+        if (stop) {
+            ctl[1] = 1;
+            ctl[2] = pit;
+        } else {
+            ctl[0] = pit + 1;
+        }
+    }
+}
 
 template <typename T>
 T *dev_alloc(size_t count) {
@@ -991,7 +1049,14 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
     BX_HIP(hipMemsetAsync(lam, 0, (size_t)nprob * kK * 8, stream()));
     BX_HIP(hipMemsetAsync(status, 0, nprob * 4, stream()));
 
-    Probs pr{d_off, d_n, d_k};
+    // device-side control of the graph-driven phases: active and run flags,
+    // the iteration at which each problem left and how, the wanted widths,
+    // and three control words (see k_bcheck)
+    int *dflags = dev_alloc<int>((size_t)5 * nprob + 4);
+    int *d_active = dflags, *d_run = dflags + nprob, *d_leave_it = dflags + 2 * nprob,
+        *d_leave_conv = dflags + 3 * nprob, *d_kw = dflags + 4 * nprob, *d_ctl = dflags + 5 * nprob;
+    upload_async(d_kw, kw.data(), nprob * 4);
+    Probs pr{d_off, d_n, d_k, d_run};
     Chunks ch{c_prob, c_r0, c_nr, d_counts};
     const int *d_nact = d_counts + 1;
     std::vector<char> active(nprob, 1);
@@ -1024,6 +1089,10 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
         upload_async(d_pcc, pcc.data(), nprob * 4);
         const int counts[2] = {nact_ch, nact};
         upload_async(d_counts, counts, sizeof counts);
+        // every problem in the lists runs
+        std::vector<int> run(nprob);
+        for (int p = 0; p < nprob; p++) run[p] = active[p] ? 1 : 0;
+        upload_async(d_run, run.data(), nprob * 4);
     };
     rebuild();
     const ColMap all = colmap_identity(kK);
@@ -1052,6 +1121,10 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
             hipLaunchKernelGGL(k_bgram, dim3(nch), dim3(kT), 0, stream(), ch, pr, N, nb,
                                bk.X, bk.R, bk.P, bk.AX, bk.AR, bk.AP, part_g);
     };
+    // threshold and smallest batch of the device check at the end of each
+    // step (0: no check; set by the graph-driven phase)
+    double check_thr = 0.0;
+    int check_min = 1;
     // grids cover every chunk / problem; workgroups past the device counts
     // exit at once, so one captured graph serves a whole phase
     auto step = [&](auto &bk, int nb, bool with_spmm) {
@@ -1068,7 +1141,7 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
                                GA, GB, lam, Cm, nbe, status);
         } else {
             RR2Args base{GA, GB, kM, 0, 0, 0, lam, lam, Cm, kM, rr_work, dstat, 1, 1};
-            rr_fast_batched(base, d_actp, d_nact, d_k, nprob, K, nb == 3, (long)kM * kM, kK,
+            rr_fast_batched(base, d_actp, d_nact, d_k, d_run, nprob, K, nb == 3, (long)kM * kM, kK,
                             (long)kM * kK, (long)rr_words);
         }
         hipLaunchKernelGGL(k_bupd<T>, dim3(nch), dim3(kT), 0, stream(), ch, pr, N, Cm,
@@ -1077,8 +1150,14 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
                            (const T *)bk.X, (const T *)bk.AX, lam, s.jacobi ? dv : nullptr, bk.R,
                            part_r);
         hipLaunchKernelGGL(k_bres_sum, dim3(nprob), dim3(64), 0, stream(), d_actp, d_nact, d_pcb,
-                           d_pcc, part_r, res2);
+                           d_pcc, d_run, part_r, res2);
         BX_HIP(hipGetLastError());
+        if (check_thr > 0.0) {
+            hipLaunchKernelGGL(k_bcheck, dim3(1), dim3(1024), 0, stream(), nprob, d_kw, check_thr,
+                               check_min, res2, status, dstat, d_active, d_run, d_leave_it,
+                               d_leave_conv, d_ctl);
+            BX_HIP(hipGetLastError());
+        }
     };
 
     std::vector<double> res_h((size_t)nprob * kK), lam_h((size_t)nprob * kK);
@@ -1086,14 +1165,86 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
     std::vector<DevStatus> ds_h(nprob);
     std::vector<char> rb_h(rb_res + rb_ds + rb_st);
     int captures = 0, replays = 0, it = 0, it32 = 0;
-    // One phase: from the Rayleigh-Ritz step in span(X) (AX current), iterate
-    // until every active problem's wanted residuals are below thr, its pencil
-    // broke down, or the phase has run it_cap iterations; done(p, conv) is
-    // called as each problem leaves.
-    auto phase = [&](auto &bk, double thr, int it_cap, int min_active, auto done) {
+    // The graph-driven phase: every step ends with k_bcheck, which takes the
+    // problems that converged or broke down out on the device (their
+    // workgroups exit from the next iteration on) and stops the phase when
+    // too few are left.  The host reads the outcome every kHostEvery
+    // iterations, calls done for the problems that left with the iteration
+    // the device recorded, and rebuilds the lists without them.  The tests
+    // are the host loop's below on the same values, so both give the same
+    // iterations and the same bits.
+    auto phase_graph = [&](auto &bk, double thr, int it_cap, int min_active, auto done) {
+        using T = typename std::remove_reference<decltype(bk)>::type::type;
+        {
+            std::vector<int> a(nprob);
+            for (int p = 0; p < nprob; p++) a[p] = active[p] ? 1 : 0;
+            upload_async(d_active, a.data(), nprob * 4);
+            const int ctl0[3] = {0, 0, 0};
+            upload_async(d_ctl, ctl0, sizeof ctl0);
+        }
+        check_thr = thr;
+        check_min = min_active;
         step(bk, 1, false);
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
+        int pit = 0;
+        std::vector<int> fl((size_t)5 * nprob + 4);
+        for (;;) {
+            download_sync(fl.data(), dflags, fl.size() * sizeof(int));
+            const int *a = fl.data(), *lit = a + 2 * nprob, *lcv = a + 3 * nprob,
+                      *ctl = a + 5 * nprob;
+            bool changed = false;
+            for (int p = 0; p < nprob; p++)
+                if (active[p] && !a[p]) {
+                    active[p] = 0;
+                    changed = true;
+                    done(p, lcv[p] != 0, lit[p]);
+                }
+            if (ctl[1]) {  // stopped: the iterations launched after the stop did nothing
+                it -= pit - ctl[2];
+                pit = ctl[2];
+                break;
+            }
+            if (pit >= it_cap) break;
+            if (changed) rebuild();
+            const int batch = std::min(kHostEvery, it_cap - pit);
+            for (int b = 0; b < batch; b++) {
+                pit++;
+                it++;
+                if (std::is_same<T, float>::value && pit % kRefresh == 0)
+                    hipLaunchKernelGGL(k_bspmm<T>, dim3(nch), dim3(kT), 0, stream(), ch, pr, N,
+                                       A->rowptr, A->colind, (const T *)A->val32,
+                                       (const T *)bk.X, bk.AX);
+                if (pit == 1) {
+                    step(bk, 2, true);
+                    continue;
+                }
+                if (!exec) {
+                    BX_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeRelaxed));
+                    step(bk, 3, true);
+                    BX_HIP(hipStreamEndCapture(stream(), &graph));
+                    BX_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+                    captures++;
+                } else {
+                    replays++;
+                }
+                BX_HIP(hipGraphLaunch(exec, stream()));
+            }
+        }
+        check_thr = 0.0;
+        if (exec) {
+            BX_HIP(hipGraphExecDestroy(exec));
+            BX_HIP(hipGraphDestroy(graph));
+        }
+        return pit;
+    };
+    // One phase: from the Rayleigh-Ritz step in span(X) (AX current), iterate
+    // until every active problem's wanted residuals are below thr, its pencil
+    // broke down, or the phase has run it_cap iterations; done(p, conv) is
+    // called as each problem leaves.  With graphs, phase_graph.
+    auto phase = [&](auto &bk, double thr, int it_cap, int min_active, auto done) {
+        if (s.graph) return phase_graph(bk, thr, it_cap, min_active, done);
+        step(bk, 1, false);
         int pit = 0;
         for (;;) {
             download_sync(rb_h.data(), rb, rb_h.size());
@@ -1129,24 +1280,7 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
                                    A->rowptr, A->colind, (const T *)A->val32, (const T *)bk.X,
                                    bk.AX);
             }
-            if (s.graph && nb == 3) {
-                if (!exec) {
-                    BX_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeRelaxed));
-                    step(bk, nb, true);
-                    BX_HIP(hipStreamEndCapture(stream(), &graph));
-                    BX_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-                    captures++;
-                } else {
-                    replays++;
-                }
-                BX_HIP(hipGraphLaunch(exec, stream()));
-            } else {
-                step(bk, nb, true);
-            }
-        }
-        if (exec) {
-            BX_HIP(hipGraphExecDestroy(exec));
-            BX_HIP(hipGraphDestroy(graph));
+            step(bk, nb, true);
         }
         return pit;
     };
@@ -1297,7 +1431,7 @@ int hip_LobpcgSolveBatched(int nprob, const hip_BatchProblem *probs, const hip_B
                     (void *)dv, (void *)d_off, (void *)d_sbase, (void *)d_n, (void *)d_k,
                     (void *)c_prob, (void *)c_nr, (void *)c_r0, (void *)d_actp, (void *)d_pcb,
                     (void *)d_pcc, (void *)lam, (void *)rb, (void *)GA, (void *)GB, (void *)Cm,
-                    (void *)nbe, (void *)part_r, (void *)part_g, (void *)rr_work})
+                    (void *)nbe, (void *)part_r, (void *)part_g, (void *)rr_work, (void *)dflags})
         BX_HIP(hipFree(b));
     hip_CSRMatrixDestroy(A);
     BX_HIP(hipFree(d_counts));

@@ -49,10 +49,11 @@ size_t rr_fast_work_words(int n);
 // The same solver over a batch: workgroup i takes problem act[i] (i < *nact)
 // with k = a = kvec[p] (and p = k when with_p), its buffers at p times the
 // strides (Gram buffers s_gram, Ritz values s_lam, coefficients s_coef, work
-// s_work, one DevStatus each); grid nprob.
+// s_work, one DevStatus each); grid nprob.  Problems with run[p] == 0 are
+// skipped (run may be null).
 void rr_fast_batched(const RR2Args &base, const int *act, const int *nact, const int *kvec,
-                     int nprob, int kmax, bool with_p, long s_gram, long s_lam, long s_coef,
-                     long s_work);
+                     const int *run, int nprob, int kmax, bool with_p, long s_gram, long s_lam,
+                     long s_coef, long s_work);
 
 // Rayleigh-Ritz on raw Gram blocks of [X R P], folding in the Cholesky
 // normalisation of R and P (see dense.hip); coef gets (k+a+p) x k coordinates

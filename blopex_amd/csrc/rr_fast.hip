@@ -1142,10 +1142,12 @@ __global__ void __launch_bounds__(kT) k_rr_tri_big(RR2Args r, double *gA, double
 // width kvec[p] (X and R, and P when with_p); its Gram buffers, Ritz values,
 // coefficients, work space and status sit at p times the given strides
 __global__ void __launch_bounds__(kT)
-    k_rr_tri_batched(RR2Args base, const int *act, const int *nact, const int *kvec, int with_p,
-                     long s_gram, long s_lam, long s_coef, long s_work) {
+    k_rr_tri_batched(RR2Args base, const int *act, const int *nact, const int *kvec,
+                     const int *run, int with_p, long s_gram, long s_lam, long s_coef,
+                     long s_work) {
     if ((int)blockIdx.x >= *nact) return;
     const int p = act[blockIdx.x];
+    if (run && !run[p]) return;
     RR2Args r = base;
     r.ga += (size_t)p * s_gram;
     r.gb += (size_t)p * s_gram;
@@ -1167,8 +1169,8 @@ size_t rr_fast_work_words(int n) {
 }
 
 void rr_fast_batched(const RR2Args &base, const int *act, const int *nact, const int *kvec,
-                     int nprob, int kmax, bool with_p, long s_gram, long s_lam, long s_coef,
-                     long s_work) {
+                     const int *run, int nprob, int kmax, bool with_p, long s_gram, long s_lam,
+                     long s_coef, long s_work) {
     const int n0 = (with_p ? 3 : 2) * kmax;
     BX_REQUIRE(rr_fast_fits(kmax, kmax, with_p ? kmax : 0), "rr_fast_batched: k %d too large",
                kmax);
@@ -1191,7 +1193,7 @@ void rr_fast_batched(const RR2Args &base, const int *act, const int *nact, const
         attr_set = true;
     }
     hipLaunchKernelGGL(k_rr_tri_batched, dim3(nprob), dim3(kT), lds, stream(), base, act, nact,
-                       kvec, (int)with_p, s_gram, s_lam, s_coef, s_work);
+                       kvec, run, (int)with_p, s_gram, s_lam, s_coef, s_work);
     BX_HIP(hipGetLastError());
 }
 
