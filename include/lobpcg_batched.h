@@ -37,7 +37,10 @@ typedef struct hip_BatchSettings {
     int jacobi;         /* 1: T = diag(A)^-1 (default 1) */
     int seed;           /* X0: drand48 after srand48(seed), problem by problem, column by
                            column (default 1) */
-    int graph;          /* 1: replay each iteration's launches from a hipGraph (default 1) */
+    int graph;          /* 1: replay each iteration's launches from a hipGraph, the
+                           convergence test on the device and read by the host every 4
+                           iterations (default 1); 0: launches and test from the host every
+                           iteration (the same results) */
     int guard;          /* extra columns iterated beside the k wanted, up to
                            HIP_BATCH_MAX_K + 8 columns in all and n/3; only the k wanted
                            are tested for convergence and returned (default 2) */
