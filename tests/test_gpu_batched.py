@@ -113,3 +113,27 @@ def test_batched_mixed_precision():
     f = bl.lobpcg_batched(problems, tol=1e-6, maxit=1000)
     for a, b in zip(r.lam, f.lam):
         assert np.max(np.abs(a - b) / b) < 1e-9
+
+
+def test_batched_mixed_graph_matches_eager():
+    """MIXED with graph replay (the convergence test and the end of the FP32
+    phase, once fewer than 5% of the problems are left in it, decided on the
+    device) against the eager path (both on the host): the same iterations,
+    FP32 iterations and bits."""
+    say("batched mixed: graph against eager (40 problems)")
+    problems = pb.batched_problems(40, smin=16, smax=40)
+    g = bl.lobpcg_batched(problems, tol=1e-6, maxit=1000, want_X=True, mode="mixed",
+                          autotune=False)
+    e = bl.lobpcg_batched(problems, tol=1e-6, maxit=1000, want_X=True, mode="mixed",
+                          autotune=False, graph=False)
+    say(f"  graph {g.stats['iterations']} it ({g.stats['iterations_fp32']} FP32, "
+        f"{g.stats['graph_replays']} replays); eager {e.stats['iterations']} it")
+    assert g.stats["graph_replays"] > 0 and e.stats["graph_replays"] == 0
+    assert g.stats["iterations"] == e.stats["iterations"]
+    assert g.stats["iterations_fp32"] == e.stats["iterations_fp32"]
+    np.testing.assert_array_equal(g.iterations, e.iterations)
+    np.testing.assert_array_equal(g.flags, e.flags)
+    for a, b in zip(g.lam, e.lam):
+        np.testing.assert_array_equal(a, b)
+    for a, b in zip(g.X, e.X):
+        np.testing.assert_array_equal(a, b)
