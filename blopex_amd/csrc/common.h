@@ -55,7 +55,8 @@ void upload_async(void *dst, const void *src, size_t bytes);
 void download_sync(void *dst, const void *src, size_t bytes);
 // The same copy in two halves: download_start enqueues it, download_finish
 // waits for the copy alone (work enqueued in between keeps running) and
-// copies the bytes out.  One such copy at a time; no download_sync between.
+// copies the bytes out.  One such copy at a time; download_sync may run in
+// between (it stages through another pinned buffer).
 void download_start(const void *src, size_t bytes);
 void download_finish(void *dst, size_t bytes);
 

@@ -142,11 +142,14 @@ void download_sync(void *dst, const void *src, size_t bytes) {
 
 namespace {
 hipEvent_t g_download_ev = nullptr;
+// a pinned slot of its own, so that work enqueued between download_start and
+// download_finish (a caller's operator among it) may use download_sync
+constexpr int kAheadSlot = kSlots - 2 - kStage;
 }  // namespace
 
 void download_start(const void *src, size_t bytes) {
     hipStream_t s = stream();
-    void *h = pinned(kSlots - 1 - kStage, bytes ? bytes : 1);
+    void *h = pinned(kAheadSlot, bytes ? bytes : 1);
     if (!g_download_ev) BX_HIP(hipEventCreateWithFlags(&g_download_ev, hipEventDisableTiming));
     if (bytes) BX_HIP(hipMemcpyAsync(h, src, bytes, hipMemcpyDeviceToHost, s));
     BX_HIP(hipEventRecord(g_download_ev, s));
@@ -154,7 +157,7 @@ void download_start(const void *src, size_t bytes) {
 
 void download_finish(void *dst, size_t bytes) {
     BX_HIP(hipEventSynchronize(g_download_ev));
-    if (bytes) memcpy(dst, pinned(kSlots - 1 - kStage, bytes), bytes);
+    if (bytes) memcpy(dst, pinned(kAheadSlot, bytes), bytes);
 }
 
 ColMap colmap_of(const hip_Multi_Vector *x) {

@@ -15,12 +15,15 @@ of the 128^3 workload itself.
 * The look-ahead product: the next iteration's A R is enqueued behind the
   read of the residual norms and used only if the active set is unchanged.
   With the class timers on (settings.profile), which turn it off, a solve in
-  which columns leave the active set one by one gives the same bits.
+  which columns leave the active set one by one gives the same bits; so does
+  a solve whose operator A reads device memory back to the host inside its
+  apply (it runs while the residual-norm read is in flight).
 * The 128^3 workload's generator at 20^3 (k = 32, Jacobi) against the
   reference's fixture, FP64 and MIXED; and the 128^3 workload itself with
   the bench's settings: residuals recomputed on the host from the returned
   X, X'X = I, MIXED eigenvalues against the FP64 ones.
 """
+import ctypes
 import sys
 
 import numpy as np
@@ -126,6 +129,51 @@ def test_lookahead_product_leaves_the_results_unchanged(mode, case):
         f"active-set changes at {sorted(set(first))}")
     assert a.flag == 0 and b.flag == 0
     assert len(set(first)) >= 3
+    assert a.iterations == b.iterations
+    np.testing.assert_array_equal(a.lam, b.lam)
+    np.testing.assert_array_equal(a.res, b.res)
+    np.testing.assert_array_equal(a.X, b.X)
+    np.testing.assert_array_equal(a.res_hist, b.res_hist)
+
+
+def test_lookahead_with_an_operator_that_reads_back():
+    """A caller's operator that copies device results to the host inside its
+    apply (here the MFMA layout probe after the CSR product) runs between the
+    start and the end of the look-ahead read of the residual norms: the solve
+    gives the same bits as with the library's CSR operator."""
+    say("look-ahead, operator with a device-to-host read, 16^3 lognormal, k=12, fp64")
+    lib = L.load()
+    A = pb.laplacian_7pt(16, 16, 16, sigma=1.0, seed=3)
+    X0 = po.drand48_block(A.n, 12, 1)
+    fn = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)
+    csr = fn(L.symbol_address("HipCSRMatMultiVec"))
+    rows, cols = np.zeros(256, np.int32), np.zeros(256, np.int32)
+    calls = [0]
+
+    def apply_and_read(a, x, y):
+        csr(a, x, y)
+        lib.blopex_amd_kernel_mfma_probe(L.HIPMV_FP64, rows.ctypes.data, cols.ctypes.data)
+        calls[0] += 1
+
+    cb = fn(apply_and_read)
+    out = []
+    for own in (False, True):
+        dA = bl.DeviceCSR(A)
+        dT = bl.DeviceDiag(jacobi_of=dA)
+        X = bl.DeviceBlock.from_numpy(X0)
+        try:
+            if own:
+                dA.op = ctypes.cast(cb, ctypes.c_void_p).value
+            r = bl.solve_device(dA, X, None, dT, tol=1e-7, maxit=800)
+            r.X = X.to_numpy()
+            out.append(r)
+        finally:
+            for o in (X, dT, dA):
+                o.close()
+    a, b = out
+    say(f"  flag {a.flag}/{b.flag} it {a.iterations}/{b.iterations} operator calls {calls[0]}")
+    assert a.flag == 0 and b.flag == 0
+    assert calls[0] >= b.iterations
     assert a.iterations == b.iterations
     np.testing.assert_array_equal(a.lam, b.lam)
     np.testing.assert_array_equal(a.res, b.res)
